@@ -1,0 +1,78 @@
+"""Builds the gfx950 library (motex_amd/lib/libmotex.so) and the motex command-line tool
+(motex_amd/bin/motex) in-tree with hipcc.  ``python -m motex_amd.build`` or
+``motex_amd.build.build()``.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "lib", "libmotex.so")
+BIN = os.path.join(HERE, "bin", "motex")
+ROOT = os.path.dirname(HERE)
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("MOTEX_OFFLOAD_ARCH", "gfx950")
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
+          "-Wall", "-Wno-unused-result"]
+# Packed FP32 instructions are kept out of every device code object.
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+LIB_SOURCES = ["hd_kernels.hip", "ed_kernels.hip", "struct_kernels.hip", "runtime.hip",
+               "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip"]
+CLI_SOURCES = ["cli/main.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/switches.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def _obj(src):
+    return os.path.join(OBJ, src.replace("/", "_") + ".o")
+
+
+def _compile_lib(src):
+    out = _obj(src)
+    _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", out])
+    return out
+
+
+def _compile_cli(src):
+    out = _obj(src)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+          f"-I{CSRC}", "-c", os.path.join(CSRC, src), "-o", out])
+    return out
+
+
+def build(verbose: bool = False) -> tuple[str, str]:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    os.makedirs(os.path.dirname(BIN), exist_ok=True)
+    lib_srcs = [s for s in LIB_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    cli_srcs = [s for s in CLI_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    workers = min(8, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(workers) as ex:
+        lib_objs = list(ex.map(_compile_lib, lib_srcs))
+        cli_objs = list(ex.map(_compile_cli, cli_srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *lib_objs, "-lpthread"])
+    os.replace(tmp, LIB)
+    if cli_objs:
+        _run(["g++", "-o", BIN, *cli_objs, f"-L{os.path.dirname(LIB)}", "-lmotex",
+              "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"])
+    if verbose:
+        print("built", LIB, BIN if cli_objs else "")
+    return LIB, BIN
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,275 @@
+// Hamming-distance FLASM kernels for gfx950.
+//
+// What is computed (the per-pair operator motifs_extraction_hd, functions.cc:526-559, over
+// libflasm::flasm_hd, libflasm.h:74): for x (length m) and text y (length n), every pair
+// (pos_x, pos_t) with pos_x >= k-1, pos_t >= k-1 and
+//     HD(x[pos_x-k+1 .. pos_x], y[pos_t-k+1 .. pos_t]) <= e
+// is an occurrence; v[pos_x] counts them and u[pos_x] counts the texts with at least one.
+//
+// Layout.  All sequences are concatenated into one text; for each symbol c a bit table
+// NE_c holds bit p = (text[p] != c).  A wave owns 64 lanes x 32 diagonals per block
+// (q = pos_t - pos_x) and sweeps the rows pos_x of x.  For its 32 diagonals a lane keeps
+// the mismatch count of the current k-window as P bit-sliced planes; one row step adds the
+// entering mismatch bits (NE_{x[px]} read at diagonal offset px) and subtracts the leaving
+// ones (NE_{x[px-k]} at offset px-k).  Both are funnel-shifted reads of two LDS words, so
+// no counter ever moves between lanes.  A bit-sliced "count <= e" gives 32 hit flags per
+// lane per row; hits are rare, so a wave-uniform slow path folds them into u and v
+// (distinct texts per row for u, a global flag word per factor for texts split across
+// work items).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "motex_internal.h"
+
+namespace motex {
+
+__global__ void ne_build_kernel(const uint8_t *__restrict__ codes, uint32_t total,
+                                uint32_t sigma, uint32_t W, uint32_t *__restrict__ ne)
+{
+    uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    int64_t p0 = (int64_t)w * 32 - (int64_t)kPadBits;
+    uint8_t cs[32];
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+        int64_t p = p0 + t;
+        cs[t] = (p >= 0 && p < (int64_t)total) ? codes[p] : 0xFF;
+    }
+    for (uint32_t c = 0; c < sigma; ++c) {
+        uint32_t bits = 0;
+#pragma unroll
+        for (int t = 0; t < 32; ++t) bits |= (uint32_t)(cs[t] != c) << t;
+        ne[(size_t)c * W + w] = bits;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n, uint32_t val)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = val;
+}
+
+// Index of the sequence containing text position pt among seq_start[j0 .. j0+ny-1].
+__device__ __forceinline__ uint32_t find_seq(const uint32_t *__restrict__ seq_start,
+                                             uint32_t j0, uint32_t ny, uint32_t pt)
+{
+    uint32_t lo = j0, hi = j0 + ny - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (seq_start[mid] <= pt) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// "count <= E" over P bit-sliced planes, least significant plane first.
+template <int P, int E>
+__device__ __forceinline__ uint32_t le_const(const uint32_t *c, uint32_t e)
+{
+    uint32_t res = ~0u;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        uint32_t eb;
+        if constexpr (E >= 0) eb = ((E >> p) & 1) ? ~0u : 0u;
+        else eb = ((e >> p) & 1) ? ~0u : 0u;
+        res = (~c[p] & (res | eb)) | (res & eb);
+    }
+    return res;
+}
+
+// Folds the hit flags of one row into u/v.  Wave-uniform: all lanes call it together.
+template <int B>
+__device__ __forceinline__ void fold_row(const HdArgs &a, const HdItem &I, uint32_t px,
+                                                   const uint32_t (&res)[B])
+{
+    const uint32_t lane = threadIdx.x;
+    uint32_t du = 0, dv = 0;
+    int last_y = -1;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        uint32_t h = res[b];
+        uint32_t cnt = 0, dist = 0;
+        int fy = -1, ly = -1;
+        while (h) {
+            uint32_t t = __builtin_ctz(h);
+            h &= h - 1;
+            int64_t pt = (int64_t)I.q0 + (int64_t)(kDiagPerBlock * b + 32 * lane + t) + px;
+            if (pt < (int64_t)I.ys || pt >= (int64_t)I.ye) continue;
+            uint32_t y = I.ny == 1 ? I.j0 : find_seq(a.seq_start, I.j0, I.ny, (uint32_t)pt);
+            if ((uint32_t)pt - a.seq_start[y] + 1 < a.k) continue;  // window starts before y
+            cnt++;
+            if ((int)y != ly) { dist++; if (fy < 0) fy = (int)y; ly = (int)y; }
+        }
+        uint64_t mask = __ballot(cnt != 0);
+        while (mask) {
+            uint32_t L = (uint32_t)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            uint32_t lc = __builtin_amdgcn_readlane(cnt, L);
+            uint32_t ld = __builtin_amdgcn_readlane(dist, L);
+            int lf = __builtin_amdgcn_readlane(fy, L);
+            int ll = __builtin_amdgcn_readlane(ly, L);
+            dv += lc;
+            du += ld - (lf == last_y ? 1u : 0u);
+            last_y = ll;
+        }
+    }
+    if (dv && lane == 0) {
+        if (I.split != kNoSplit) {
+            uint32_t xf = I.xf_base + px;
+            uint32_t bit = 1u << (xf & 31);
+            uint32_t old = atomicOr(a.split_flags + (size_t)I.split * a.split_words + (xf >> 5), bit);
+            du = (old & bit) ? 0u : 1u;
+        }
+        if (du) atomicAdd(a.u + I.uv_base + px, du);
+        atomicAdd(a.v + I.uv_base + px, dv);
+    }
+}
+
+// Persistent batched kernel: one wave per workgroup pulls items from the queue head.
+template <int B, int P, int E>
+__global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
+{
+    extern __shared__ uint32_t smem[];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t wl = a.wl;
+    uint32_t *ne_l = smem;
+    uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + (a.sigma + 1) * wl);
+    const uint32_t zero_row = a.sigma * wl;
+    for (uint32_t w = lane; w < wl; w += 64) ne_l[zero_row + w] = 0;
+
+    for (uint32_t round = 0;; ++round) {
+        uint32_t it;
+        if (a.persistent) {
+            it = 0;
+            if (lane == 0) it = atomicAdd(a.queue, 1u);
+            it = __builtin_amdgcn_readfirstlane(it);
+        } else {
+            it = round == 0 ? blockIdx.x : a.nitems;
+        }
+        if (it >= a.nitems) break;
+        const HdItem I = a.items[it];
+        const uint32_t nrows = I.r1 - I.px_start;
+        const uint32_t wbase = I.wbase;
+        __syncthreads();  // previous item finished reading LDS
+        for (uint32_t c = 0; c < a.sigma; ++c) {
+            const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
+            for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
+        }
+        for (uint32_t r = lane; r < nrows; r += 64)
+            xoff[r] = (uint16_t)(a.codes[I.x_start + I.px_start + r] * wl);
+        __syncthreads();
+
+        uint32_t cnt[B][P];
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int p = 0; p < P; ++p) cnt[b][p] = 0;
+
+        const uint32_t rep0 = I.r0 > a.k - 1 ? I.r0 : a.k - 1;
+        const uint32_t pxs5 = I.px_start >> 5;
+        for (uint32_t r = 0; r < nrows; ++r) {
+            const uint32_t px = I.px_start + r;
+            const uint32_t wn = xoff[r] + (px >> 5) - pxs5 + lane;
+            uint32_t wo, sho;
+            if (r >= a.k) {
+                wo = xoff[r - a.k] + ((px - a.k) >> 5) - pxs5 + lane;
+                sho = (px - a.k) & 31;
+            } else {
+                wo = zero_row + lane;
+                sho = 0;
+            }
+            const uint32_t shn = px & 31;
+            uint32_t res[B];
+            uint32_t any = 0;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 64 * b + 1], ne_l[wn + 64 * b], shn);
+                uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 64 * b + 1], ne_l[wo + 64 * b], sho);
+                uint32_t inc = an & ~bo, dec = bo & ~an;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    uint32_t cp = cnt[b][p];
+                    cnt[b][p] = cp ^ (inc | dec);
+                    uint32_t ni = cp & inc, nd = ~cp & dec;
+                    inc = ni;
+                    dec = nd;
+                }
+                res[b] = le_const<P, E>(cnt[b], a.e);
+                any |= res[b];
+            }
+            if (px >= rep0 && __ballot(any != 0)) fold_row<B>(a, I, px, res);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Host-side launchers
+// ------------------------------------------------------------------------------------
+
+template <int B, int P, int E>
+static hipError_t launch_bp(const HdArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL((hd_batched_kernel<B, P, E>), dim3(grid), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int B, int P>
+static hipError_t launch_e(const HdArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+{
+    switch (a.e) {
+    case 0: return launch_bp<B, P, 0>(a, grid, lds, s);
+    case 1: return launch_bp<B, P, 1>(a, grid, lds, s);
+    case 2: return launch_bp<B, P, 2>(a, grid, lds, s);
+    case 3: return launch_bp<B, P, 3>(a, grid, lds, s);
+    default: return launch_bp<B, P, -1>(a, grid, lds, s);
+    }
+}
+
+template <int B>
+static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+{
+    uint32_t planes = 32 - __builtin_clz(a.k);  // bits to hold 0..k
+    if (planes <= 3) return launch_e<B, 3>(a, grid, lds, s);
+    if (planes == 4) return launch_e<B, 4>(a, grid, lds, s);
+    if (planes == 5) return launch_e<B, 5>(a, grid, lds, s);
+    if (planes == 6) return launch_e<B, 6>(a, grid, lds, s);
+    return launch_e<B, 7>(a, grid, lds, s);
+}
+
+size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max)
+{
+    return (size_t)(sigma + 1) * wl * 4 + (size_t)((rows_max + 1) & ~1u) * 2;
+}
+
+uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)
+{
+    return 64 * blocks + (rows_max >> 5) + 3;
+}
+
+hipError_t launch_ne_build(const uint8_t *codes, uint32_t total, uint32_t sigma, uint32_t W,
+                           uint32_t *ne, hipStream_t s)
+{
+    hipLaunchKernelGGL(ne_build_kernel, dim3((W + 255) / 256), dim3(256), 0, s, codes, total,
+                       sigma, W, ne);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t val, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, val);
+    return hipGetLastError();
+}
+
+hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_t s)
+{
+    size_t lds = hd_lds_bytes(a.sigma, a.wl, a.rows_max);
+    switch (blocks) {
+    case 1: return launch_p<1>(a, grid, lds, s);
+    case 2: return launch_p<2>(a, grid, lds, s);
+    default: return launch_p<4>(a, grid, lds, s);
+    }
+}
+
+}  // namespace motex

@@ -1,0 +1,73 @@
+// Internal structures shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+namespace motex {
+
+// Bit offset of text position 0 inside the per-symbol mismatch tables ("NE" tables).
+// Positions before 0 and after the end are padding whose bits are never reported.
+constexpr uint32_t kPadBits = 4096 + 2048;
+constexpr uint32_t kDiagPerBlock = 2048;     // 64 lanes x 32 diagonals
+constexpr uint32_t kMaxHdK = 64;             // Hamming kernels: counters of up to 7 planes
+constexpr uint32_t kMaxEdK = 32;             // edit kernels: bit-vectors stay below 64 bits
+constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
+
+// One unit of the Hamming work queue: rows [r0, r1) of sequence x against the text
+// range [ys, ye) (whole sequences j0 .. j0+ny-1, or one part of a long sequence j0).
+struct HdItem {
+    uint32_t x_start;   // text position of x's first symbol
+    uint32_t uv_base;   // u/v index of x's pos_x = 0
+    uint32_t r0, r1;    // reported rows (positions in x)
+    uint32_t px_start;  // first computed row = max(0, r0 - k + 1)
+    int32_t q0;         // first diagonal (pt - px), a multiple of 32
+    uint32_t ys, ye;    // text range whose windows are reported by this item
+    uint32_t j0, ny;    // sequence range
+    uint32_t split;     // kNoSplit, or the split slot of sequence j0
+    uint32_t xf_base;   // global factor index of (x, pos_x = 0): row flags for split texts
+    uint32_t wbase;     // first NE word the item reads: floor((q0 + kPadBits)/32) + px_start/32
+    uint32_t pad_;
+};
+
+struct HdArgs {
+    const uint32_t *ne;        // sigma rows x W words
+    const uint8_t *codes;      // symbol codes of the concatenated text
+    const uint32_t *seq_start; // num_seqs + 1 entries
+    const HdItem *items;
+    uint32_t *u;
+    uint32_t *v;
+    uint32_t *split_flags;     // split slots x split_words
+    uint32_t *queue;           // work-queue head (one counter)
+    uint32_t W;                // words per NE row
+    uint32_t nitems;
+    uint32_t sigma;
+    uint32_t k, e;
+    uint32_t wl;               // LDS words per symbol row
+    uint32_t split_words;
+    uint32_t rows_max;         // largest r1 - px_start of any item
+    uint32_t persistent;       // 1: pull items from *queue; 0: item = blockIdx.x
+};
+
+// One unit of the edit-distance work queue: x against texts j0 .. j0+ny-1.
+struct EdItemDev {
+    uint32_t x_start;   // text position of x
+    uint32_t m;         // length of x
+    uint32_t uv_base;   // u/v index of x's pos_x = 0
+    uint32_t j0, ny;    // texts j0 .. j0+ny-1
+    uint32_t pad;
+};
+
+struct EdArgs {
+    const uint8_t *codes;
+    const uint32_t *seq_start;
+    const EdItemDev *items;
+    uint32_t *u, *v;
+    uint32_t *queue;
+    uint32_t *scratch;        // per workgroup: two boundary streams of stream_words words
+    uint32_t stream_words;
+    uint32_t nitems;
+    uint32_t k, e;
+    uint32_t lds_cols;        // columns held in the LDS accumulators
+    uint32_t persistent;      // 1: pull items from *queue; 0: item = blockIdx.x
+};
+
+}  // namespace motex

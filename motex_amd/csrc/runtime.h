@@ -1,0 +1,163 @@
+// Host-side runtime structures (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "../../include/motex.h"
+#include "motex_internal.h"
+
+namespace motex {
+
+int set_error(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+int validate_params(const motex_params *p);
+
+// Device allocation owned by a batch or job.
+struct DevBuf {
+    void *p = nullptr;
+    uint64_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    hipError_t alloc(uint64_t n)
+    {
+        release();
+        hipError_t e = hipMalloc(&p, n ? n : 4);
+        if (e != hipSuccess) { p = nullptr; return e; }
+        bytes = n;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+// Concatenated symbol codes of a sequence set.
+struct SeqSet {
+    uint32_t num = 0;
+    uint64_t total = 0;
+    uint32_t sigma = 0;
+    std::vector<uint32_t> len, start;   // start has num+1 entries
+    std::vector<uint8_t> codes;
+    std::vector<uint64_t> uv_off;       // num+1 entries, nb_structs * len prefix sums
+    int build(const char *const *seqs, const uint32_t *lens, uint32_t n, uint32_t nb_structs);
+};
+
+// Tile shape of the batched kernels (picked by the autotuner per alphabet/d/k/e).
+struct TileShape {
+    uint32_t blocks = 4;         // Hamming: 2048-diagonal blocks per wave
+    uint32_t rows = 512;         // Hamming: x rows per work item
+    uint32_t waves_per_cu = 16;  // persistent grid = num_cu * waves_per_cu
+    uint32_t variant = 0;        // edit kernels: 0 = 32-bit keys, 1 = 64-bit words
+    uint32_t ytile = 0;          // edit kernels: text symbols per work item
+};
+
+struct HdPlan {
+    std::vector<HdItem> items;
+    uint32_t blocks = 4, rows = 512, rows_max = 0, wl = 0, W = 0, split_slots = 0;
+};
+
+struct EdItem {
+    uint32_t x;          // sequence index of x
+    uint32_t j0, ny;     // texts j0 .. j0+ny-1
+};
+
+struct EdPlan {
+    std::vector<EdItem> items;
+    uint32_t strips_max = 0;
+};
+
+struct Job;
+
+}  // namespace motex
+
+struct motex_ctx {
+    int device = 0;
+    int num_cu = 0;
+    std::string arch;
+    size_t budget = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    std::mutex tune_mu;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, motex::TileShape> tune_cache;
+    // serving
+    std::mutex mu;
+    std::condition_variable cv;
+    std::thread worker;
+    bool stop = false;
+    bool worker_started = false;
+    uint64_t next_id = 1;
+    size_t resident_bytes = 0;
+    std::map<uint64_t, std::shared_ptr<motex::Job>> jobs;
+    std::deque<std::shared_ptr<motex::Job>> pending;
+};
+
+struct motex_batch {
+    motex_ctx *ctx = nullptr;
+    motex_params params{};
+    std::vector<uint32_t> gmin, gmax, blen, berr, S;
+    uint32_t nb_structs = 1;
+    motex::SeqSet ss;
+    motex::TileShape tile;
+    motex::HdPlan hd;
+    motex::EdPlan ed;
+    uint32_t split_words = 0;
+    uint32_t ed_nitems = 0;
+    motex_stats stats{};
+    hipStream_t stream = nullptr;   // overrides ctx->stream when set (serving)
+    motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2;
+    uint64_t work_items() const { return hd.items.size() + ed_nitems; }
+};
+
+namespace motex {
+
+std::vector<uint32_t> enumerate_structs(const motex_params *p, uint32_t *nb_structs);
+uint32_t total_motif_length(const motex_params *p);
+void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
+                   const std::vector<uint32_t> &xs, const std::vector<int32_t> *only_y,
+                   HdPlan &plan);
+TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss);
+
+// kernels (hd_kernels.hip, ed_kernels.hip)
+hipError_t launch_ne_build(const uint8_t *codes, uint32_t total, uint32_t sigma, uint32_t W,
+                           uint32_t *ne, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t val, hipStream_t s);
+hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, size_t lds, hipStream_t s);
+uint32_t ed_strip_cols(bool wide);
+hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_t s);
+size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max);
+uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max);
+
+// per-mode entry points (runtime.hip, ed_runtime.hip, struct_runtime.hip)
+int batch_upload_common(motex_batch *b);
+int batch_clear(motex_batch *b);
+int hd_prepare(motex_batch *b);
+int hd_run(motex_batch *b);
+int hd_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
+int ed_prepare(motex_batch *b);
+int ed_run(motex_batch *b);
+int ed_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
+int struct_prepare(motex_batch *b);
+int struct_run(motex_batch *b);
+int struct_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
+int pair_run(motex_batch *b, uint32_t i, uint32_t j);
+int ed_pair_run(motex_batch *b, uint32_t i, uint32_t j);
+int struct_pair_run(motex_batch *b, uint32_t i, uint32_t j);
+HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t persistent);
+void serving_shutdown(motex_ctx *c);
+
+}  // namespace motex

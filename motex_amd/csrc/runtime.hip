@@ -1,0 +1,623 @@
+// Host runtime behind include/motex.h: contexts, device-resident batches, work-queue
+// planning, the plain per-pair form and the C ABI.
+//
+// The all-pairs loop of motex.cc:494-583 becomes a plan of work items (x row tile x text
+// range) executed by one persistent kernel; the reference's per-pair u/v fold
+// (functions.cc:542-555) happens on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/motex.h"
+#include "motex_internal.h"
+#include "runtime.h"
+
+namespace motex {
+
+// ---------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return set_error(MOTEX_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------------------
+// parameter validation (the extraction-relevant checks of motex.cc:98-201)
+// ---------------------------------------------------------------------------------------
+int validate_params(const motex_params *p)
+{
+    if (!p) return set_error(MOTEX_EINVAL, "params is NULL");
+    if (p->k == 0) return set_error(MOTEX_EINVAL, "motif length k must be positive");
+    if (p->e >= p->k)
+        return set_error(MOTEX_EINVAL,
+                         "error threshold e must be less than the fixed-length of motifs k");
+    if (p->distance > 1)
+        return set_error(MOTEX_EINVAL,
+                         "distance d should be 0 (Hamming) or 1 (edit distance)");
+    if (p->alphabet > MOTEX_ALPHA_USR) return set_error(MOTEX_EINVAL, "unknown alphabet");
+    if (p->nb_gaps && (!p->gap_min || !p->gap_max || !p->box_len || !p->box_err))
+        return set_error(MOTEX_EINVAL, "structure arrays missing");
+    for (uint32_t g = 0; g < p->nb_gaps; ++g) {
+        if (p->gap_max[g] < p->gap_min[g])
+            return set_error(MOTEX_EINVAL, "gap %u: max below min", g + 1);
+        if (p->box_len[g] == 0) return set_error(MOTEX_EINVAL, "box %u has length 0", g + 1);
+    }
+    uint32_t kmax = p->distance == 0 ? kMaxHdK : kMaxEdK;
+    uint32_t longest = p->k;
+    for (uint32_t g = 0; g < p->nb_gaps; ++g) longest = std::max(longest, p->box_len[g]);
+    if (longest > kmax)
+        return set_error(MOTEX_EUNSUPPORTED, "box length %u exceeds the %s limit of %u", longest,
+                         p->distance == 0 ? "Hamming" : "edit-distance", kmax);
+    return MOTEX_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// sequence set: symbol codes of the concatenated text
+// ---------------------------------------------------------------------------------------
+int SeqSet::build(const char *const *seqs, const uint32_t *lens, uint32_t n, uint32_t nb_structs)
+{
+    num = n;
+    len.assign(lens, lens + n);
+    start.resize(n + 1);
+    uint64_t tot = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        start[i] = (uint32_t)tot;
+        tot += lens[i];
+        if (lens[i] && !seqs[i]) return set_error(MOTEX_EINVAL, "sequence %u is NULL", i);
+    }
+    if (tot + (1u << 20) >= (1ull << 31))
+        return set_error(MOTEX_EUNSUPPORTED, "total sequence length %llu too large",
+                         (unsigned long long)tot);
+    start[n] = (uint32_t)tot;
+    total = tot;
+    int map[256];
+    for (int c = 0; c < 256; ++c) map[c] = -1;
+    sigma = 0;
+    codes.resize(tot);
+    for (uint32_t i = 0; i < n; ++i) {
+        const unsigned char *s = (const unsigned char *)seqs[i];
+        uint8_t *dst = codes.data() + start[i];
+        for (uint32_t p = 0; p < lens[i]; ++p) {
+            int c = map[s[p]];
+            if (c < 0) { c = map[s[p]] = (int)sigma++; }
+            dst[p] = (uint8_t)c;
+        }
+    }
+    if (sigma > 200) return set_error(MOTEX_EUNSUPPORTED, "more than 200 distinct symbols");
+    if (sigma == 0) sigma = 1;
+    uv_off.resize(n + 1);
+    uint64_t o = 0;
+    for (uint32_t i = 0; i < n; ++i) { uv_off[i] = o; o += (uint64_t)nb_structs * lens[i]; }
+    uv_off[n] = o;
+    return MOTEX_OK;
+}
+
+// Gap combinations in the order of functions.cc:710-729 (last gap varies fastest).
+std::vector<uint32_t> enumerate_structs(const motex_params *p, uint32_t *nb_structs)
+{
+    uint32_t tot = 1;
+    for (uint32_t g = 0; g < p->nb_gaps; ++g) tot *= p->gap_max[g] - p->gap_min[g] + 1;
+    std::vector<uint32_t> S((size_t)tot * p->nb_gaps);
+    for (uint32_t s = 0; s < tot; ++s) {
+        uint32_t rem = s;
+        for (int g = (int)p->nb_gaps - 1; g >= 0; --g) {
+            uint32_t span = p->gap_max[g] - p->gap_min[g] + 1;
+            S[(size_t)s * p->nb_gaps + g] = p->gap_min[g] + rem % span;
+            rem /= span;
+        }
+    }
+    *nb_structs = tot;
+    return S;
+}
+
+uint32_t total_motif_length(const motex_params *p)
+{
+    uint32_t t = p->k;
+    for (uint32_t g = 0; g < p->nb_gaps; ++g) t += p->gap_min[g] + p->box_len[g];
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------
+// Hamming work-queue planning
+// ---------------------------------------------------------------------------------------
+static int64_t floor32(int64_t v) { return v >= 0 ? (v & ~31LL) : -(((-v) + 31) & ~31LL); }
+
+void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
+                   const std::vector<uint32_t> &xs, const std::vector<int32_t> *only_y,
+                   HdPlan &plan)
+{
+    const int64_t cap = (int64_t)kDiagPerBlock * blocks - rows - 32;
+    plan.items.clear();
+    plan.blocks = blocks;
+    plan.rows = rows;
+    plan.rows_max = rows + k - 1;
+    // text ranges: whole sequences packed up to cap positions, long ones split
+    struct Range { uint32_t ys, ye, j0, ny, split; };
+    std::vector<Range> ranges;
+    auto add_seq_ranges = [&](uint32_t j) {
+        uint32_t n = ss.len[j];
+        if (n < k) return;
+        if ((int64_t)n <= cap) { ranges.push_back({ss.start[j], ss.start[j] + n, j, 1, kNoSplit}); return; }
+        uint32_t slot = plan.split_slots++;
+        for (uint32_t p = 0; p < n; p += (uint32_t)cap) {
+            uint32_t e = std::min<uint32_t>(n, p + (uint32_t)cap);
+            ranges.push_back({ss.start[j] + p, ss.start[j] + e, j, 1, slot});
+        }
+    };
+    plan.split_slots = 0;
+    if (only_y) {
+        for (int32_t j : *only_y) add_seq_ranges((uint32_t)j);
+    } else {
+        // pack consecutive eligible whole sequences
+        uint32_t j = 0;
+        while (j < ss.num) {
+            if (ss.len[j] < k) { ++j; continue; }
+            if ((int64_t)ss.len[j] > cap) { add_seq_ranges(j); ++j; continue; }
+            Range r{ss.start[j], ss.start[j] + ss.len[j], j, 1, kNoSplit};
+            uint32_t jj = j + 1;
+            while (jj < ss.num && (int64_t)ss.len[jj] <= cap &&
+                   (int64_t)(ss.start[jj] + ss.len[jj]) - r.ys <= cap) {
+                if (ss.len[jj] >= k) { r.ye = ss.start[jj] + ss.len[jj]; r.ny = jj - j + 1; }
+                ++jj;
+            }
+            // trailing ineligible sequences are not part of the range
+            ranges.push_back(r);
+            j = r.j0 + r.ny;
+            while (j < jj && ss.len[j] < k) ++j;
+            if (j < r.j0 + r.ny) j = r.j0 + r.ny;
+        }
+    }
+    for (uint32_t i : xs) {
+        uint32_t m = ss.len[i];
+        if (m < k) continue;
+        for (uint32_t r0 = k - 1; r0 < m; r0 += rows) {
+            uint32_t r1 = std::min(m, r0 + rows);
+            uint32_t pxs = r0 - (k - 1);
+            for (const Range &R : ranges) {
+                HdItem it{};
+                it.x_start = ss.start[i];
+                it.uv_base = (uint32_t)ss.uv_off[i];
+                it.r0 = r0; it.r1 = r1; it.px_start = pxs;
+                int64_t q0 = floor32((int64_t)R.ys - (int64_t)r1 + 1);
+                it.q0 = (int32_t)q0;
+                it.ys = R.ys; it.ye = R.ye; it.j0 = R.j0; it.ny = R.ny; it.split = R.split;
+                it.xf_base = (uint32_t)ss.uv_off[i];
+                int64_t wb = (q0 + (int64_t)kPadBits) / 32 + (pxs >> 5);  // q0 + kPadBits is a multiple of 32
+                it.wbase = wb < 0 ? 0x80000000u : (uint32_t)wb;
+                plan.items.push_back(it);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------------------
+}  // namespace motex
+
+using namespace motex;
+
+extern "C" const char *motex_last_error(void) { return g_err.c_str(); }
+
+extern "C" int motex_ctx_create(int device, size_t budget, motex_ctx **out)
+{
+    if (!out) return set_error(MOTEX_EINVAL, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return set_error(MOTEX_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return set_error(MOTEX_EINVAL, "device %d out of range", device);
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    std::unique_ptr<motex_ctx> c(new motex_ctx());
+    c->device = device;
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    c->num_cu = prop.multiProcessorCount;
+    c->arch = prop.gcnArchName;
+    size_t freeb = 0, totb = 0;
+    if ((e = hipMemGetInfo(&freeb, &totb)) != hipSuccess) return hip_fail(e, "hipMemGetInfo");
+    c->budget = budget ? budget : (size_t)(freeb * 0.9);
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "hipStreamCreate");
+    if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&c->ev2)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    *out = c.release();
+    return MOTEX_OK;
+}
+
+extern "C" void motex_ctx_destroy(motex_ctx *c)
+{
+    if (!c) return;
+    serving_shutdown(c);
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->ev2) hipEventDestroy(c->ev2);
+    delete c;
+}
+
+// ---------------------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------------------
+namespace motex {
+
+int batch_upload_common(motex_batch *b)
+{
+    hipError_t e;
+    const SeqSet &ss = b->ss;
+    if ((e = b->d_codes.alloc(std::max<uint64_t>(ss.total, 1) + 64)) != hipSuccess) return hip_fail(e, "alloc codes");
+    if (ss.total && (e = hipMemcpy(b->d_codes.p, ss.codes.data(), ss.total, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy codes");
+    if ((e = b->d_seq_start.alloc((ss.num + 1) * 4)) != hipSuccess) return hip_fail(e, "alloc starts");
+    if ((e = hipMemcpy(b->d_seq_start.p, ss.start.data(), (ss.num + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy starts");
+    uint64_t uvn = std::max<uint64_t>(ss.uv_off[ss.num], 1);
+    if ((e = b->d_u.alloc(uvn * 4)) != hipSuccess) return hip_fail(e, "alloc u");
+    if ((e = b->d_v.alloc(uvn * 4)) != hipSuccess) return hip_fail(e, "alloc v");
+    if ((e = b->d_queue.alloc(256)) != hipSuccess) return hip_fail(e, "alloc queue");
+    return MOTEX_OK;
+}
+
+int hd_prepare(motex_batch *b)
+{
+    motex_ctx *c = b->ctx;
+    const SeqSet &ss = b->ss;
+    TileShape ts = choose_tile(c, b->params, ss);
+    b->tile = ts;
+    std::vector<uint32_t> xs(ss.num);
+    for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
+    plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd);
+    // longest items first: they set the tail of the persistent schedule
+    std::stable_sort(b->hd.items.begin(), b->hd.items.end(), [](const HdItem &a, const HdItem &z) {
+        uint64_t ca = (uint64_t)(a.r1 - a.px_start) * (a.ye - a.ys + a.r1 - a.r0);
+        uint64_t cz = (uint64_t)(z.r1 - z.px_start) * (z.ye - z.ys + z.r1 - z.r0);
+        return ca > cz;
+    });
+    b->hd.wl = hd_lds_words(ts.blocks, b->hd.rows_max);
+    if (hd_lds_bytes(ss.sigma, b->hd.wl, b->hd.rows_max) > 65536 || (uint64_t)ss.sigma * b->hd.wl >= 65536)
+        return set_error(MOTEX_EUNSUPPORTED, "alphabet of %u symbols does not fit the kernel's LDS window", ss.sigma);
+    for (const HdItem &it : b->hd.items)
+        if ((int32_t)it.wbase < 0) return set_error(MOTEX_EINVAL, "internal: negative table window");
+    // NE words: cover every item's LDS window
+    uint64_t W = (ss.total + kPadBits) / 32 + 2;
+    for (const HdItem &it : b->hd.items) W = std::max<uint64_t>(W, (uint64_t)it.wbase + b->hd.wl + 1);
+    b->hd.W = (uint32_t)W;
+    hipError_t e;
+    if ((e = b->d_ne.alloc((uint64_t)W * ss.sigma * 4)) != hipSuccess) return hip_fail(e, "alloc NE tables");
+    if ((e = launch_ne_build((const uint8_t *)b->d_codes.p, (uint32_t)ss.total, ss.sigma, (uint32_t)W,
+                             (uint32_t *)b->d_ne.p, c->stream)) != hipSuccess)
+        return hip_fail(e, "NE build");
+    uint64_t ni = std::max<size_t>(b->hd.items.size(), 1);
+    if ((e = b->d_items.alloc(ni * sizeof(HdItem))) != hipSuccess) return hip_fail(e, "alloc items");
+    if (!b->hd.items.empty() &&
+        (e = hipMemcpy(b->d_items.p, b->hd.items.data(), b->hd.items.size() * sizeof(HdItem), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy items");
+    b->split_words = (uint32_t)((ss.uv_off[ss.num] + 31) / 32 + 1);
+    uint64_t sf = (uint64_t)std::max<uint32_t>(b->hd.split_slots, 1) * b->split_words;
+    if ((e = b->d_split.alloc(sf * 4)) != hipSuccess) return hip_fail(e, "alloc split flags");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "NE build sync");
+    return MOTEX_OK;
+}
+
+HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t persistent)
+{
+    HdArgs a{};
+    a.ne = (const uint32_t *)b->d_ne.p;
+    a.codes = (const uint8_t *)b->d_codes.p;
+    a.seq_start = (const uint32_t *)b->d_seq_start.p;
+    a.items = (const HdItem *)items.p;
+    a.u = (uint32_t *)b->d_u.p;
+    a.v = (uint32_t *)b->d_v.p;
+    a.split_flags = (uint32_t *)b->d_split.p;
+    a.queue = (uint32_t *)b->d_queue.p;
+    a.W = b->hd.W;
+    a.nitems = (uint32_t)plan.items.size();
+    a.sigma = b->ss.sigma;
+    a.k = b->params.k;
+    a.e = b->params.e;
+    a.wl = plan.wl;
+    a.split_words = b->split_words;
+    a.rows_max = plan.rows_max;
+    a.persistent = persistent;
+    return a;
+}
+
+int batch_clear(motex_batch *b)
+{
+    hipError_t e;
+    hipStream_t s = b->ctx->stream;
+    uint64_t uvn = std::max<uint64_t>(b->ss.uv_off[b->ss.num], 1);
+    if ((e = launch_fill_u32((uint32_t *)b->d_u.p, uvn, 0, s)) != hipSuccess) return hip_fail(e, "clear u");
+    if ((e = launch_fill_u32((uint32_t *)b->d_v.p, uvn, 0, s)) != hipSuccess) return hip_fail(e, "clear v");
+    if (b->d_split.p && (e = launch_fill_u32((uint32_t *)b->d_split.p, b->d_split.bytes / 4, 0, s)) != hipSuccess)
+        return hip_fail(e, "clear flags");
+    if ((e = launch_fill_u32((uint32_t *)b->d_queue.p, 64, 0, s)) != hipSuccess) return hip_fail(e, "clear queue");
+    return MOTEX_OK;
+}
+
+int hd_run(motex_batch *b)
+{
+    motex_ctx *c = b->ctx;
+    hipError_t e;
+    int rc = batch_clear(b);
+    if (rc) return rc;
+    if ((e = hipEventRecord(c->ev0, c->stream)) != hipSuccess) return hip_fail(e, "event");
+    if (!b->hd.items.empty()) {
+        HdArgs a = hd_args(b, b->hd, b->d_items, 1);
+        uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->num_cu * b->tile.waves_per_cu, b->hd.items.size());
+        if ((e = launch_hd(a, b->tile.blocks, grid, c->stream)) != hipSuccess) return hip_fail(e, "launch Hamming kernel");
+    }
+    if ((e = hipEventRecord(c->ev1, c->stream)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "Hamming kernel");
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    b->stats.kernel_ms = ms;
+    b->stats.main_kernel_ms = ms;
+    return MOTEX_OK;
+}
+
+// Plain per-pair form: one launch per (x, y) pair.  The launch covers the pair with a few
+// row tiles (one wave each) and the smallest block count that holds the text.
+static const uint32_t kPlainRows = 128;
+
+static uint32_t plain_blocks(uint32_t n)
+{
+    uint32_t need = n + kPlainRows + 32;
+    return need <= kDiagPerBlock ? 1 : (need <= 2 * kDiagPerBlock ? 2 : 4);
+}
+
+int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32_t>> &pairs, double *ms_out)
+{
+    motex_ctx *c = b->ctx;
+    hipStream_t s = b->stream ? b->stream : c->stream;
+    hipError_t e;
+    const SeqSet &ss = b->ss;
+    const uint32_t k = b->params.k;
+    struct Launch { uint32_t first, count, blocks; };
+    std::vector<HdItem> all;
+    std::vector<Launch> launches;
+    uint32_t slots = 0;
+    HdPlan pp;
+    for (auto &pr : pairs) {
+        uint32_t i = pr.first, j = pr.second;
+        if (ss.len[i] < k || ss.len[j] < k) continue;
+        std::vector<int32_t> oy{(int32_t)j};
+        std::vector<uint32_t> xs{i};
+        uint32_t blocks = plain_blocks(ss.len[j]);
+        plan_hd_items(ss, k, blocks, kPlainRows, xs, &oy, pp);
+        for (HdItem &it : pp.items) if (it.split != kNoSplit) it.split = slots;  // one slot per launch
+        if (pp.split_slots) slots++;
+        launches.push_back({(uint32_t)all.size(), (uint32_t)pp.items.size(), blocks});
+        all.insert(all.end(), pp.items.begin(), pp.items.end());
+    }
+    if (ms_out) *ms_out = 0;
+    if (all.empty()) return MOTEX_OK;
+    DevBuf items, flags;
+    if ((e = items.alloc(all.size() * sizeof(HdItem))) != hipSuccess) return hip_fail(e, "alloc plain items");
+    if ((e = hipMemcpy(items.p, all.data(), all.size() * sizeof(HdItem), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy plain items");
+    if ((e = flags.alloc((uint64_t)std::max<uint32_t>(slots, 1) * b->split_words * 4)) != hipSuccess)
+        return hip_fail(e, "alloc plain flags");
+    if ((e = launch_fill_u32((uint32_t *)flags.p, flags.bytes / 4, 0, s)) != hipSuccess) return hip_fail(e, "clear flags");
+    HdPlan shape;
+    shape.rows_max = kPlainRows + k - 1;
+    hipEventRecord(c->ev0, s);
+    for (const Launch &L : launches) {
+        shape.wl = hd_lds_words(L.blocks, shape.rows_max);
+        HdArgs a = hd_args(b, shape, items, 0);
+        a.items = (const HdItem *)items.p + L.first;
+        a.nitems = L.count;
+        a.split_flags = (uint32_t *)flags.p;
+        if ((e = launch_hd(a, L.blocks, L.count, s)) != hipSuccess) return hip_fail(e, "launch plain kernel");
+    }
+    hipEventRecord(c->ev1, s);
+    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "plain kernel");
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (ms_out) *ms_out = ms;
+    return MOTEX_OK;
+}
+
+int hd_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x)
+{
+    const SeqSet &ss = b->ss;
+    double total = 0;
+    for (uint32_t i = first_x; i < first_x + num_x && i < ss.num; ++i) {
+        std::vector<std::pair<uint32_t, uint32_t>> pairs;
+        for (uint32_t j = 0; j < ss.num; ++j) pairs.push_back({i, j});
+        double ms = 0;
+        int rc = hd_launch_pairs(b, pairs, &ms);
+        if (rc) return rc;
+        total += ms;
+    }
+    b->stats.kernel_ms = total;
+    b->stats.main_kernel_ms = total;
+    return MOTEX_OK;
+}
+
+int pair_run(motex_batch *b, uint32_t i, uint32_t j)
+{
+    int rc = batch_clear(b);
+    if (rc) return rc;
+    if (b->params.nb_gaps) return struct_pair_run(b, i, j);
+    if (b->params.distance == 1) return ed_pair_run(b, i, j);
+    std::vector<std::pair<uint32_t, uint32_t>> pairs{{i, j}};
+    double ms = 0;
+    rc = hd_launch_pairs(b, pairs, &ms);
+    b->stats.kernel_ms = ms;
+    b->stats.main_kernel_ms = ms;
+    return rc;
+}
+
+double pair_cells(const SeqSet &ss, const motex_params &p, uint64_t *pairs, uint64_t *skipped)
+{
+    uint32_t tot = total_motif_length(&p);
+    double sm = 0, sn = 0;
+    uint64_t ok = 0;
+    for (uint32_t i = 0; i < ss.num; ++i)
+        if (ss.len[i] >= tot) { sm += ss.len[i] - p.k + 1; sn += ss.len[i]; ++ok; }
+    *pairs = ok * ok;
+    *skipped = (uint64_t)ss.num * ss.num - ok * ok;
+    return sm * sn;
+}
+
+}  // namespace motex
+
+extern "C" int motex_batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens,
+                                  uint32_t num, const motex_params *p, motex_batch **out)
+{
+    if (!c || !out) return set_error(MOTEX_EINVAL, "ctx/out is NULL");
+    *out = nullptr;
+    int rc = validate_params(p);
+    if (rc) return rc;
+    if (num && (!seqs || !lens)) return set_error(MOTEX_EINVAL, "seqs/lens is NULL");
+    hipError_t e;
+    if ((e = hipSetDevice(c->device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    std::unique_ptr<motex_batch> b(new motex_batch());
+    b->ctx = c;
+    b->params = *p;
+    b->gmin.assign(p->gap_min, p->gap_min + p->nb_gaps);
+    b->gmax.assign(p->gap_max, p->gap_max + p->nb_gaps);
+    b->blen.assign(p->box_len, p->box_len + p->nb_gaps);
+    b->berr.assign(p->box_err, p->box_err + p->nb_gaps);
+    b->params.gap_min = b->gmin.data();
+    b->params.gap_max = b->gmax.data();
+    b->params.box_len = b->blen.data();
+    b->params.box_err = b->berr.data();
+    b->S = enumerate_structs(&b->params, &b->nb_structs);
+    if ((rc = b->ss.build(seqs, lens, num, b->nb_structs))) return rc;
+    b->stats.cells = pair_cells(b->ss, b->params, &b->stats.pairs, &b->stats.skipped_pairs);
+    if ((rc = batch_upload_common(b.get()))) return rc;
+    if (p->nb_gaps) rc = struct_prepare(b.get());
+    else if (p->distance == 0) rc = hd_prepare(b.get());
+    else rc = ed_prepare(b.get());
+    if (rc) return rc;
+    b->stats.tile_blocks = b->tile.blocks;
+    b->stats.tile_rows = b->tile.rows;
+    b->stats.waves_per_cu = b->tile.waves_per_cu;
+    b->stats.kernel_variant = b->tile.variant;
+    *out = b.release();
+    return MOTEX_OK;
+}
+
+extern "C" int motex_batch_run(motex_batch *b)
+{
+    if (!b) return set_error(MOTEX_EINVAL, "batch is NULL");
+    hipSetDevice(b->ctx->device);
+    if (b->params.nb_gaps) return struct_run(b);
+    if (b->params.distance == 0) return hd_run(b);
+    return ed_run(b);
+}
+
+extern "C" int motex_batch_clear(motex_batch *b)
+{
+    if (!b) return set_error(MOTEX_EINVAL, "batch is NULL");
+    hipSetDevice(b->ctx->device);
+    int rc = batch_clear(b);
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(b->ctx->stream);
+    return e == hipSuccess ? MOTEX_OK : hip_fail(e, "clear");
+}
+
+extern "C" int motex_batch_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x)
+{
+    if (!b) return set_error(MOTEX_EINVAL, "batch is NULL");
+    hipSetDevice(b->ctx->device);
+    if (b->params.nb_gaps) return struct_run_plain(b, first_x, num_x);
+    if (b->params.distance == 0) return hd_run_plain(b, first_x, num_x);
+    return ed_run_plain(b, first_x, num_x);
+}
+
+extern "C" int motex_batch_download(motex_batch *b, uint32_t *const *u, uint32_t *const *v)
+{
+    if (!b || !u || !v) return set_error(MOTEX_EINVAL, "NULL argument");
+    hipSetDevice(b->ctx->device);
+    const SeqSet &ss = b->ss;
+    uint64_t n = ss.uv_off[ss.num];
+    std::vector<uint32_t> hu(n), hv(n);
+    hipError_t e;
+    if (n) {
+        if ((e = hipMemcpy(hu.data(), b->d_u.p, n * 4, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "copy u");
+        if ((e = hipMemcpy(hv.data(), b->d_v.p, n * 4, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "copy v");
+    }
+    for (uint32_t i = 0; i < ss.num; ++i) {
+        uint64_t o = ss.uv_off[i], cnt = ss.uv_off[i + 1] - o;
+        if (!cnt) continue;
+        if (!u[i] || !v[i]) return set_error(MOTEX_EINVAL, "u/v row %u is NULL", i);
+        for (uint64_t t = 0; t < cnt; ++t) { u[i][t] += hu[o + t]; v[i][t] += hv[o + t]; }
+    }
+    return MOTEX_OK;
+}
+
+extern "C" int motex_batch_stats(const motex_batch *b, motex_stats *out)
+{
+    if (!b || !out) return set_error(MOTEX_EINVAL, "NULL argument");
+    *out = b->stats;
+    out->work_items = b->work_items();
+    return MOTEX_OK;
+}
+
+extern "C" void motex_batch_destroy(motex_batch *b)
+{
+    if (!b) return;
+    hipSetDevice(b->ctx->device);
+    delete b;
+}
+
+extern "C" int motex_extract_all_pairs(motex_ctx *c, const char *const *seqs, const uint32_t *lens,
+                                       uint32_t num, const motex_params *p, uint32_t *const *u,
+                                       uint32_t *const *v)
+{
+    motex_batch *b = nullptr;
+    int rc = motex_batch_create(c, seqs, lens, num, p, &b);
+    if (rc) return rc;
+    rc = motex_batch_run(b);
+    if (!rc) rc = motex_batch_download(b, u, v);
+    motex_batch_destroy(b);
+    return rc;
+}
+
+extern "C" int motex_extract_pair(motex_ctx *c, const char *x, uint32_t m, const char *y, uint32_t n,
+                                  const motex_params *p, uint32_t *u, uint32_t *v)
+{
+    if (!c) return set_error(MOTEX_EINVAL, "ctx is NULL");
+    int rc = validate_params(p);
+    if (rc) return rc;
+    const char *seqs[2] = {x, y};
+    uint32_t lens[2] = {m, n};
+    motex_batch *b = nullptr;
+    if ((rc = motex_batch_create(c, seqs, lens, 2, p, &b))) return rc;
+    rc = pair_run(b, 0, 1);
+    if (!rc) {
+        uint32_t dummy_u = 0, dummy_v = 0;
+        std::vector<uint32_t> yu((size_t)b->nb_structs * n + 1), yv((size_t)b->nb_structs * n + 1);
+        uint32_t *us[2] = {u, yu.data()}, *vs[2] = {v, yv.data()};
+        (void)dummy_u; (void)dummy_v;
+        rc = motex_batch_download(b, us, vs);
+    }
+    motex_batch_destroy(b);
+    return rc;
+}

@@ -1,0 +1,37 @@
+// Tile-shape selection, structured-motif and serving entry points.
+#include <hip/hip_runtime.h>
+
+#include "runtime.h"
+
+namespace motex {
+
+TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss)
+{
+    TileShape t;
+    t.blocks = ss.sigma <= 8 ? 4 : 2;
+    t.rows = 512;
+    t.waves_per_cu = 16;
+    // keep the Hamming kernel's LDS window under 64 KB
+    while (t.blocks > 1 &&
+           hd_lds_bytes(ss.sigma, hd_lds_words(t.blocks, t.rows + p.k), t.rows + p.k) > 65536)
+        t.blocks /= 2;
+    while (t.rows > 64 &&
+           hd_lds_bytes(ss.sigma, hd_lds_words(t.blocks, t.rows + p.k), t.rows + p.k) > 65536)
+        t.rows /= 2;
+    (void)c;
+    return t;
+}
+
+int struct_prepare(motex_batch *b) { (void)b; return set_error(MOTEX_EUNSUPPORTED, "structured motifs: not built yet"); }
+int struct_run(motex_batch *b) { (void)b; return set_error(MOTEX_EUNSUPPORTED, "structured motifs: not built yet"); }
+int struct_run_plain(motex_batch *b, uint32_t, uint32_t) { (void)b; return set_error(MOTEX_EUNSUPPORTED, "structured motifs: not built yet"); }
+int struct_pair_run(motex_batch *b, uint32_t, uint32_t) { (void)b; return set_error(MOTEX_EUNSUPPORTED, "structured motifs: not built yet"); }
+void serving_shutdown(motex_ctx *c) { (void)c; }
+
+}  // namespace motex
+
+extern "C" int motex_submit(motex_ctx *, const motex_job *, motex_job_id *) { return motex::set_error(MOTEX_EUNSUPPORTED, "serving: not built yet"); }
+extern "C" int motex_poll(motex_ctx *, motex_job_id, int *) { return motex::set_error(MOTEX_EUNSUPPORTED, "serving: not built yet"); }
+extern "C" int motex_wait(motex_ctx *, motex_job_id) { return motex::set_error(MOTEX_EUNSUPPORTED, "serving: not built yet"); }
+extern "C" int motex_job_release(motex_ctx *, motex_job_id) { return motex::set_error(MOTEX_EUNSUPPORTED, "serving: not built yet"); }
+extern "C" int motex_job_times(motex_ctx *, motex_job_id, double *, double *, double *) { return motex::set_error(MOTEX_EUNSUPPORTED, "serving: not built yet"); }

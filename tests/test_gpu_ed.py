@@ -1,0 +1,56 @@
+"""GPU parity of the edit-distance (MaxShift) path against the oracle (bit-exact u and v)."""
+import numpy as np
+import pytest
+
+from motex_amd import MotexParams, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_seqs(rng, n, lo, hi, alpha):
+    a = np.frombuffer(alpha, np.uint8)
+    return [a[rng.integers(0, len(a), rng.integers(lo, hi + 1))].tobytes() for _ in range(n)]
+
+
+def check(ctx, oracle, seqs, k, e, alphabet="DNA"):
+    print(f"[ed parity] n={len(seqs)} total={sum(map(len, seqs))} k={k} e={e}", flush=True)
+    u, v, off = ctx.extract_all_pairs(seqs, MotexParams(k=k, e=e, distance=1, alphabet=alphabet))
+    ou, ov, ooff, _ = oracle.all_pairs(seqs, 1, k, e)
+    bad = np.nonzero((u != ou) | (v != ov))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: gpu {u[bad[:5]]},{v[bad[:5]]} oracle {ou[bad[:5]]},{ov[bad[:5]]}"
+    return u, v
+
+
+def test_config3_subset(ctx, oracle):
+    w = synth.config3(scale=0.03)  # 15 protein sequences, lengths 200..2000
+    u, v = check(ctx, oracle, w.seqs, w.k, w.e, "PROT")
+    assert v.sum() > 0
+
+
+@pytest.mark.parametrize("seed,k,e,alpha,n,lo,hi", [
+    (1, 5, 1, b"ACGT", 10, 3, 400),        # short texts (< k) are skipped
+    (2, 8, 2, b"AC", 8, 60, 700),          # dense hits, several strips
+    (3, 1, 0, b"ACGT", 6, 1, 90),          # k = 1
+    (4, 14, 3, b"ACGT", 6, 100, 600),      # widest 32-bit word case
+    (5, 12, 0, b"AC", 6, 100, 300),        # e = 0
+    (6, 20, 4, b"ACGT", 5, 100, 400),      # 64-bit words
+    (7, 32, 6, b"AC", 4, 100, 300),        # k = 32
+    (8, 3, 2, b"ACGT", 8, 2, 70),          # e = k-1, texts shorter than one wave
+])
+def test_random_edges(ctx, oracle, seed, k, e, alpha, n, lo, hi):
+    rng = np.random.default_rng(100 + seed)
+    check(ctx, oracle, rand_seqs(rng, n, lo, hi, alpha), k, e)
+
+
+def test_plain_form_equals_batched(ctx):
+    rng = np.random.default_rng(5)
+    seqs = rand_seqs(rng, 6, 100, 500, b"ACGT")
+    p = MotexParams(k=6, e=1, distance=1)
+    b = ctx.batch(seqs, p)
+    b.run()
+    ub, vb, _ = b.download()
+    b.clear()
+    b.run_plain()
+    up, vp, _ = b.download()
+    assert np.array_equal(ub, up) and np.array_equal(vb, vp)
+    b.close()

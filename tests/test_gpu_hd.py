@@ -1,0 +1,94 @@
+"""GPU parity of the Hamming-distance path against the oracle (bit-exact u and v)."""
+import numpy as np
+import pytest
+
+import motex_amd
+from motex_amd import MotexParams, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_seqs(rng, n, lo, hi, alpha):
+    a = np.frombuffer(alpha, np.uint8)
+    return [a[rng.integers(0, len(a), rng.integers(lo, hi + 1))].tobytes() for _ in range(n)]
+
+
+def check(ctx, oracle, seqs, k, e, alphabet="DNA"):
+    print(f"[hd parity] n={len(seqs)} total={sum(map(len, seqs))} k={k} e={e}", flush=True)
+    u, v, off = ctx.extract_all_pairs(seqs, MotexParams(k=k, e=e, distance=0, alphabet=alphabet))
+    ou, ov, ooff, _ = oracle.all_pairs(seqs, 0, k, e)
+    assert np.array_equal(off[:-1], ooff)
+    bad = np.nonzero((u != ou) | (v != ov))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: gpu {u[bad[:5]]},{v[bad[:5]]} oracle {ou[bad[:5]]},{ov[bad[:5]]}"
+    return u, v
+
+
+def test_config1_full(ctx, oracle):
+    w = synth.config1()
+    u, v = check(ctx, oracle, w.seqs, w.k, w.e)
+    assert v.sum() > 0
+
+
+def test_config2_subset(ctx, oracle):
+    w = synth.config2(scale=0.05)  # 60 sequences of 1000 nt
+    check(ctx, oracle, w.seqs, w.k, w.e)
+
+
+@pytest.mark.parametrize("seed,k,e,alpha,n,lo,hi", [
+    (1, 5, 1, b"ACGT", 20, 3, 300),       # short sequences (< k) mixed in
+    (2, 8, 2, b"ACGTN", 15, 50, 400),      # N is an ordinary symbol
+    (3, 1, 0, b"ACGT", 8, 1, 100),         # k = 1
+    (4, 12, 0, b"AC", 12, 100, 500),       # e = 0 (exact)
+    (5, 12, 11, b"ACGT", 6, 12, 80),       # e = k-1
+    (6, 31, 7, b"AC", 6, 40, 300),         # five counter planes
+    (7, 64, 20, b"AC", 5, 60, 300),        # k = 64 (seven planes)
+    (8, 10, 2, b"ARNDCQEGHILKMFPSTWYV", 10, 100, 900),  # protein alphabet
+    (9, 6, 1, b"0123456789", 10, 50, 300),  # USR alphabet
+])
+def test_random_edges(ctx, oracle, seed, k, e, alpha, n, lo, hi):
+    rng = np.random.default_rng(seed)
+    alphabet = "PROT" if len(alpha) == 20 else ("USR" if alpha[0:1] == b"0" else "DNA")
+    check(ctx, oracle, rand_seqs(rng, n, lo, hi, alpha), k, e, alphabet)
+
+
+def test_long_sequence_split(ctx, oracle):
+    # texts longer than one work item's range are split; u must still count the text once
+    rng = np.random.default_rng(42)
+    seqs = rand_seqs(rng, 2, 20000, 20000, b"AC") + [b"ACACACACACAC" * 100]
+    check(ctx, oracle, seqs, 10, 2)
+
+
+def test_empty_and_single(ctx, oracle):
+    check(ctx, oracle, [b"ACGTACGTAC"], 4, 1)
+    check(ctx, oracle, [b"ACG", b"AC"], 4, 1)   # every pair skipped
+
+
+def test_plain_form_equals_batched(ctx, oracle):
+    w = synth.config1(scale=0.2)
+    p = MotexParams(k=w.k, e=w.e)
+    b = ctx.batch(w.seqs, p)
+    b.run()
+    ub, vb, off = b.download()
+    b.clear()
+    b.run_plain()
+    up, vp, _ = b.download()
+    assert np.array_equal(ub, up) and np.array_equal(vb, vp)
+    b.close()
+
+
+def test_extract_pair_accumulates(ctx, oracle):
+    rng = np.random.default_rng(7)
+    x, y = rand_seqs(rng, 2, 300, 600, b"ACG")
+    p = MotexParams(k=7, e=2)
+    u = np.full(len(x), 5, np.uint32)
+    v = np.full(len(x), 9, np.uint32)
+    ctx.extract_pair(x, y, p, u, v)
+    ou, ov = oracle.pair(0, x, y, 7, 2)
+    assert np.array_equal(u, ou + 5) and np.array_equal(v, ov + 9)
+
+
+def test_invalid_params_raise(ctx):
+    with pytest.raises(motex_amd.MotexError):
+        ctx.extract_all_pairs([b"ACGT"], MotexParams(k=4, e=4))
+    with pytest.raises(motex_amd.MotexError):
+        ctx.extract_all_pairs([b"ACGT"], MotexParams(k=4, e=1, distance=2))
