@@ -120,19 +120,21 @@ int ed_run(motex_batch *b)
     motex_ctx *c = b->ctx;
     hipStream_t s = b->stream ? b->stream : c->stream;
     hipError_t e;
+    if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "event");
     int rc = batch_clear(b);
     if (rc) return rc;
-    if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventRecord(c->ev2, s)) != hipSuccess) return hip_fail(e, "event");
     if (b->ed_nitems) {
         uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->num_cu * b->tile.waves_per_cu, b->ed_nitems);
         if ((rc = ed_launch(b, b->d_items.p, b->ed_nitems, grid, 1, s))) return rc;
     }
     if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "edit kernel");
-    float ms = 0;
+    float ms = 0, kms = 0;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    hipEventElapsedTime(&kms, c->ev2, c->ev1);
     b->stats.kernel_ms = ms;
-    b->stats.main_kernel_ms = ms;
+    b->stats.main_kernel_ms = kms;
     return MOTEX_OK;
 }
 

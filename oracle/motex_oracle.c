@@ -31,9 +31,11 @@
  *     (a different algorithm), the black-box probe facts recorded for libflasm
  *     (TTTTTACGTC / CGTCGGGGGG, k=5, e=1: ACGTC u=1 v=1, TTTTT v=2, GGGGG v=3), e=0
  *     equals exact k-mer counting, and the transpose symmetry H(x,y) = H(y,x)^T.
- *   - orc_ed_pair: PARTIALLY PINNED.  e=0 equals exact k-mer counting; the recorded
- *     probe facts (an interior exact hit yields three end positions at e=1; a
- *     length-(k-1) text prefix at the text start is an occurrence) hold; bitminmax's
+ *   - orc_ed_pair: PARTIALLY PINNED.  At e=0 every hit is an exact occurrence (the
+ *     greedy bitminmax choice can also miss some, which is a property of the recurrence,
+ *     not of this restatement); the recorded probe facts (an interior exact hit yields
+ *     three end positions at e=1; a length-(k-1) text prefix at the text start is an
+ *     occurrence) hold; bitminmax's
  *     case analysis equals "largest word among those of least popcount" exhaustively
  *     on small words.  Whether libflasm's binary equals the MaxShift recurrence for
  *     every e >= 1 input is parity unpinned: the binary may not be run here and the
@@ -310,11 +312,28 @@ uint32_t orc_recurse(uint32_t nb_gaps, const uint32_t *gmin, const uint32_t *gma
  * uv_off[i] and holds nb_structs*len[i] words (caller-zeroed, accumulated).  threads <= 0
  * means the OpenMP default.  Returns the number of skipped (short) pairs, or negative
  * on failure. */
+int64_t orc_all_pairs_range(int distance, const char *seqs, const uint64_t *off,
+                            const uint32_t *len, uint32_t num, uint32_t x0, uint32_t x1,
+                            uint32_t k, uint32_t e, uint32_t nb_gaps, const uint32_t *gmin,
+                            const uint32_t *gmax, const uint32_t *blens, const uint32_t *berrs,
+                            const uint64_t *uv_off, uint32_t *u, uint32_t *v, int threads);
+
 int64_t orc_all_pairs(int distance, const char *seqs, const uint64_t *off, const uint32_t *len,
                       uint32_t num, uint32_t k, uint32_t e, uint32_t nb_gaps,
                       const uint32_t *gmin, const uint32_t *gmax, const uint32_t *blens,
                       const uint32_t *berrs, const uint64_t *uv_off, uint32_t *u, uint32_t *v,
                       int threads)
+{
+    return orc_all_pairs_range(distance, seqs, off, len, num, 0, num, k, e, nb_gaps, gmin, gmax,
+                               blens, berrs, uv_off, u, v, threads);
+}
+
+/* x sequences [x0, x1) against every y: a bounded sample of the all-pairs loop. */
+int64_t orc_all_pairs_range(int distance, const char *seqs, const uint64_t *off,
+                            const uint32_t *len, uint32_t num, uint32_t x0, uint32_t x1,
+                            uint32_t k, uint32_t e, uint32_t nb_gaps, const uint32_t *gmin,
+                            const uint32_t *gmax, const uint32_t *blens, const uint32_t *berrs,
+                            const uint64_t *uv_off, uint32_t *u, uint32_t *v, int threads)
 {
     uint32_t total_mot = k;
     for (uint32_t g = 0; g < nb_gaps; ++g) total_mot += gmin[g] + blens[g];
@@ -329,9 +348,9 @@ int64_t orc_all_pairs(int distance, const char *seqs, const uint64_t *off, const
     int failed = 0;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
-#pragma omp parallel for schedule(static) reduction(+ : skipped)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : skipped)
 #endif
-    for (int64_t i = 0; i < (int64_t)num; ++i) {
+    for (int64_t i = x0; i < (int64_t)x1; ++i) {
         uint32_t m = len[i];
         for (uint32_t j = 0; j < num; ++j) {
             uint32_t n = len[j];

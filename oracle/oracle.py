@@ -45,6 +45,10 @@ def lib():
                                     C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p, u64p,
                                     u32p, u32p, C.c_int]
         L.orc_all_pairs.restype = C.c_int64
+        L.orc_all_pairs_range.argtypes = [C.c_int, C.c_char_p, u64p, u32p, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p,
+                                          u32p, u32p, u32p, u64p, u32p, u32p, C.c_int]
+        L.orc_all_pairs_range.restype = C.c_int64
         L.orc_bitminmax.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_bitminmax.restype = C.c_uint64
         L.orc_write.argtypes = [C.c_void_p, C.c_char_p, u64p, u32p, C.c_uint32, u64p, u32p, u32p,
@@ -124,8 +128,10 @@ def pack(seqs: list[bytes]):
     return b"".join(seqs), off, lens
 
 
-def all_pairs(seqs: list[bytes], distance: int, k: int, e: int, boxes=(), threads: int = 0):
-    """Returns (u, v, uv_off, skipped) with u, v flat (nb_structs*len per sequence)."""
+def all_pairs(seqs: list[bytes], distance: int, k: int, e: int, boxes=(), threads: int = 0,
+              x_range=None):
+    """Returns (u, v, uv_off, skipped) with u, v flat (nb_structs*len per sequence).
+    x_range=(x0, x1) restricts the x sequences (every y is still searched)."""
     blob, off, lens = pack(seqs)
     gmin = _arr32([b[0] for b in boxes]); gmax = _arr32([b[1] for b in boxes])
     bl = _arr32([b[2] for b in boxes]); be = _arr32([b[3] for b in boxes])
@@ -139,9 +145,10 @@ def all_pairs(seqs: list[bytes], distance: int, k: int, e: int, boxes=(), thread
     tot = int(sizes.sum())
     u = np.zeros(max(tot, 1), np.uint32)
     v = np.zeros(max(tot, 1), np.uint32)
-    sk = lib().orc_all_pairs(distance, blob, _p64(off), _p32(lens), len(seqs), k, e, len(boxes),
-                             _p32(gmin), _p32(gmax), _p32(bl), _p32(be), _p64(uv_off), _p32(u),
-                             _p32(v), threads)
+    x0, x1 = x_range if x_range else (0, len(seqs))
+    sk = lib().orc_all_pairs_range(distance, blob, _p64(off), _p32(lens), len(seqs), x0, x1, k, e,
+                                   len(boxes), _p32(gmin), _p32(gmax), _p32(bl), _p32(be),
+                                   _p64(uv_off), _p32(u), _p32(v), threads)
     if sk < 0:
         raise ValueError("orc_all_pairs failed")
     return u[:tot], v[:tot], uv_off, int(sk)
