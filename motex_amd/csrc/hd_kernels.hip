@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "motex_internal.h"
+#include "hd_device.h"
 
 namespace motex {
 
@@ -47,33 +48,6 @@ __global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n, uint32_t v
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = val;
-}
-
-// Index of the sequence containing text position pt among seq_start[j0 .. j0+ny-1].
-__device__ __forceinline__ uint32_t find_seq(const uint32_t *__restrict__ seq_start,
-                                             uint32_t j0, uint32_t ny, uint32_t pt)
-{
-    uint32_t lo = j0, hi = j0 + ny - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (seq_start[mid] <= pt) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-// "count <= E" over P bit-sliced planes, least significant plane first.
-template <int P, int E>
-__device__ __forceinline__ uint32_t le_const(const uint32_t *c, uint32_t e)
-{
-    uint32_t res = ~0u;
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        uint32_t eb;
-        if constexpr (E >= 0) eb = ((E >> p) & 1) ? ~0u : 0u;
-        else eb = ((e >> p) & 1) ? ~0u : 0u;
-        res = (~c[p] & (res | eb)) | (res & eb);
-    }
-    return res;
 }
 
 // Folds the hit flags of one row into u/v.  Wave-uniform: all lanes call it together.

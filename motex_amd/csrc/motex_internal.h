@@ -70,4 +70,39 @@ struct EdArgs {
     uint32_t persistent;      // 1: pull items from *queue; 0: item = blockIdx.x
 };
 
+constexpr uint32_t kMaxGaps = 4;
+constexpr uint32_t kMaxBoxes = kMaxGaps + 1;
+constexpr uint32_t kStructMaxLen = 4096;    // longest sequence of the structured kernels
+
+// One (x, y) pair of the structured-motif work queue.
+struct StructItem {
+    uint32_t x_start, m;    // x: text position and length
+    uint32_t y_start, n;    // y: text position and length
+    uint32_t uv_base;       // u/v index of (structure 0, pos_x 0) of x
+    uint32_t pad;
+};
+
+struct StructArgs {
+    const uint32_t *ne;       // Hamming NE tables (as HdArgs)
+    const uint8_t *codes;
+    const StructItem *items;
+    const uint32_t *S;        // nb_structs x nb_gaps gap lengths (last gap fastest)
+    uint32_t *u, *v;
+    uint32_t *queue;
+    uint32_t *scratch;        // per workgroup: box bitmaps, then two edit boundary streams
+    uint64_t scratch_words;   // per workgroup
+    uint64_t bm_words;        // one box bitmap: ceil(n_max/32) * m_max words
+    uint32_t stream_words;
+    uint32_t W, sigma, wl;
+    uint32_t nitems, persistent;
+    uint32_t distance;
+    uint32_t nb_gaps, nb_structs;
+    uint32_t blen[kMaxBoxes], berr[kMaxBoxes];  // box b (box 0 is (k, e))
+    uint32_t gmin[kMaxGaps], gmax[kMaxGaps];
+    uint32_t ndist;                             // distinct box shapes
+    uint32_t dist_of[kMaxBoxes];                // box b -> distinct shape
+    uint32_t dlen[kMaxBoxes], derr[kMaxBoxes];
+    uint32_t m_max, hcap;
+};
+
 }  // namespace motex

@@ -22,10 +22,6 @@ TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss)
     return t;
 }
 
-int struct_prepare(motex_batch *b) { (void)b; return set_error(MOTEX_EUNSUPPORTED, "structured motifs: not built yet"); }
-int struct_run(motex_batch *b) { (void)b; return set_error(MOTEX_EUNSUPPORTED, "structured motifs: not built yet"); }
-int struct_run_plain(motex_batch *b, uint32_t, uint32_t) { (void)b; return set_error(MOTEX_EUNSUPPORTED, "structured motifs: not built yet"); }
-int struct_pair_run(motex_batch *b, uint32_t, uint32_t) { (void)b; return set_error(MOTEX_EUNSUPPORTED, "structured motifs: not built yet"); }
 void serving_shutdown(motex_ctx *c) { (void)c; }
 
 }  // namespace motex

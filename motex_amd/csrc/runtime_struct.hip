@@ -1,0 +1,240 @@
+// Structured-motif batches: pair work queue, scratch sizing and launches of
+// struct_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "runtime.h"
+
+namespace motex {
+
+hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStream_t s);
+
+static const uint32_t kStructHitCap = 4096;
+
+static size_t struct_lds_bytes(const StructArgs &a)
+{
+    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + a.m_max + 1) * 4;
+}
+
+// Fills the shape fields of StructArgs from the parameters.
+static int struct_shape(const motex_batch *b, StructArgs &a)
+{
+    const motex_params &p = b->params;
+    if (p.nb_gaps > kMaxGaps)
+        return set_error(MOTEX_EUNSUPPORTED, "structured motifs with more than %u gaps", kMaxGaps);
+    a.nb_gaps = p.nb_gaps;
+    a.nb_structs = b->nb_structs;
+    a.distance = p.distance;
+    a.blen[0] = p.k;
+    a.berr[0] = p.e;
+    for (uint32_t g = 0; g < p.nb_gaps; ++g) {
+        a.gmin[g] = p.gap_min[g];
+        a.gmax[g] = p.gap_max[g];
+        a.blen[g + 1] = p.box_len[g];
+        a.berr[g + 1] = p.box_err[g];
+    }
+    a.ndist = 0;
+    for (uint32_t bx = 0; bx <= p.nb_gaps; ++bx) {
+        uint32_t d = 0;
+        while (d < a.ndist && !(a.dlen[d] == a.blen[bx] && a.derr[d] == a.berr[bx])) ++d;
+        if (d == a.ndist) { a.dlen[d] = a.blen[bx]; a.derr[d] = a.berr[bx]; a.ndist++; }
+        a.dist_of[bx] = d;
+    }
+    return MOTEX_OK;
+}
+
+struct StructPlan {
+    std::vector<StructItem> items;
+};
+
+static void plan_pairs(const SeqSet &ss, uint32_t total_mot, const std::vector<std::pair<uint32_t, uint32_t>> *only,
+                       std::vector<StructItem> &out)
+{
+    out.clear();
+    auto add = [&](uint32_t i, uint32_t j) {
+        if (ss.len[i] < total_mot || ss.len[j] < total_mot) return;
+        out.push_back({ss.start[i], ss.len[i], ss.start[j], ss.len[j], (uint32_t)ss.uv_off[i], 0});
+    };
+    if (only) {
+        for (auto &pr : *only) add(pr.first, pr.second);
+    } else {
+        for (uint32_t i = 0; i < ss.num; ++i)
+            for (uint32_t j = 0; j < ss.num; ++j) add(i, j);
+    }
+}
+
+int struct_prepare(motex_batch *b)
+{
+    const SeqSet &ss = b->ss;
+    StructArgs a{};
+    int rc = struct_shape(b, a);
+    if (rc) return rc;
+    uint32_t maxlen = 0;
+    for (uint32_t i = 0; i < ss.num; ++i) maxlen = std::max(maxlen, ss.len[i]);
+    if (maxlen > kStructMaxLen)
+        return set_error(MOTEX_EUNSUPPORTED, "structured motifs: sequences longer than %u", kStructMaxLen);
+    hipError_t e;
+    if (b->params.distance == 0) {
+        // NE tables over the whole text; every pair's diagonal window must fit inside
+        uint32_t wl = 64 + ((maxlen - 1) >> 5) + 3;
+        uint64_t W = (ss.total + kPadBits) / 32 + kDiagPerBlock / 32 + wl + 8;
+        if ((uint64_t)(ss.sigma + 1) * wl >= 65536 || ss.sigma * wl >= 65536)
+            return set_error(MOTEX_EUNSUPPORTED, "alphabet too large for the structured kernel");
+        b->hd.W = (uint32_t)W;
+        b->hd.wl = wl;
+        if ((e = b->d_ne.alloc(W * ss.sigma * 4)) != hipSuccess) return hip_fail(e, "alloc NE tables");
+        if ((e = launch_ne_build((const uint8_t *)b->d_codes.p, (uint32_t)ss.total, ss.sigma, (uint32_t)W,
+                                 (uint32_t *)b->d_ne.p, b->ctx->stream)) != hipSuccess)
+            return hip_fail(e, "NE build");
+    }
+    std::vector<StructItem> items;
+    plan_pairs(ss, total_motif_length(&b->params), nullptr, items);
+    std::stable_sort(items.begin(), items.end(), [](const StructItem &x, const StructItem &z) {
+        return (uint64_t)x.m * x.n > (uint64_t)z.m * z.n;
+    });
+    if ((e = b->d_items.alloc(std::max<size_t>(items.size(), 1) * sizeof(StructItem))) != hipSuccess)
+        return hip_fail(e, "alloc items");
+    if (!items.empty() &&
+        (e = hipMemcpy(b->d_items.p, items.data(), items.size() * sizeof(StructItem), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy items");
+    b->ed_nitems = (uint32_t)items.size();
+    if ((e = b->d_aux2.alloc(std::max<size_t>(b->S.size(), 1) * 4)) != hipSuccess) return hip_fail(e, "alloc S");
+    if (!b->S.empty() &&
+        (e = hipMemcpy(b->d_aux2.p, b->S.data(), b->S.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy S");
+    b->tile = choose_tile(b->ctx, b->params, ss);
+    // cells: one DP per distinct box shape and pair
+    double cells = 0;
+    for (const StructItem &it : items)
+        for (uint32_t d = 0; d < a.ndist; ++d)
+            if (it.m >= a.dlen[d]) cells += (double)(it.m - a.dlen[d] + 1) * it.n;
+    b->stats.cells = cells;
+    return hipStreamSynchronize(b->ctx->stream) == hipSuccess ? MOTEX_OK : set_error(MOTEX_EDEVICE, "NE build");
+}
+
+static int struct_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_t grid,
+                         uint32_t persistent, hipStream_t s)
+{
+    const SeqSet &ss = b->ss;
+    StructArgs a{};
+    int rc = struct_shape(b, a);
+    if (rc) return rc;
+    uint32_t maxlen = 0;
+    for (uint32_t i = 0; i < ss.num; ++i) maxlen = std::max(maxlen, ss.len[i]);
+    a.ne = (const uint32_t *)b->d_ne.p;
+    a.codes = (const uint8_t *)b->d_codes.p;
+    a.items = (const StructItem *)items;
+    a.S = (const uint32_t *)b->d_aux2.p;
+    a.u = (uint32_t *)b->d_u.p;
+    a.v = (uint32_t *)b->d_v.p;
+    a.queue = (uint32_t *)b->d_queue.p;
+    a.W = b->hd.W;
+    a.sigma = ss.sigma;
+    a.wl = b->params.distance == 0 ? b->hd.wl : 0;
+    a.nitems = nitems;
+    a.persistent = persistent;
+    a.m_max = std::max<uint32_t>(maxlen, 1);
+    a.hcap = kStructHitCap;
+    a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
+    a.stream_words = (maxlen + 128) * 2;
+    a.scratch_words = a.ndist * a.bm_words + 2ull * a.stream_words;
+    uint64_t need = (uint64_t)grid * a.scratch_words * 4;
+    hipError_t e;
+    if (b->d_aux.bytes < need && (e = b->d_aux.alloc(need)) != hipSuccess) return hip_fail(e, "alloc structured scratch");
+    a.scratch = (uint32_t *)b->d_aux.p;
+    size_t lds = struct_lds_bytes(a);
+    if (lds > 65536) return set_error(MOTEX_EUNSUPPORTED, "structured kernel LDS window too large");
+    if ((e = launch_struct(a, grid, lds, s)) != hipSuccess) return hip_fail(e, "launch structured kernel");
+    return MOTEX_OK;
+}
+
+static uint32_t struct_grid(motex_batch *b, uint32_t nitems)
+{
+    // bound the scratch to about 2 GiB
+    const SeqSet &ss = b->ss;
+    uint32_t maxlen = 0;
+    for (uint32_t i = 0; i < ss.num; ++i) maxlen = std::max(maxlen, ss.len[i]);
+    uint64_t per = ((uint64_t)(maxlen + 31) / 32 * maxlen * (b->params.nb_gaps + 1) + 2ull * (maxlen + 128) * 2) * 4;
+    uint64_t cap = std::max<uint64_t>(1, (2ull << 30) / std::max<uint64_t>(per, 1));
+    uint64_t g = std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, nitems);
+    return (uint32_t)std::max<uint64_t>(1, std::min(g, cap));
+}
+
+int struct_run(motex_batch *b)
+{
+    motex_ctx *c = b->ctx;
+    hipStream_t s = b->stream ? b->stream : c->stream;
+    hipError_t e;
+    if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "event");
+    int rc = batch_clear(b);
+    if (rc) return rc;
+    if ((e = hipEventRecord(c->ev2, s)) != hipSuccess) return hip_fail(e, "event");
+    if (b->ed_nitems) {
+        if ((rc = struct_launch(b, b->d_items.p, b->ed_nitems, struct_grid(b, b->ed_nitems), 1, s))) return rc;
+    }
+    if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "structured kernel");
+    float ms = 0, kms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    hipEventElapsedTime(&kms, c->ev2, c->ev1);
+    b->stats.kernel_ms = ms;
+    b->stats.main_kernel_ms = kms;
+    return MOTEX_OK;
+}
+
+static int struct_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32_t>> &pairs, double *ms_out)
+{
+    motex_ctx *c = b->ctx;
+    hipStream_t s = b->stream ? b->stream : c->stream;
+    hipError_t e;
+    std::vector<StructItem> items;
+    plan_pairs(b->ss, total_motif_length(&b->params), &pairs, items);
+    if (ms_out) *ms_out = 0;
+    if (items.empty()) return MOTEX_OK;
+    DevBuf d;
+    if ((e = d.alloc(items.size() * sizeof(StructItem))) != hipSuccess) return hip_fail(e, "alloc items");
+    if ((e = hipMemcpy(d.p, items.data(), items.size() * sizeof(StructItem), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy items");
+    hipEventRecord(c->ev0, s);
+    for (size_t t = 0; t < items.size(); ++t) {
+        int rc = struct_launch(b, (const StructItem *)d.p + t, 1, 1, 0, s);
+        if (rc) return rc;
+    }
+    hipEventRecord(c->ev1, s);
+    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "plain structured kernel");
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (ms_out) *ms_out = ms;
+    return MOTEX_OK;
+}
+
+int struct_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x)
+{
+    const SeqSet &ss = b->ss;
+    double total = 0;
+    for (uint32_t i = first_x; i < first_x + num_x && i < ss.num; ++i) {
+        std::vector<std::pair<uint32_t, uint32_t>> pairs;
+        for (uint32_t j = 0; j < ss.num; ++j) pairs.push_back({i, j});
+        double ms = 0;
+        int rc = struct_launch_pairs(b, pairs, &ms);
+        if (rc) return rc;
+        total += ms;
+    }
+    b->stats.kernel_ms = total;
+    b->stats.main_kernel_ms = total;
+    return MOTEX_OK;
+}
+
+int struct_pair_run(motex_batch *b, uint32_t i, uint32_t j)
+{
+    std::vector<std::pair<uint32_t, uint32_t>> pairs{{i, j}};
+    double ms = 0;
+    int rc = struct_launch_pairs(b, pairs, &ms);
+    b->stats.kernel_ms = ms;
+    b->stats.main_kernel_ms = ms;
+    return rc;
+}
+
+}  // namespace motex

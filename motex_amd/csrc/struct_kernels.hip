@@ -1,0 +1,288 @@
+// Structured-motif kernels for gfx950 (structured_motifs_extraction_{hd,ed},
+// functions.cc:564-705 and 773-915).
+//
+// One persistent wave per workgroup takes one (x, y) pair at a time:
+//   1. per distinct box shape (length, errors) the FLASM hits of the pair go into a bitmap
+//      in the wave's scratch, word-major: bm[(pos_t >> 5) * m + pos_x], bit pos_t & 31.
+//      Hamming boxes use the diagonal sweep of the Hamming kernel (bit-sliced window
+//      counts; hits are rare and set bits with atomic OR); edit boxes use the strip
+//      wavefront of the edit kernel in bitmap mode (every word written once).  A box whose
+//      error bound is at least m passes everywhere, as the reference's B matrices, which
+//      hold m where FLASM returned no tuple (functions.cc:597-598), do.
+//   2. the merge of functions.cc:647-690: every box-0 hit (pos_t, pos_x) is gathered into
+//      an LDS list; for every x-side gap combination kx the y-side combinations that hit
+//      every box are counted -- explicit loops over all but the last gap, a popcount of a
+//      bit window of the last box's column for the last one.  v[kx*m + pos_x] adds the
+//      count; u[kx*m + pos_x] adds 1 once per pair, deduplicated with per-column stamps.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ed_device.h"
+#include "hd_device.h"
+#include "motex_internal.h"
+
+namespace motex {
+
+__device__ __forceinline__ uint32_t bm_bit(const uint32_t *bm, uint32_t m, uint32_t px, uint32_t pt)
+{
+    return (bm[(size_t)(pt >> 5) * m + px] >> (pt & 31)) & 1u;
+}
+
+// Set bits of column px over text positions [lo, hi] (inclusive).
+__device__ __forceinline__ uint32_t bm_window(const uint32_t *bm, uint32_t m, uint32_t px,
+                                              uint32_t lo, uint32_t hi)
+{
+    uint32_t cnt = 0;
+    for (uint32_t w = lo >> 5; w <= (hi >> 5); ++w) {
+        uint32_t word = bm[(size_t)w * m + px];
+        uint32_t b0 = w == (lo >> 5) ? (lo & 31) : 0;
+        uint32_t b1 = w == (hi >> 5) ? (hi & 31) : 31;
+        uint32_t mask = (b1 == 31 ? ~0u : ((1u << (b1 + 1)) - 1)) & ~((1u << b0) - 1);
+        cnt += __builtin_popcount(word & mask);
+    }
+    return cnt;
+}
+
+// Number of y-side gap combinations that complete the structured motif whose box 0 ends
+// at (pos_x = px0, pos_t = pt0) with x-side combination kx (functions.cc:653-685).
+__device__ uint32_t struct_walk(const StructArgs &a, uint32_t *const bm[kMaxBoxes], uint32_t m,
+                                uint32_t n, uint32_t kx, uint32_t px0, uint32_t pt0)
+{
+    const uint32_t G = a.nb_gaps;
+    uint32_t jx[kMaxGaps];
+    uint32_t px = px0;
+    for (uint32_t g = 0; g < G; ++g) {
+        px += a.S[(size_t)kx * G + g] + a.blen[g + 1];
+        if (px >= m) return 0;  // box g+1 does not fit in x
+        jx[g] = px;
+    }
+    // odometer over the y-side gaps 0 .. G-2; the last gap is a window popcount
+    uint32_t s[kMaxGaps], pt[kMaxGaps + 1];
+    uint32_t total = 0;
+    int g = 0;
+    pt[0] = pt0;
+    s[0] = a.gmin[0];
+    if (G == 1) {
+        uint32_t lo = pt0 + a.gmin[0] + a.blen[1], hi = pt0 + a.gmax[0] + a.blen[1];
+        if (lo >= n) return 0;
+        if (hi >= n) hi = n - 1;
+        return bm_window(bm[1], m, jx[0], lo, hi);
+    }
+    for (;;) {
+        if (s[g] > a.gmax[g]) {
+            if (g == 0) break;
+            --g;
+            ++s[g];
+            continue;
+        }
+        uint32_t p = pt[g] + s[g] + a.blen[g + 1];
+        if (p >= n) {  // this and every longer gap run past the text
+            if (g == 0) break;
+            --g;
+            ++s[g];
+            continue;
+        }
+        if (!bm_bit(bm[g + 1], m, jx[g], p)) { ++s[g]; continue; }
+        if (g + 2 == (int)G) {
+            uint32_t lo = p + a.gmin[g + 1] + a.blen[g + 2], hi = p + a.gmax[g + 1] + a.blen[g + 2];
+            if (lo < n) total += bm_window(bm[g + 2], m, jx[g + 1], lo, hi < n ? hi : n - 1);
+            ++s[g];
+            continue;
+        }
+        pt[g + 1] = p;
+        ++g;
+        s[g] = a.gmin[g];
+    }
+    return total;
+}
+
+// Hamming hits of one box shape for the pair, by the diagonal sweep, into bm.
+template <int P>
+__device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t len, uint32_t err,
+                              uint32_t *bm, uint32_t *ne_l, const uint16_t *xoff)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t m = I.m, n = I.n, wl = a.wl;
+    const uint32_t zero_row = a.sigma * wl;
+    int64_t q0 = (int64_t)I.y_start - (int64_t)(m - 1);
+    q0 = q0 >= 0 ? (q0 & ~31LL) : -(((-q0) + 31) & ~31LL);
+    const int64_t qend = (int64_t)I.y_start + n;  // diagonals below this can hold hits
+    for (int64_t qb = q0; qb < qend; qb += kDiagPerBlock) {
+        const uint32_t wbase = (uint32_t)((qb + (int64_t)kPadBits) / 32);
+        __syncthreads();
+        for (uint32_t c = 0; c < a.sigma; ++c) {
+            const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
+            for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
+        }
+        __syncthreads();
+        uint32_t cnt[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) cnt[p] = 0;
+        for (uint32_t px = 0; px < m; ++px) {
+            const uint32_t wn = xoff[px] + (px >> 5) + lane;
+            uint32_t wo, sho;
+            if (px >= len) {
+                wo = xoff[px - len] + ((px - len) >> 5) + lane;
+                sho = (px - len) & 31;
+            } else {
+                wo = zero_row + lane;
+                sho = 0;
+            }
+            uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 1], ne_l[wn], px & 31);
+            uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 1], ne_l[wo], sho);
+            uint32_t inc = an & ~bo, dec = bo & ~an;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                uint32_t cp = cnt[p];
+                cnt[p] = cp ^ (inc | dec);
+                uint32_t ni = cp & inc, nd = ~cp & dec;
+                inc = ni;
+                dec = nd;
+            }
+            uint32_t res = le_const<P, -1>(cnt, err);
+            if (px + 1 >= len && __ballot(res != 0)) {
+                uint32_t h = res;
+                while (h) {
+                    uint32_t t = __builtin_ctz(h);
+                    h &= h - 1;
+                    int64_t pt = qb + 32 * lane + t + px - (int64_t)I.y_start;
+                    if (pt + 1 < (int64_t)len || pt >= (int64_t)n) continue;
+                    atomicOr(bm + (size_t)((uint32_t)pt >> 5) * m + px, 1u << ((uint32_t)pt & 31));
+                }
+            }
+        }
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
+{
+    extern __shared__ uint32_t smem[];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t wl = a.wl;
+    // LDS: NE window (sigma+1 rows) | x offsets (m_max u16) | hit list | column stamps
+    uint32_t *ne_l = smem;
+    uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + (a.sigma + 1) * wl);
+    uint32_t *hits = smem + (a.sigma + 1) * wl + (a.m_max + 1) / 2;
+    uint32_t *stamps = hits + a.hcap;
+    uint32_t *hcount = stamps + a.m_max;
+    for (uint32_t w = lane; w < wl; w += 64) ne_l[a.sigma * wl + w] = 0;
+    for (uint32_t c = lane; c < a.m_max; c += 64) stamps[c] = 0;
+    uint32_t *scr = a.scratch + (size_t)blockIdx.x * a.scratch_words;
+    uint32_t *buf0 = scr + (size_t)a.ndist * a.bm_words;
+    uint32_t *buf1 = buf0 + a.stream_words;
+    uint32_t stamp_base = 1;
+
+    for (uint32_t round = 0;; ++round) {
+        uint32_t it;
+        if (a.persistent) {
+            it = 0;
+            if (lane == 0) it = atomicAdd(a.queue, 1u);
+            it = __builtin_amdgcn_readfirstlane(it);
+        } else {
+            it = round == 0 ? blockIdx.x : a.nitems;
+        }
+        if (it >= a.nitems) break;
+        const StructItem I = a.items[it];
+        const uint32_t m = I.m, n = I.n;
+        const uint32_t wn = (n + 31) / 32;
+        uint32_t *bmd[kMaxBoxes];
+        for (uint32_t d = 0; d < a.ndist; ++d) bmd[d] = scr + (size_t)d * wn * m;
+        uint32_t *bm[kMaxBoxes];
+        for (uint32_t b = 0; b <= a.nb_gaps; ++b) bm[b] = bmd[a.dist_of[b]];
+        __syncthreads();
+        if (a.distance == 0) {
+            for (uint32_t r = lane; r < m; r += 64)
+                xoff[r] = (uint16_t)(a.codes[I.x_start + r] * wl);
+        }
+        // 1. box bitmaps
+        for (uint32_t d = 0; d < a.ndist; ++d) {
+            const uint32_t len = a.dlen[d], err = a.derr[d];
+            uint32_t *out = bmd[d];
+            if (err >= m) {
+                // passes everywhere inside the pair (functions.cc:597-598, 651, 669)
+                for (size_t w = lane; w < (size_t)wn * m; w += 64) {
+                    uint32_t word = (uint32_t)(w / m);
+                    uint32_t rem = n - 32 * word;
+                    out[w] = rem >= 32 ? ~0u : ((1u << rem) - 1);
+                }
+                continue;
+            }
+            if (a.distance == 0) {
+                for (size_t w = lane; w < (size_t)wn * m; w += 64) out[w] = 0;
+                __threadfence();
+                __syncthreads();
+                struct_hd_box<P>(a, I, len, err, out, ne_l, xoff);
+            } else {
+                EdOut eo{out, m};
+                if (len <= 14)
+                    ed_pair<4, false, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
+                                            buf0, buf1, nullptr, nullptr, nullptr, nullptr, eo);
+                else
+                    ed_pair<2, true, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
+                                           buf0, buf1, nullptr, nullptr, nullptr, nullptr, eo);
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        // 2. merge: gather box-0 hits word row by word row, then count per x combination
+        if (lane == 0) *hcount = 0;
+        __syncthreads();
+        const uint32_t *b0 = bm[0];
+        const uint32_t chunks = (m + 63) / 64;
+        for (uint32_t wc = 0; wc <= wn * chunks; ++wc) {
+            // flush before a chunk that could overflow the list, and at the end
+            uint32_t have = __builtin_amdgcn_readfirstlane(*hcount);
+            bool flush = (wc == wn * chunks) ? have > 0 : have + 64 * 32 > a.hcap;
+            if (flush) {
+                for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
+                    const uint32_t stamp = stamp_base + kx;
+                    for (uint32_t h = lane; h < have; h += 64) {
+                        uint32_t e = hits[h];
+                        uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
+                        uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
+                        if (c) {
+                            uint32_t old = atomicExch(stamps + px0, stamp);
+                            size_t o = (size_t)I.uv_base + (size_t)kx * m + px0;
+                            if (old != stamp) atomicAdd(a.u + o, 1u);
+                            atomicAdd(a.v + o, c);
+                        }
+                    }
+                }
+                __syncthreads();
+                if (lane == 0) *hcount = 0;
+                __syncthreads();
+            }
+            if (wc == wn * chunks) break;
+            const uint32_t w = wc / chunks, px = (wc % chunks) * 64 + lane;
+            uint32_t word = px < m ? b0[(size_t)w * m + px] : 0u;
+            if (word) {
+                uint32_t off = atomicAdd(hcount, (uint32_t)__builtin_popcount(word));
+                while (word) {
+                    uint32_t t = __builtin_ctz(word);
+                    word &= word - 1;
+                    hits[off++] = (px << 16) | (32 * w + t);
+                }
+            }
+            __syncthreads();
+        }
+        stamp_base += a.nb_structs;
+    }
+}
+
+hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+{
+    uint32_t planes = 1;
+    for (uint32_t d = 0; d < a.ndist; ++d) {
+        uint32_t p = 32 - __builtin_clz(a.dlen[d]);
+        planes = p > planes ? p : planes;
+    }
+    if (planes <= 3) hipLaunchKernelGGL(struct_kernel<3>, dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 4) hipLaunchKernelGGL(struct_kernel<4>, dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 5) hipLaunchKernelGGL(struct_kernel<5>, dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 6) hipLaunchKernelGGL(struct_kernel<6>, dim3(grid), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL(struct_kernel<7>, dim3(grid), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace motex
