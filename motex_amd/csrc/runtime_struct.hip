@@ -15,7 +15,7 @@ static const uint32_t kStructHitCap = 4096;
 
 static size_t struct_lds_bytes(const StructArgs &a)
 {
-    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + a.m_max + 1) * 4;
+    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + 1) * 4;
 }
 
 // Fills the shape fields of StructArgs from the parameters.
@@ -139,7 +139,8 @@ static int struct_launch(motex_batch *b, const void *items, uint32_t nitems, uin
     a.hcap = kStructHitCap;
     a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
     a.stream_words = (maxlen + 128) * 2;
-    a.scratch_words = a.ndist * a.bm_words + 2ull * a.stream_words;
+    a.scratch_words = a.ndist * a.bm_words + 2ull * a.stream_words +
+                      ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1;
     uint64_t need = (uint64_t)grid * a.scratch_words * 4;
     hipError_t e;
     if (b->d_aux.bytes < need && (e = b->d_aux.alloc(need)) != hipSuccess) return hip_fail(e, "alloc structured scratch");
@@ -156,7 +157,8 @@ static uint32_t struct_grid(motex_batch *b, uint32_t nitems)
     const SeqSet &ss = b->ss;
     uint32_t maxlen = 0;
     for (uint32_t i = 0; i < ss.num; ++i) maxlen = std::max(maxlen, ss.len[i]);
-    uint64_t per = ((uint64_t)(maxlen + 31) / 32 * maxlen * (b->params.nb_gaps + 1) + 2ull * (maxlen + 128) * 2) * 4;
+    uint64_t per = ((uint64_t)(maxlen + 31) / 32 * maxlen * (b->params.nb_gaps + 1) + 2ull * (maxlen + 128) * 2 +
+                    ((uint64_t)b->nb_structs * maxlen + 31) / 32 + 1) * 4;
     uint64_t cap = std::max<uint64_t>(1, (2ull << 30) / std::max<uint64_t>(per, 1));
     uint64_t g = std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, nitems);
     return (uint32_t)std::max<uint64_t>(1, std::min(g, cap));

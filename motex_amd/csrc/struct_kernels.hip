@@ -13,7 +13,8 @@
 //      an LDS list; for every x-side gap combination kx the y-side combinations that hit
 //      every box are counted -- explicit loops over all but the last gap, a popcount of a
 //      bit window of the last box's column for the last one.  v[kx*m + pos_x] adds the
-//      count; u[kx*m + pos_x] adds 1 once per pair, deduplicated with per-column stamps.
+//      count; u[kx*m + pos_x] adds 1 once per pair, deduplicated with a per-pair bitmap of
+//      (kx, pos_x) entries already counted.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -160,18 +161,17 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x;
     const uint32_t wl = a.wl;
-    // LDS: NE window (sigma+1 rows) | x offsets (m_max u16) | hit list | column stamps
+    // LDS: NE window (sigma+1 rows) | x offsets (m_max u16) | hit list | count
     uint32_t *ne_l = smem;
     uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + (a.sigma + 1) * wl);
     uint32_t *hits = smem + (a.sigma + 1) * wl + (a.m_max + 1) / 2;
-    uint32_t *stamps = hits + a.hcap;
-    uint32_t *hcount = stamps + a.m_max;
+    uint32_t *hcount = hits + a.hcap;
     for (uint32_t w = lane; w < wl; w += 64) ne_l[a.sigma * wl + w] = 0;
-    for (uint32_t c = lane; c < a.m_max; c += 64) stamps[c] = 0;
+    // scratch: box bitmaps | two edit boundary streams | counted (kx, pos_x) bits
     uint32_t *scr = a.scratch + (size_t)blockIdx.x * a.scratch_words;
     uint32_t *buf0 = scr + (size_t)a.ndist * a.bm_words;
     uint32_t *buf1 = buf0 + a.stream_words;
-    uint32_t stamp_base = 1;
+    uint32_t *counted = buf1 + a.stream_words;
 
     for (uint32_t round = 0;; ++round) {
         uint32_t it;
@@ -226,6 +226,9 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
         __threadfence();
         __syncthreads();
         // 2. merge: gather box-0 hits word row by word row, then count per x combination
+        const uint32_t cwords = (a.nb_structs * m + 31) / 32;
+        for (uint32_t w = lane; w < cwords; w += 64) counted[w] = 0;
+        __threadfence();
         if (lane == 0) *hcount = 0;
         __syncthreads();
         const uint32_t *b0 = bm[0];
@@ -236,15 +239,15 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             bool flush = (wc == wn * chunks) ? have > 0 : have + 64 * 32 > a.hcap;
             if (flush) {
                 for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
-                    const uint32_t stamp = stamp_base + kx;
                     for (uint32_t h = lane; h < have; h += 64) {
                         uint32_t e = hits[h];
                         uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
                         uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
                         if (c) {
-                            uint32_t old = atomicExch(stamps + px0, stamp);
-                            size_t o = (size_t)I.uv_base + (size_t)kx * m + px0;
-                            if (old != stamp) atomicAdd(a.u + o, 1u);
+                            uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
+                            uint32_t old = atomicOr(counted + (idx >> 5), bit);
+                            size_t o = (size_t)I.uv_base + idx;
+                            if (!(old & bit)) atomicAdd(a.u + o, 1u);
                             atomicAdd(a.v + o, c);
                         }
                     }
@@ -266,7 +269,6 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             }
             __syncthreads();
         }
-        stamp_base += a.nb_structs;
     }
 }
 
