@@ -66,6 +66,24 @@ def cpu_baseline(w, gpu_u, gpu_v, off, nx):
             "parity_on_sample": parity}
 
 
+def cpu_baseline_prefix(ctx, w, params, seconds=12.0):
+    """One long sequence: the oracle (like the reference) gives a single pair one thread, so
+    a prefix sized for ~seconds of CPU work is timed, and the GPU checked on the same prefix."""
+    from oracle import oracle as O
+    O.build()
+    P = int(min(len(w.seqs[0]), (seconds * 1.0e9) ** 0.5))
+    pre = [w.seqs[0][:P]]
+    t0 = time.perf_counter()
+    ou, ov, _, _ = O.all_pairs(pre, w.distance, w.k, w.e, threads=1)
+    dt = time.perf_counter() - t0
+    gu, gv, _ = ctx.extract_all_pairs(pre, params)
+    cells = float((P - w.k + 1) * P)
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
+            "sample": f"self pair of the first {P} symbols ({cells:.3e} cells, {dt:.1f} s; one "
+                      "thread, as the reference gives a single sequence one thread)",
+            "parity_on_sample": bool(np.array_equal(ou, gu) and np.array_equal(ov, gv))}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -99,18 +117,20 @@ def main():
     # plain one-launch-per-pair form on a sample of x sequences (every y each)
     nplain = min(a.plain_x, len(w.seqs))
     L = np.array(w.lens, dtype=np.int64)
-    batch.clear()
-    tp = time.perf_counter()
-    pst = batch.run_plain(0, nplain)
-    tp = time.perf_counter() - tp
-    plain_cells = float(np.sum(np.maximum(L[:nplain] - w.k + 1, 0)) * np.sum(L[L >= w.k]))
-    plain = {"value": round(plain_cells / tp / 1e9, 4), "unit": "GCUPS",
-             "device_value": round(plain_cells / (pst["kernel_ms"] / 1e3) / 1e9, 4),
-             "sample": f"x 0..{nplain - 1} x all texts, {nplain * len(w.seqs)} launches"}
-    pu, pv, _ = batch.download()
-    end = int(off[nplain])
-    plain["parity_with_batched"] = bool(np.array_equal(pu[:end], gpu_u[:end]) and
-                                        np.array_equal(pv[:end], gpu_v[:end]))
+    plain = None
+    if nplain > 0:
+        batch.clear()
+        tp = time.perf_counter()
+        pst = batch.run_plain(0, nplain)
+        tp = time.perf_counter() - tp
+        plain_cells = float(np.sum(np.maximum(L[:nplain] - w.k + 1, 0)) * np.sum(L[L >= w.k]))
+        plain = {"value": round(plain_cells / tp / 1e9, 4), "unit": "GCUPS",
+                 "device_value": round(plain_cells / max(pst["kernel_ms"], 1e-6) * 1e3 / 1e9, 4),
+                 "sample": f"x 0..{nplain - 1} x all texts, {nplain * len(w.seqs)} launches"}
+        pu, pv, _ = batch.download()
+        end = int(off[nplain])
+        plain["parity_with_batched"] = bool(np.array_equal(pu[:end], gpu_u[:end]) and
+                                            np.array_equal(pv[:end], gpu_v[:end]))
 
     kms = float(np.mean(kernel_ms))
     ops = OPS_PER_CELL[w.distance]
@@ -144,9 +164,11 @@ def main():
                    "work_items": st["work_items"], "upload_s": round(t_up, 3)},
         "roofline": roofline,
         "plain": plain,
-        "speedup_vs_plain": round(value / plain["value"], 2) if plain["value"] else None,
+        "speedup_vs_plain": round(value / plain["value"], 2) if plain and plain["value"] else None,
     }
-    if not a.no_cpu:
+    if not a.no_cpu and len(w.seqs) == 1:
+        out["cpu_baseline"] = cpu_baseline_prefix(ctx, w, params)
+    elif not a.no_cpu:
         nx = a.cpu_x
         if nx <= 0:
             cores = len(os.sched_getaffinity(0))

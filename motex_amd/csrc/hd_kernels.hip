@@ -50,14 +50,18 @@ __global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n, uint32_t v
     for (; i < n; i += stride) p[i] = val;
 }
 
-// Folds the hit flags of one row into u/v.  Wave-uniform: all lanes call it together.
+// Folds the hit flags of one row into the item's per-row LDS accumulators.  Wave-uniform:
+// all lanes call it together.  Hits come in increasing text position (block, lane, bit)
+// and a whole text lies in one range, so within a row a text is new unless it equals the
+// last one counted.  Ranges that are parts of one long text count it at most once per row.
 template <int B>
-__device__ __forceinline__ void fold_row(const HdArgs &a, const HdItem &I, uint32_t px,
-                                                   const uint32_t (&res)[B])
+__device__ __forceinline__ void fold_row(const HdArgs &a, const HdItem &I, const HdRange &R, int32_t q0,
+                                         uint32_t px, uint32_t r, const uint32_t (&res)[B],
+                                         uint32_t *acc_v, uint32_t *acc_u)
 {
     const uint32_t lane = threadIdx.x;
     uint32_t du = 0, dv = 0;
-    int last_y = -1;
+    int lasty = -1;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         uint32_t h = res[b];
@@ -66,9 +70,9 @@ __device__ __forceinline__ void fold_row(const HdArgs &a, const HdItem &I, uint3
         while (h) {
             uint32_t t = __builtin_ctz(h);
             h &= h - 1;
-            int64_t pt = (int64_t)I.q0 + (int64_t)(kDiagPerBlock * b + 32 * lane + t) + px;
-            if (pt < (int64_t)I.ys || pt >= (int64_t)I.ye) continue;
-            uint32_t y = I.ny == 1 ? I.j0 : find_seq(a.seq_start, I.j0, I.ny, (uint32_t)pt);
+            int64_t pt = (int64_t)q0 + (int64_t)(kDiagPerBlock * b + 32 * lane + t) + px;
+            if (pt < (int64_t)R.ys || pt >= (int64_t)R.ye) continue;
+            uint32_t y = R.ny == 1 ? R.j0 : find_seq(a.seq_start, R.j0, R.ny, (uint32_t)pt);
             if ((uint32_t)pt - a.seq_start[y] + 1 < a.k) continue;  // window starts before y
             cnt++;
             if ((int)y != ly) { dist++; if (fy < 0) fy = (int)y; ly = (int)y; }
@@ -82,19 +86,14 @@ __device__ __forceinline__ void fold_row(const HdArgs &a, const HdItem &I, uint3
             int lf = __builtin_amdgcn_readlane(fy, L);
             int ll = __builtin_amdgcn_readlane(ly, L);
             dv += lc;
-            du += ld - (lf == last_y ? 1u : 0u);
-            last_y = ll;
+            du += ld - (lf == lasty ? 1u : 0u);
+            lasty = ll;
         }
     }
     if (dv && lane == 0) {
-        if (I.split != kNoSplit) {
-            uint32_t xf = I.xf_base + px;
-            uint32_t bit = 1u << (xf & 31);
-            uint32_t old = atomicOr(a.split_flags + (size_t)I.split * a.split_words + (xf >> 5), bit);
-            du = (old & bit) ? 0u : 1u;
-        }
-        if (du) atomicAdd(a.u + I.uv_base + px, du);
-        atomicAdd(a.v + I.uv_base + px, dv);
+        acc_v[r] += dv;
+        if (I.split != kNoSplit) acc_u[r] = 1;
+        else acc_u[r] += du;
     }
 }
 
@@ -105,8 +104,11 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x;
     const uint32_t wl = a.wl;
+    // LDS: NE window (sigma + 1 rows, the last all zero) | per-row u, v | x offsets
     uint32_t *ne_l = smem;
-    uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + (a.sigma + 1) * wl);
+    uint32_t *acc_v = smem + (a.sigma + 1) * wl;
+    uint32_t *acc_u = acc_v + a.rows_max;
+    uint16_t *xoff = reinterpret_cast<uint16_t *>(acc_u + a.rows_max);
     const uint32_t zero_row = a.sigma * wl;
     for (uint32_t w = lane; w < wl; w += 64) ne_l[zero_row + w] = 0;
 
@@ -122,55 +124,81 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
         if (it >= a.nitems) break;
         const HdItem I = a.items[it];
         const uint32_t nrows = I.r1 - I.px_start;
-        const uint32_t wbase = I.wbase;
-        __syncthreads();  // previous item finished reading LDS
-        for (uint32_t c = 0; c < a.sigma; ++c) {
-            const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
-            for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
-        }
-        for (uint32_t r = lane; r < nrows; r += 64)
+        __syncthreads();  // previous item finished with LDS
+        for (uint32_t r = lane; r < nrows; r += 64) {
             xoff[r] = (uint16_t)(a.codes[I.x_start + I.px_start + r] * wl);
-        __syncthreads();
-
-        uint32_t cnt[B][P];
-#pragma unroll
-        for (int b = 0; b < B; ++b)
-#pragma unroll
-            for (int p = 0; p < P; ++p) cnt[b][p] = 0;
-
+            acc_v[r] = 0;
+            acc_u[r] = 0;
+        }
         const uint32_t rep0 = I.r0 > a.k - 1 ? I.r0 : a.k - 1;
         const uint32_t pxs5 = I.px_start >> 5;
-        for (uint32_t r = 0; r < nrows; ++r) {
-            const uint32_t px = I.px_start + r;
-            const uint32_t wn = xoff[r] + (px >> 5) - pxs5 + lane;
-            uint32_t wo, sho;
-            if (r >= a.k) {
-                wo = xoff[r - a.k] + ((px - a.k) >> 5) - pxs5 + lane;
-                sho = (px - a.k) & 31;
-            } else {
-                wo = zero_row + lane;
-                sho = 0;
+        for (uint32_t ri = I.rb; ri < I.re; ++ri) {
+            const HdRange R = a.ranges[ri];
+            // first diagonal (pt - px) of this range, a multiple of 32
+            const int32_t q0 = ((int32_t)R.ys - (int32_t)I.r1 + 1) & ~31;
+            const uint32_t wbase = (uint32_t)(((q0 + (int32_t)kPadBits) >> 5) + (int32_t)pxs5);
+            __syncthreads();
+            for (uint32_t c = 0; c < a.sigma; ++c) {
+                const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
+                for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
             }
-            const uint32_t shn = px & 31;
-            uint32_t res[B];
-            uint32_t any = 0;
+            __syncthreads();
+
+            uint32_t cnt[B][P];
 #pragma unroll
-            for (int b = 0; b < B; ++b) {
-                uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 64 * b + 1], ne_l[wn + 64 * b], shn);
-                uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 64 * b + 1], ne_l[wo + 64 * b], sho);
-                uint32_t inc = an & ~bo, dec = bo & ~an;
+            for (int b = 0; b < B; ++b)
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    uint32_t cp = cnt[b][p];
-                    cnt[b][p] = cp ^ (inc | dec);
-                    uint32_t ni = cp & inc, nd = ~cp & dec;
-                    inc = ni;
-                    dec = nd;
+                for (int p = 0; p < P; ++p) cnt[b][p] = 0;
+
+            for (uint32_t r = 0; r < nrows; ++r) {
+                const uint32_t px = I.px_start + r;
+                const uint32_t wn = xoff[r] + (px >> 5) - pxs5 + lane;
+                uint32_t wo, sho;
+                if (r >= a.k) {
+                    wo = xoff[r - a.k] + ((px - a.k) >> 5) - pxs5 + lane;
+                    sho = (px - a.k) & 31;
+                } else {
+                    wo = zero_row + lane;
+                    sho = 0;
                 }
-                res[b] = le_const<P, E>(cnt[b], a.e);
-                any |= res[b];
+                const uint32_t shn = px & 31;
+                uint32_t res[B];
+                uint32_t any = 0;
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 64 * b + 1], ne_l[wn + 64 * b], shn);
+                    uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 64 * b + 1], ne_l[wo + 64 * b], sho);
+                    uint32_t inc = an & ~bo, dec = bo & ~an;
+#pragma unroll
+                    for (int p = 0; p < P; ++p) {
+                        uint32_t cp = cnt[b][p];
+                        cnt[b][p] = cp ^ (inc | dec);
+                        uint32_t ni = cp & inc, nd = ~cp & dec;
+                        inc = ni;
+                        dec = nd;
+                    }
+                    res[b] = le_const<P, E>(cnt[b], a.e);
+                    any |= res[b];
+                }
+                if (px >= rep0 && __ballot(any != 0))
+                    fold_row<B>(a, I, R, q0, px, r, res, acc_v, acc_u);
             }
-            if (px >= rep0 && __ballot(any != 0)) fold_row<B>(a, I, px, res);
+        }
+        __syncthreads();
+        // flush the item's rows; a long text split over several items counts once per row
+        for (uint32_t r = lane; r < nrows; r += 64) {
+            uint32_t dv = acc_v[r];
+            if (!dv) continue;
+            uint32_t du = acc_u[r];
+            const uint32_t px = I.px_start + r;
+            if (I.split != kNoSplit && du) {
+                uint32_t xf = I.xf_base + px;
+                uint32_t bit = 1u << (xf & 31);
+                uint32_t old = atomicOr(a.split_flags + (size_t)I.split * a.split_words + (xf >> 5), bit);
+                du = (old & bit) ? 0u : 1u;
+            }
+            if (du) atomicAdd(a.u + I.uv_base + px, du);
+            atomicAdd(a.v + I.uv_base + px, dv);
         }
     }
 }
@@ -211,7 +239,7 @@ static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream
 
 size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max)
 {
-    return (size_t)(sigma + 1) * wl * 4 + (size_t)((rows_max + 1) & ~1u) * 2;
+    return (size_t)(sigma + 1) * wl * 4 + (size_t)rows_max * 8 + (size_t)((rows_max + 1) & ~1u) * 2;
 }
 
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)

@@ -12,20 +12,24 @@ constexpr uint32_t kMaxHdK = 64;             // Hamming kernels: counters of up 
 constexpr uint32_t kMaxEdK = 32;             // edit kernels: bit-vectors stay below 64 bits
 constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
 
-// One unit of the Hamming work queue: rows [r0, r1) of sequence x against the text
-// range [ys, ye) (whole sequences j0 .. j0+ny-1, or one part of a long sequence j0).
+// A text range of the Hamming work queue: whole sequences j0 .. j0+ny-1, or one part of a
+// long sequence j0 (ny = 1), covering text positions [ys, ye).
+struct HdRange {
+    uint32_t ys, ye, j0, ny;
+};
+
+// One unit of the Hamming work queue: rows [r0, r1) of sequence x against the text ranges
+// [rb, re), processed in order by one wave that keeps per-row u/v in LDS.
 struct HdItem {
     uint32_t x_start;   // text position of x's first symbol
     uint32_t uv_base;   // u/v index of x's pos_x = 0
+    uint32_t xf_base;   // global factor index of (x, pos_x = 0): row flags for split texts
     uint32_t r0, r1;    // reported rows (positions in x)
     uint32_t px_start;  // first computed row = max(0, r0 - k + 1)
-    int32_t q0;         // first diagonal (pt - px), a multiple of 32
-    uint32_t ys, ye;    // text range whose windows are reported by this item
-    uint32_t j0, ny;    // sequence range
-    uint32_t split;     // kNoSplit, or the split slot of sequence j0
-    uint32_t xf_base;   // global factor index of (x, pos_x = 0): row flags for split texts
-    uint32_t wbase;     // first NE word the item reads: floor((q0 + kPadBits)/32) + px_start/32
-    uint32_t pad_;
+    uint32_t rb, re;    // text ranges
+    uint32_t split;     // kNoSplit (whole texts only) or the split slot of the one long text
+                        // every range of the item is a part of
+    uint32_t pad[3];
 };
 
 struct HdArgs {
@@ -33,6 +37,7 @@ struct HdArgs {
     const uint8_t *codes;      // symbol codes of the concatenated text
     const uint32_t *seq_start; // num_seqs + 1 entries
     const HdItem *items;
+    const HdRange *ranges;
     uint32_t *u;
     uint32_t *v;
     uint32_t *split_flags;     // split slots x split_words
@@ -43,7 +48,7 @@ struct HdArgs {
     uint32_t k, e;
     uint32_t wl;               // LDS words per symbol row
     uint32_t split_words;
-    uint32_t rows_max;         // largest r1 - px_start of any item
+    uint32_t rows_max;         // largest r1 - px_start of any item (LDS row arrays)
     uint32_t persistent;       // 1: pull items from *queue; 0: item = blockIdx.x
 };
 

@@ -60,14 +60,15 @@ struct SeqSet {
 // Tile shape of the batched kernels (picked by the autotuner per alphabet/d/k/e).
 struct TileShape {
     uint32_t blocks = 4;         // Hamming: 2048-diagonal blocks per wave
-    uint32_t rows = 512;         // Hamming: x rows per work item
-    uint32_t waves_per_cu = 16;  // persistent grid = num_cu * waves_per_cu
+    uint32_t rows = 256;         // Hamming: x rows per work item
+    uint32_t waves_per_cu = 24;  // persistent grid = num_cu * waves_per_cu
     uint32_t variant = 0;        // edit kernels: 0 = 32-bit keys, 1 = 64-bit words
     uint32_t ytile = 0;          // edit kernels: text symbols per work item
 };
 
 struct HdPlan {
     std::vector<HdItem> items;
+    std::vector<HdRange> ranges;
     uint32_t blocks = 4, rows = 512, rows_max = 0, wl = 0, W = 0, split_slots = 0;
 };
 
@@ -119,7 +120,7 @@ struct motex_batch {
     uint32_t ed_nitems = 0;
     motex_stats stats{};
     hipStream_t stream = nullptr;   // overrides ctx->stream when set (serving)
-    motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2;
+    motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2, d_ranges;
     uint64_t work_items() const { return hd.items.size() + ed_nitems; }
 };
 
@@ -129,7 +130,7 @@ std::vector<uint32_t> enumerate_structs(const motex_params *p, uint32_t *nb_stru
 uint32_t total_motif_length(const motex_params *p);
 void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
                    const std::vector<uint32_t> &xs, const std::vector<int32_t> *only_y,
-                   HdPlan &plan);
+                   HdPlan &plan, uint32_t group_max);
 TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss);
 
 // kernels (hd_kernels.hip, ed_kernels.hip)

@@ -145,70 +145,116 @@ static int64_t floor32(int64_t v) { return v >= 0 ? (v & ~31LL) : -(((-v) + 31) 
 
 void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
                    const std::vector<uint32_t> &xs, const std::vector<int32_t> *only_y,
-                   HdPlan &plan)
+                   HdPlan &plan, uint32_t group_max)
 {
     const int64_t cap = (int64_t)kDiagPerBlock * blocks - rows - 32;
     plan.items.clear();
+    plan.ranges.clear();
     plan.blocks = blocks;
     plan.rows = rows;
     plan.rows_max = rows + k - 1;
-    // text ranges: whole sequences packed up to cap positions, long ones split
-    struct Range { uint32_t ys, ye, j0, ny, split; };
-    std::vector<Range> ranges;
+    plan.split_slots = 0;
+    // text ranges: whole sequences packed up to cap positions, long ones split in parts
+    std::vector<uint32_t> split_of;  // per range: kNoSplit or slot
     auto add_seq_ranges = [&](uint32_t j) {
         uint32_t n = ss.len[j];
         if (n < k) return;
-        if ((int64_t)n <= cap) { ranges.push_back({ss.start[j], ss.start[j] + n, j, 1, kNoSplit}); return; }
+        if ((int64_t)n <= cap) {
+            plan.ranges.push_back({ss.start[j], ss.start[j] + n, j, 1});
+            split_of.push_back(kNoSplit);
+            return;
+        }
         uint32_t slot = plan.split_slots++;
         for (uint32_t p = 0; p < n; p += (uint32_t)cap) {
             uint32_t e = std::min<uint32_t>(n, p + (uint32_t)cap);
-            ranges.push_back({ss.start[j] + p, ss.start[j] + e, j, 1, slot});
+            plan.ranges.push_back({ss.start[j] + p, ss.start[j] + e, j, 1});
+            split_of.push_back(slot);
         }
     };
-    plan.split_slots = 0;
     if (only_y) {
         for (int32_t j : *only_y) add_seq_ranges((uint32_t)j);
     } else {
-        // pack consecutive eligible whole sequences
         uint32_t j = 0;
         while (j < ss.num) {
             if (ss.len[j] < k) { ++j; continue; }
             if ((int64_t)ss.len[j] > cap) { add_seq_ranges(j); ++j; continue; }
-            Range r{ss.start[j], ss.start[j] + ss.len[j], j, 1, kNoSplit};
+            HdRange r{ss.start[j], ss.start[j] + ss.len[j], j, 1};
             uint32_t jj = j + 1;
             while (jj < ss.num && (int64_t)ss.len[jj] <= cap &&
                    (int64_t)(ss.start[jj] + ss.len[jj]) - r.ys <= cap) {
                 if (ss.len[jj] >= k) { r.ye = ss.start[jj] + ss.len[jj]; r.ny = jj - j + 1; }
                 ++jj;
             }
-            // trailing ineligible sequences are not part of the range
-            ranges.push_back(r);
+            plan.ranges.push_back(r);
+            split_of.push_back(kNoSplit);
             j = r.j0 + r.ny;
             while (j < jj && ss.len[j] < k) ++j;
-            if (j < r.j0 + r.ny) j = r.j0 + r.ny;
         }
+    }
+    // groups of consecutive ranges: whole texts only, or parts of one long text
+    struct Group { uint32_t rb, re, split; };
+    std::vector<Group> groups;
+    for (uint32_t r = 0; r < plan.ranges.size();) {
+        uint32_t e = r + 1;
+        while (e < plan.ranges.size() && e - r < group_max && split_of[e] == split_of[r]) ++e;
+        groups.push_back({r, e, split_of[r]});
+        r = e;
     }
     for (uint32_t i : xs) {
         uint32_t m = ss.len[i];
         if (m < k) continue;
         for (uint32_t r0 = k - 1; r0 < m; r0 += rows) {
             uint32_t r1 = std::min(m, r0 + rows);
-            uint32_t pxs = r0 - (k - 1);
-            for (const Range &R : ranges) {
+            for (const Group &g : groups) {
                 HdItem it{};
                 it.x_start = ss.start[i];
                 it.uv_base = (uint32_t)ss.uv_off[i];
-                it.r0 = r0; it.r1 = r1; it.px_start = pxs;
-                int64_t q0 = floor32((int64_t)R.ys - (int64_t)r1 + 1);
-                it.q0 = (int32_t)q0;
-                it.ys = R.ys; it.ye = R.ye; it.j0 = R.j0; it.ny = R.ny; it.split = R.split;
                 it.xf_base = (uint32_t)ss.uv_off[i];
-                int64_t wb = (q0 + (int64_t)kPadBits) / 32 + (pxs >> 5);  // q0 + kPadBits is a multiple of 32
-                it.wbase = wb < 0 ? 0x80000000u : (uint32_t)wb;
+                it.r0 = r0;
+                it.r1 = r1;
+                it.px_start = r0 - (k - 1);
+                it.rb = g.rb;
+                it.re = g.re;
+                it.split = g.split;
                 plan.items.push_back(it);
             }
         }
     }
+}
+
+// First NE word an (item, range) reads; mirrors the kernel.
+static int64_t hd_wbase(const HdItem &it, const HdRange &R)
+{
+    int64_t q0 = ((int64_t)R.ys - (int64_t)it.r1 + 1);
+    q0 = q0 >= 0 ? (q0 & ~31LL) : -(((-q0) + 31) & ~31LL);
+    int64_t qp = q0 + (int64_t)kPadBits;
+    int64_t w = qp >= 0 ? qp / 32 : -((-qp + 31) / 32);
+    return w + (it.px_start >> 5);
+}
+
+// Upload ranges + items; size the NE tables for every window the items read.
+static int hd_upload_plan(motex_batch *b, HdPlan &plan, DevBuf &d_items, DevBuf &d_ranges, uint64_t *Wout)
+{
+    hipError_t e;
+    uint64_t W = (b->ss.total + kPadBits) / 32 + 2;
+    for (const HdItem &it : plan.items)
+        for (uint32_t r = it.rb; r < it.re; ++r) {
+            int64_t wb = hd_wbase(it, plan.ranges[r]);
+            if (wb < 0) return set_error(MOTEX_EINVAL, "internal: negative table window");
+            W = std::max<uint64_t>(W, (uint64_t)wb + plan.wl + 1);
+        }
+    *Wout = W;
+    if ((e = d_items.alloc(std::max<size_t>(plan.items.size(), 1) * sizeof(HdItem))) != hipSuccess)
+        return hip_fail(e, "alloc items");
+    if (!plan.items.empty() &&
+        (e = hipMemcpy(d_items.p, plan.items.data(), plan.items.size() * sizeof(HdItem), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy items");
+    if ((e = d_ranges.alloc(std::max<size_t>(plan.ranges.size(), 1) * sizeof(HdRange))) != hipSuccess)
+        return hip_fail(e, "alloc ranges");
+    if (!plan.ranges.empty() &&
+        (e = hipMemcpy(d_ranges.p, plan.ranges.data(), plan.ranges.size() * sizeof(HdRange), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "copy ranges");
+    return MOTEX_OK;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -289,32 +335,40 @@ int hd_prepare(motex_batch *b)
     b->tile = ts;
     std::vector<uint32_t> xs(ss.num);
     for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
-    plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd);
+    // group size: aim at ~16 items per resident wave so the tail of the queue is short
+    plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, 1u << 30);
+    uint64_t pairs_units = 0;  // tiles x ranges
+    {
+        uint64_t ntiles = 0;
+        for (uint32_t i = 0; i < ss.num; ++i)
+            if (ss.len[i] >= b->params.k) ntiles += (ss.len[i] - b->params.k + 1 + ts.rows - 1) / ts.rows;
+        pairs_units = ntiles * b->hd.ranges.size();
+    }
+    uint64_t target_items = (uint64_t)c->num_cu * ts.waves_per_cu * 16;
+    uint32_t gmax = (uint32_t)std::max<uint64_t>(1, pairs_units / std::max<uint64_t>(target_items, 1));
+    plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, gmax);
     // longest items first: they set the tail of the persistent schedule
-    std::stable_sort(b->hd.items.begin(), b->hd.items.end(), [](const HdItem &a, const HdItem &z) {
-        uint64_t ca = (uint64_t)(a.r1 - a.px_start) * (a.ye - a.ys + a.r1 - a.r0);
-        uint64_t cz = (uint64_t)(z.r1 - z.px_start) * (z.ye - z.ys + z.r1 - z.r0);
-        return ca > cz;
+    std::vector<uint64_t> span(b->hd.ranges.size() + 1, 0);
+    for (size_t r = 0; r < b->hd.ranges.size(); ++r) span[r + 1] = span[r] + (b->hd.ranges[r].ye - b->hd.ranges[r].ys);
+    std::stable_sort(b->hd.items.begin(), b->hd.items.end(), [&](const HdItem &x, const HdItem &z) {
+        uint64_t cx = (uint64_t)(x.r1 - x.px_start) * (span[x.re] - span[x.rb] + (x.re - x.rb) * (x.r1 - x.r0));
+        uint64_t cz = (uint64_t)(z.r1 - z.px_start) * (span[z.re] - span[z.rb] + (z.re - z.rb) * (z.r1 - z.r0));
+        return cx > cz;
     });
     b->hd.wl = hd_lds_words(ts.blocks, b->hd.rows_max);
     if (hd_lds_bytes(ss.sigma, b->hd.wl, b->hd.rows_max) > 65536 || (uint64_t)ss.sigma * b->hd.wl >= 65536)
         return set_error(MOTEX_EUNSUPPORTED, "alphabet of %u symbols does not fit the kernel's LDS window", ss.sigma);
-    for (const HdItem &it : b->hd.items)
-        if ((int32_t)it.wbase < 0) return set_error(MOTEX_EINVAL, "internal: negative table window");
-    // NE words: cover every item's LDS window
-    uint64_t W = (ss.total + kPadBits) / 32 + 2;
-    for (const HdItem &it : b->hd.items) W = std::max<uint64_t>(W, (uint64_t)it.wbase + b->hd.wl + 1);
+    uint64_t W = 0;
+    int rc = hd_upload_plan(b, b->hd, b->d_items, b->d_ranges, &W);
+    if (rc) return rc;
+    // the plain form's windows (one block, short row tiles) must fit as well
+    W = std::max<uint64_t>(W, (ss.total + kPadBits) / 32 + 4 * 64 + 64);
     b->hd.W = (uint32_t)W;
     hipError_t e;
     if ((e = b->d_ne.alloc((uint64_t)W * ss.sigma * 4)) != hipSuccess) return hip_fail(e, "alloc NE tables");
     if ((e = launch_ne_build((const uint8_t *)b->d_codes.p, (uint32_t)ss.total, ss.sigma, (uint32_t)W,
                              (uint32_t *)b->d_ne.p, c->stream)) != hipSuccess)
         return hip_fail(e, "NE build");
-    uint64_t ni = std::max<size_t>(b->hd.items.size(), 1);
-    if ((e = b->d_items.alloc(ni * sizeof(HdItem))) != hipSuccess) return hip_fail(e, "alloc items");
-    if (!b->hd.items.empty() &&
-        (e = hipMemcpy(b->d_items.p, b->hd.items.data(), b->hd.items.size() * sizeof(HdItem), hipMemcpyHostToDevice)) != hipSuccess)
-        return hip_fail(e, "copy items");
     b->split_words = (uint32_t)((ss.uv_off[ss.num] + 31) / 32 + 1);
     uint64_t sf = (uint64_t)std::max<uint32_t>(b->hd.split_slots, 1) * b->split_words;
     if ((e = b->d_split.alloc(sf * 4)) != hipSuccess) return hip_fail(e, "alloc split flags");
@@ -329,6 +383,7 @@ HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t
     a.codes = (const uint8_t *)b->d_codes.p;
     a.seq_start = (const uint32_t *)b->d_seq_start.p;
     a.items = (const HdItem *)items.p;
+    a.ranges = (const HdRange *)b->d_ranges.p;
     a.u = (uint32_t *)b->d_u.p;
     a.v = (uint32_t *)b->d_v.p;
     a.split_flags = (uint32_t *)b->d_split.p;
@@ -400,38 +455,47 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
     const SeqSet &ss = b->ss;
     const uint32_t k = b->params.k;
     struct Launch { uint32_t first, count, blocks; };
-    std::vector<HdItem> all;
+    HdPlan all, pp;
     std::vector<Launch> launches;
     uint32_t slots = 0;
-    HdPlan pp;
     for (auto &pr : pairs) {
         uint32_t i = pr.first, j = pr.second;
         if (ss.len[i] < k || ss.len[j] < k) continue;
         std::vector<int32_t> oy{(int32_t)j};
         std::vector<uint32_t> xs{i};
         uint32_t blocks = plain_blocks(ss.len[j]);
-        plan_hd_items(ss, k, blocks, kPlainRows, xs, &oy, pp);
-        for (HdItem &it : pp.items) if (it.split != kNoSplit) it.split = slots;  // one slot per launch
+        plan_hd_items(ss, k, blocks, kPlainRows, xs, &oy, pp, 1u << 30);
+        uint32_t roff = (uint32_t)all.ranges.size();
+        for (HdItem &it : pp.items) {
+            it.rb += roff;
+            it.re += roff;
+            if (it.split != kNoSplit) it.split = slots;  // one slot per launch
+        }
         if (pp.split_slots) slots++;
-        launches.push_back({(uint32_t)all.size(), (uint32_t)pp.items.size(), blocks});
-        all.insert(all.end(), pp.items.begin(), pp.items.end());
+        launches.push_back({(uint32_t)all.items.size(), (uint32_t)pp.items.size(), blocks});
+        all.items.insert(all.items.end(), pp.items.begin(), pp.items.end());
+        all.ranges.insert(all.ranges.end(), pp.ranges.begin(), pp.ranges.end());
     }
     if (ms_out) *ms_out = 0;
-    if (all.empty()) return MOTEX_OK;
-    DevBuf items, flags;
-    if ((e = items.alloc(all.size() * sizeof(HdItem))) != hipSuccess) return hip_fail(e, "alloc plain items");
-    if ((e = hipMemcpy(items.p, all.data(), all.size() * sizeof(HdItem), hipMemcpyHostToDevice)) != hipSuccess)
-        return hip_fail(e, "copy plain items");
+    if (all.items.empty()) return MOTEX_OK;
+    all.rows_max = kPlainRows + k - 1;
+    all.wl = hd_lds_words(4, all.rows_max);
+    DevBuf items, ranges, flags;
+    uint64_t W = 0;
+    int rc = hd_upload_plan(b, all, items, ranges, &W);
+    if (rc) return rc;
+    if (W > b->hd.W) return set_error(MOTEX_EINVAL, "internal: plain window beyond the NE tables");
     if ((e = flags.alloc((uint64_t)std::max<uint32_t>(slots, 1) * b->split_words * 4)) != hipSuccess)
         return hip_fail(e, "alloc plain flags");
     if ((e = launch_fill_u32((uint32_t *)flags.p, flags.bytes / 4, 0, s)) != hipSuccess) return hip_fail(e, "clear flags");
     HdPlan shape;
-    shape.rows_max = kPlainRows + k - 1;
+    shape.rows_max = all.rows_max;
     hipEventRecord(c->ev0, s);
     for (const Launch &L : launches) {
         shape.wl = hd_lds_words(L.blocks, shape.rows_max);
         HdArgs a = hd_args(b, shape, items, 0);
         a.items = (const HdItem *)items.p + L.first;
+        a.ranges = (const HdRange *)ranges.p;
         a.nitems = L.count;
         a.split_flags = (uint32_t *)flags.p;
         if ((e = launch_hd(a, L.blocks, L.count, s)) != hipSuccess) return hip_fail(e, "launch plain kernel");
