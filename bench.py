@@ -41,8 +41,9 @@ def parse():
     ap.add_argument("--cpu-x", type=int, default=0,
                     help="x sequences in the CPU baseline sample (0: sized for ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving", action="store_true", help="also run the serving stream")
-    ap.add_argument("--serving-jobs", type=int, default=64)
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving stream (p50/p99 job latency)")
+    ap.add_argument("--serving-jobs", type=int, default=256)
     return ap.parse_args()
 
 
@@ -176,9 +177,14 @@ def main():
             rate = 0.4e9 * cores if w.distance == 0 else 0.06e9 * cores
             nx = int(max(1, min(len(w.seqs), 15.0 * rate / per_x)))
         out["cpu_baseline"] = cpu_baseline(w, gpu_u, gpu_v, off, nx)
-    if a.serving:
+    if not a.no_serving:
         from motex_amd import serving
-        out["serving"] = serving.run_stream(ctx, njobs=a.serving_jobs)
+        sv = serving.run_stream(ctx, njobs=a.serving_jobs)
+        sv["workload"] = ("serving: jobs of 20-400 DNA/PROT sequences of 100-2000 symbols, k 8-14, "
+                          "e 0-3, d 0/1, q 20-80, Poisson arrivals at 70% of measured capacity")
+        out["serving"] = sv
+        out["p50_job_latency_ms"] = sv["p50_latency_ms"]
+        out["p99_job_latency_ms"] = sv["p99_latency_ms"]
     batch.close()
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -22,10 +22,21 @@ int set_error(int code, const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);
 int validate_params(const motex_params *p);
 
+// Stream whose stream-ordered allocator DevBuf uses on this thread (serving jobs), or null
+// for plain hipMalloc/hipFree.
+extern thread_local hipStream_t g_alloc_stream;
+
+struct AllocStreamGuard {
+    hipStream_t prev;
+    explicit AllocStreamGuard(hipStream_t s) : prev(g_alloc_stream) { g_alloc_stream = s; }
+    ~AllocStreamGuard() { g_alloc_stream = prev; }
+};
+
 // Device allocation owned by a batch or job.
 struct DevBuf {
     void *p = nullptr;
     uint64_t bytes = 0;
+    hipStream_t s = nullptr;  // allocated stream-ordered on this stream
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -33,14 +44,18 @@ struct DevBuf {
     hipError_t alloc(uint64_t n)
     {
         release();
-        hipError_t e = hipMalloc(&p, n ? n : 4);
+        s = g_alloc_stream;
+        hipError_t e = s ? hipMallocAsync(&p, n ? n : 4, s) : hipMalloc(&p, n ? n : 4);
         if (e != hipSuccess) { p = nullptr; return e; }
         bytes = n;
         return hipSuccess;
     }
     void release()
     {
-        if (p) hipFree(p);
+        if (p) {
+            if (s) hipFreeAsync(p, s);
+            else hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
     }
@@ -118,13 +133,27 @@ struct motex_batch {
     motex::EdPlan ed;
     uint32_t split_words = 0;
     uint32_t ed_nitems = 0;
+    uint32_t grid_cap = 0;           // serving: bound on the persistent grid (0 = none)
     motex_stats stats{};
     hipStream_t stream = nullptr;   // overrides ctx->stream when set (serving)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // run start, end, main kernel start
+    ~motex_batch()
+    {
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (ev2) hipEventDestroy(ev2);
+    }
     motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2, d_ranges;
     uint64_t work_items() const { return hd.items.size() + ed_nitems; }
 };
 
 namespace motex {
+
+inline hipStream_t bstream(const motex_batch *b) { return b->stream ? b->stream : b->ctx->stream; }
+int batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num,
+                 const motex_params *p, hipStream_t stream, motex_batch **out);
+int batch_launch(motex_batch *b);   // enqueue clear + batched kernel, no wait
+int batch_finish(motex_batch *b);   // wait for the last launch, record its timings
 
 std::vector<uint32_t> enumerate_structs(const motex_params *p, uint32_t *nb_structs);
 uint32_t total_motif_length(const motex_params *p);
@@ -147,13 +176,13 @@ uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max);
 int batch_upload_common(motex_batch *b);
 int batch_clear(motex_batch *b);
 int hd_prepare(motex_batch *b);
-int hd_run(motex_batch *b);
+int hd_launch(motex_batch *b);
 int hd_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
 int ed_prepare(motex_batch *b);
-int ed_run(motex_batch *b);
+int ed_launch_all(motex_batch *b);
 int ed_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
 int struct_prepare(motex_batch *b);
-int struct_run(motex_batch *b);
+int struct_launch_all(motex_batch *b);
 int struct_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
 int pair_run(motex_batch *b, uint32_t i, uint32_t j);
 int ed_pair_run(motex_batch *b, uint32_t i, uint32_t j);

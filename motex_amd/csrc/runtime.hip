@@ -25,6 +25,7 @@ namespace motex {
 // errors
 // ---------------------------------------------------------------------------------------
 static thread_local std::string g_err;
+thread_local hipStream_t g_alloc_stream = nullptr;
 
 int set_error(int code, const char *fmt, ...)
 {
@@ -247,12 +248,12 @@ static int hd_upload_plan(motex_batch *b, HdPlan &plan, DevBuf &d_items, DevBuf 
     if ((e = d_items.alloc(std::max<size_t>(plan.items.size(), 1) * sizeof(HdItem))) != hipSuccess)
         return hip_fail(e, "alloc items");
     if (!plan.items.empty() &&
-        (e = hipMemcpy(d_items.p, plan.items.data(), plan.items.size() * sizeof(HdItem), hipMemcpyHostToDevice)) != hipSuccess)
+        (e = hipMemcpyAsync(d_items.p, plan.items.data(), plan.items.size() * sizeof(HdItem), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
     if ((e = d_ranges.alloc(std::max<size_t>(plan.ranges.size(), 1) * sizeof(HdRange))) != hipSuccess)
         return hip_fail(e, "alloc ranges");
     if (!plan.ranges.empty() &&
-        (e = hipMemcpy(d_ranges.p, plan.ranges.data(), plan.ranges.size() * sizeof(HdRange), hipMemcpyHostToDevice)) != hipSuccess)
+        (e = hipMemcpyAsync(d_ranges.p, plan.ranges.data(), plan.ranges.size() * sizeof(HdRange), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy ranges");
     return MOTEX_OK;
 }
@@ -315,10 +316,10 @@ int batch_upload_common(motex_batch *b)
     hipError_t e;
     const SeqSet &ss = b->ss;
     if ((e = b->d_codes.alloc(std::max<uint64_t>(ss.total, 1) + 64)) != hipSuccess) return hip_fail(e, "alloc codes");
-    if (ss.total && (e = hipMemcpy(b->d_codes.p, ss.codes.data(), ss.total, hipMemcpyHostToDevice)) != hipSuccess)
+    if (ss.total && (e = hipMemcpyAsync(b->d_codes.p, ss.codes.data(), ss.total, hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy codes");
     if ((e = b->d_seq_start.alloc((ss.num + 1) * 4)) != hipSuccess) return hip_fail(e, "alloc starts");
-    if ((e = hipMemcpy(b->d_seq_start.p, ss.start.data(), (ss.num + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(b->d_seq_start.p, ss.start.data(), (ss.num + 1) * 4, hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy starts");
     uint64_t uvn = std::max<uint64_t>(ss.uv_off[ss.num], 1);
     if ((e = b->d_u.alloc(uvn * 4)) != hipSuccess) return hip_fail(e, "alloc u");
@@ -367,12 +368,12 @@ int hd_prepare(motex_batch *b)
     hipError_t e;
     if ((e = b->d_ne.alloc((uint64_t)W * ss.sigma * 4)) != hipSuccess) return hip_fail(e, "alloc NE tables");
     if ((e = launch_ne_build((const uint8_t *)b->d_codes.p, (uint32_t)ss.total, ss.sigma, (uint32_t)W,
-                             (uint32_t *)b->d_ne.p, c->stream)) != hipSuccess)
+                             (uint32_t *)b->d_ne.p, bstream(b))) != hipSuccess)
         return hip_fail(e, "NE build");
     b->split_words = (uint32_t)((ss.uv_off[ss.num] + 31) / 32 + 1);
     uint64_t sf = (uint64_t)std::max<uint32_t>(b->hd.split_slots, 1) * b->split_words;
     if ((e = b->d_split.alloc(sf * 4)) != hipSuccess) return hip_fail(e, "alloc split flags");
-    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "NE build sync");
+    if ((e = hipStreamSynchronize(bstream(b))) != hipSuccess) return hip_fail(e, "NE build sync");
     return MOTEX_OK;
 }
 
@@ -403,7 +404,7 @@ HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t
 int batch_clear(motex_batch *b)
 {
     hipError_t e;
-    hipStream_t s = b->stream ? b->stream : b->ctx->stream;
+    hipStream_t s = bstream(b);
     uint64_t uvn = std::max<uint64_t>(b->ss.uv_off[b->ss.num], 1);
     if ((e = launch_fill_u32((uint32_t *)b->d_u.p, uvn, 0, s)) != hipSuccess) return hip_fail(e, "clear u");
     if ((e = launch_fill_u32((uint32_t *)b->d_v.p, uvn, 0, s)) != hipSuccess) return hip_fail(e, "clear v");
@@ -413,25 +414,44 @@ int batch_clear(motex_batch *b)
     return MOTEX_OK;
 }
 
-int hd_run(motex_batch *b)
+int hd_launch(motex_batch *b)
 {
     motex_ctx *c = b->ctx;
-    hipStream_t s = b->stream ? b->stream : c->stream;
+    hipStream_t s = bstream(b);
     hipError_t e;
-    if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "event");
-    int rc = batch_clear(b);
-    if (rc) return rc;
-    if ((e = hipEventRecord(c->ev2, s)) != hipSuccess) return hip_fail(e, "event");
     if (!b->hd.items.empty()) {
         HdArgs a = hd_args(b, b->hd, b->d_items, 1);
         uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->num_cu * b->tile.waves_per_cu, b->hd.items.size());
+        if (b->grid_cap) grid = std::min(grid, b->grid_cap);
         if ((e = launch_hd(a, b->tile.blocks, grid, s)) != hipSuccess) return hip_fail(e, "launch Hamming kernel");
     }
-    if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "event");
-    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "Hamming kernel");
+    return MOTEX_OK;
+}
+
+int batch_launch(motex_batch *b)
+{
+    hipStream_t s = bstream(b);
+    AllocStreamGuard guard(b->stream);  // scratch allocated at launch follows the batch's stream
+    hipError_t e;
+    if ((e = hipEventRecord(b->ev0, s)) != hipSuccess) return hip_fail(e, "event");
+    int rc = batch_clear(b);
+    if (rc) return rc;
+    if ((e = hipEventRecord(b->ev2, s)) != hipSuccess) return hip_fail(e, "event");
+    if (b->params.nb_gaps) rc = struct_launch_all(b);
+    else if (b->params.distance == 0) rc = hd_launch(b);
+    else rc = ed_launch_all(b);
+    if (rc) return rc;
+    if ((e = hipEventRecord(b->ev1, s)) != hipSuccess) return hip_fail(e, "event");
+    return MOTEX_OK;
+}
+
+int batch_finish(motex_batch *b)
+{
+    hipError_t e;
+    if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return hip_fail(e, "extraction kernel");
     float ms = 0, kms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    hipEventElapsedTime(&kms, c->ev2, c->ev1);
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    hipEventElapsedTime(&kms, b->ev2, b->ev1);
     b->stats.kernel_ms = ms;
     b->stats.main_kernel_ms = kms;
     return MOTEX_OK;
@@ -450,7 +470,7 @@ static uint32_t plain_blocks(uint32_t n)
 int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32_t>> &pairs, double *ms_out)
 {
     motex_ctx *c = b->ctx;
-    hipStream_t s = b->stream ? b->stream : c->stream;
+    hipStream_t s = bstream(b);
     hipError_t e;
     const SeqSet &ss = b->ss;
     const uint32_t k = b->params.k;
@@ -490,7 +510,7 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
     if ((e = launch_fill_u32((uint32_t *)flags.p, flags.bytes / 4, 0, s)) != hipSuccess) return hip_fail(e, "clear flags");
     HdPlan shape;
     shape.rows_max = all.rows_max;
-    hipEventRecord(c->ev0, s);
+    hipEventRecord(b->ev0, s);
     for (const Launch &L : launches) {
         shape.wl = hd_lds_words(L.blocks, shape.rows_max);
         HdArgs a = hd_args(b, shape, items, 0);
@@ -500,10 +520,10 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
         a.split_flags = (uint32_t *)flags.p;
         if ((e = launch_hd(a, L.blocks, L.count, s)) != hipSuccess) return hip_fail(e, "launch plain kernel");
     }
-    hipEventRecord(c->ev1, s);
-    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "plain kernel");
+    hipEventRecord(b->ev1, s);
+    if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return hip_fail(e, "plain kernel");
     float ms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
     if (ms_out) *ms_out = ms;
     return MOTEX_OK;
 }
@@ -556,6 +576,12 @@ double pair_cells(const SeqSet &ss, const motex_params &p, uint64_t *pairs, uint
 extern "C" int motex_batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens,
                                   uint32_t num, const motex_params *p, motex_batch **out)
 {
+    return batch_create(c, seqs, lens, num, p, nullptr, out);
+}
+
+int motex::batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num,
+                        const motex_params *p, hipStream_t stream, motex_batch **out)
+{
     if (!c || !out) return set_error(MOTEX_EINVAL, "ctx/out is NULL");
     *out = nullptr;
     int rc = validate_params(p);
@@ -565,6 +591,11 @@ extern "C" int motex_batch_create(motex_ctx *c, const char *const *seqs, const u
     if ((e = hipSetDevice(c->device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
     std::unique_ptr<motex_batch> b(new motex_batch());
     b->ctx = c;
+    b->stream = stream;
+    AllocStreamGuard guard(stream);
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess ||
+        (e = hipEventCreate(&b->ev2)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
     b->params = *p;
     b->gmin.assign(p->gap_min, p->gap_min + p->nb_gaps);
     b->gmax.assign(p->gap_max, p->gap_max + p->nb_gaps);
@@ -582,6 +613,7 @@ extern "C" int motex_batch_create(motex_ctx *c, const char *const *seqs, const u
     else if (p->distance == 0) rc = hd_prepare(b.get());
     else rc = ed_prepare(b.get());
     if (rc) return rc;
+    if ((e = hipStreamSynchronize(bstream(b.get()))) != hipSuccess) return hip_fail(e, "batch upload");
     b->stats.tile_blocks = b->tile.blocks;
     b->stats.tile_rows = b->tile.rows;
     b->stats.waves_per_cu = b->tile.waves_per_cu;
@@ -594,9 +626,8 @@ extern "C" int motex_batch_run(motex_batch *b)
 {
     if (!b) return set_error(MOTEX_EINVAL, "batch is NULL");
     hipSetDevice(b->ctx->device);
-    if (b->params.nb_gaps) return struct_run(b);
-    if (b->params.distance == 0) return hd_run(b);
-    return ed_run(b);
+    int rc = batch_launch(b);
+    return rc ? rc : batch_finish(b);
 }
 
 extern "C" int motex_batch_clear(motex_batch *b)
@@ -605,7 +636,7 @@ extern "C" int motex_batch_clear(motex_batch *b)
     hipSetDevice(b->ctx->device);
     int rc = batch_clear(b);
     if (rc) return rc;
-    hipError_t e = hipStreamSynchronize(b->ctx->stream);
+    hipError_t e = hipStreamSynchronize(bstream(b));
     return e == hipSuccess ? MOTEX_OK : hip_fail(e, "clear");
 }
 

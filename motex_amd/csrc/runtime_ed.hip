@@ -76,9 +76,10 @@ int ed_prepare(motex_batch *b)
     uint64_t ni = std::max<size_t>(sorted.size(), 1);
     if ((e = b->d_items.alloc(ni * sizeof(EdItemDev))) != hipSuccess) return hip_fail(e, "alloc items");
     if (!sorted.empty() &&
-        (e = hipMemcpy(b->d_items.p, sorted.data(), sorted.size() * sizeof(EdItemDev), hipMemcpyHostToDevice)) != hipSuccess)
+        (e = hipMemcpyAsync(b->d_items.p, sorted.data(), sorted.size() * sizeof(EdItemDev), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
     b->ed_nitems = (uint32_t)sorted.size();
+    if ((e = hipStreamSynchronize(bstream(b))) != hipSuccess) return hip_fail(e, "upload items");
     return MOTEX_OK;
 }
 
@@ -115,34 +116,19 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
     return MOTEX_OK;
 }
 
-int ed_run(motex_batch *b)
+int ed_launch_all(motex_batch *b)
 {
-    motex_ctx *c = b->ctx;
-    hipStream_t s = b->stream ? b->stream : c->stream;
-    hipError_t e;
-    if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "event");
-    int rc = batch_clear(b);
-    if (rc) return rc;
-    if ((e = hipEventRecord(c->ev2, s)) != hipSuccess) return hip_fail(e, "event");
-    if (b->ed_nitems) {
-        uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->num_cu * b->tile.waves_per_cu, b->ed_nitems);
-        if ((rc = ed_launch(b, b->d_items.p, b->ed_nitems, grid, 1, s))) return rc;
-    }
-    if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "event");
-    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "edit kernel");
-    float ms = 0, kms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    hipEventElapsedTime(&kms, c->ev2, c->ev1);
-    b->stats.kernel_ms = ms;
-    b->stats.main_kernel_ms = kms;
-    return MOTEX_OK;
+    if (!b->ed_nitems) return MOTEX_OK;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, b->ed_nitems);
+    if (b->grid_cap) grid = std::min(grid, b->grid_cap);
+    return ed_launch(b, b->d_items.p, b->ed_nitems, grid, 1, bstream(b));
 }
 
 // Plain form: one launch per pair (one work item, one wave).
 static int ed_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32_t>> &pairs, double *ms_out)
 {
     motex_ctx *c = b->ctx;
-    hipStream_t s = b->stream ? b->stream : c->stream;
+    hipStream_t s = bstream(b);
     hipError_t e;
     const SeqSet &ss = b->ss;
     std::vector<EdItemDev> all;
@@ -154,17 +140,17 @@ static int ed_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t,
     if (all.empty()) return MOTEX_OK;
     DevBuf items;
     if ((e = items.alloc(all.size() * sizeof(EdItemDev))) != hipSuccess) return hip_fail(e, "alloc items");
-    if ((e = hipMemcpy(items.p, all.data(), all.size() * sizeof(EdItemDev), hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(items.p, all.data(), all.size() * sizeof(EdItemDev), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
-    hipEventRecord(c->ev0, s);
+    hipEventRecord(b->ev0, s);
     for (size_t t = 0; t < all.size(); ++t) {
         int rc = ed_launch(b, (const EdItemDev *)items.p + t, 1, 1, 0, s);
         if (rc) return rc;
     }
-    hipEventRecord(c->ev1, s);
-    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "plain edit kernel");
+    hipEventRecord(b->ev1, s);
+    if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return hip_fail(e, "plain edit kernel");
     float ms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
     if (ms_out) *ms_out = ms;
     return MOTEX_OK;
 }

@@ -86,7 +86,7 @@ int struct_prepare(motex_batch *b)
         b->hd.wl = wl;
         if ((e = b->d_ne.alloc(W * ss.sigma * 4)) != hipSuccess) return hip_fail(e, "alloc NE tables");
         if ((e = launch_ne_build((const uint8_t *)b->d_codes.p, (uint32_t)ss.total, ss.sigma, (uint32_t)W,
-                                 (uint32_t *)b->d_ne.p, b->ctx->stream)) != hipSuccess)
+                                 (uint32_t *)b->d_ne.p, bstream(b))) != hipSuccess)
             return hip_fail(e, "NE build");
     }
     std::vector<StructItem> items;
@@ -97,12 +97,12 @@ int struct_prepare(motex_batch *b)
     if ((e = b->d_items.alloc(std::max<size_t>(items.size(), 1) * sizeof(StructItem))) != hipSuccess)
         return hip_fail(e, "alloc items");
     if (!items.empty() &&
-        (e = hipMemcpy(b->d_items.p, items.data(), items.size() * sizeof(StructItem), hipMemcpyHostToDevice)) != hipSuccess)
+        (e = hipMemcpyAsync(b->d_items.p, items.data(), items.size() * sizeof(StructItem), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
     b->ed_nitems = (uint32_t)items.size();
     if ((e = b->d_aux2.alloc(std::max<size_t>(b->S.size(), 1) * 4)) != hipSuccess) return hip_fail(e, "alloc S");
     if (!b->S.empty() &&
-        (e = hipMemcpy(b->d_aux2.p, b->S.data(), b->S.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        (e = hipMemcpyAsync(b->d_aux2.p, b->S.data(), b->S.size() * 4, hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy S");
     b->tile = choose_tile(b->ctx, b->params, ss);
     // cells: one DP per distinct box shape and pair
@@ -111,7 +111,7 @@ int struct_prepare(motex_batch *b)
         for (uint32_t d = 0; d < a.ndist; ++d)
             if (it.m >= a.dlen[d]) cells += (double)(it.m - a.dlen[d] + 1) * it.n;
     b->stats.cells = cells;
-    return hipStreamSynchronize(b->ctx->stream) == hipSuccess ? MOTEX_OK : set_error(MOTEX_EDEVICE, "NE build");
+    return hipStreamSynchronize(bstream(b)) == hipSuccess ? MOTEX_OK : set_error(MOTEX_EDEVICE, "NE build");
 }
 
 static int struct_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_t grid,
@@ -164,32 +164,18 @@ static uint32_t struct_grid(motex_batch *b, uint32_t nitems)
     return (uint32_t)std::max<uint64_t>(1, std::min(g, cap));
 }
 
-int struct_run(motex_batch *b)
+int struct_launch_all(motex_batch *b)
 {
-    motex_ctx *c = b->ctx;
-    hipStream_t s = b->stream ? b->stream : c->stream;
-    hipError_t e;
-    if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "event");
-    int rc = batch_clear(b);
-    if (rc) return rc;
-    if ((e = hipEventRecord(c->ev2, s)) != hipSuccess) return hip_fail(e, "event");
-    if (b->ed_nitems) {
-        if ((rc = struct_launch(b, b->d_items.p, b->ed_nitems, struct_grid(b, b->ed_nitems), 1, s))) return rc;
-    }
-    if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "event");
-    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "structured kernel");
-    float ms = 0, kms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    hipEventElapsedTime(&kms, c->ev2, c->ev1);
-    b->stats.kernel_ms = ms;
-    b->stats.main_kernel_ms = kms;
-    return MOTEX_OK;
+    if (!b->ed_nitems) return MOTEX_OK;
+    uint32_t grid = struct_grid(b, b->ed_nitems);
+    if (b->grid_cap) grid = std::min(grid, b->grid_cap);
+    return struct_launch(b, b->d_items.p, b->ed_nitems, grid, 1, bstream(b));
 }
 
 static int struct_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32_t>> &pairs, double *ms_out)
 {
     motex_ctx *c = b->ctx;
-    hipStream_t s = b->stream ? b->stream : c->stream;
+    hipStream_t s = bstream(b);
     hipError_t e;
     std::vector<StructItem> items;
     plan_pairs(b->ss, total_motif_length(&b->params), &pairs, items);
@@ -197,17 +183,17 @@ static int struct_launch_pairs(motex_batch *b, const std::vector<std::pair<uint3
     if (items.empty()) return MOTEX_OK;
     DevBuf d;
     if ((e = d.alloc(items.size() * sizeof(StructItem))) != hipSuccess) return hip_fail(e, "alloc items");
-    if ((e = hipMemcpy(d.p, items.data(), items.size() * sizeof(StructItem), hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(d.p, items.data(), items.size() * sizeof(StructItem), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
-    hipEventRecord(c->ev0, s);
+    hipEventRecord(b->ev0, s);
     for (size_t t = 0; t < items.size(); ++t) {
         int rc = struct_launch(b, (const StructItem *)d.p + t, 1, 1, 0, s);
         if (rc) return rc;
     }
-    hipEventRecord(c->ev1, s);
-    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) return hip_fail(e, "plain structured kernel");
+    hipEventRecord(b->ev1, s);
+    if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return hip_fail(e, "plain structured kernel");
     float ms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
     if (ms_out) *ms_out = ms;
     return MOTEX_OK;
 }

@@ -1,0 +1,66 @@
+"""Serving: jobs submitted while others run give the same u/v as the same job alone."""
+import numpy as np
+import pytest
+
+import motex_amd
+from motex_amd import MotexParams, serving, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_jobs():
+    jobs = [synth.serving_job(i, scale=0.08) for i in range(10)]
+    w4 = synth.config4a(scale=0.01)
+    jobs.append(synth.Workload("structured", "DNA", w4.seqs, 0, w4.k, w4.e, 40, boxes=list(w4.boxes)))
+    w3 = synth.config3(scale=0.01)
+    jobs.append(synth.Workload("edit", "PROT", w3.seqs, 1, w3.k, w3.e, 30))
+    return jobs
+
+
+def test_results_same_alone_and_in_the_mix(ctx):
+    jobs = mixed_jobs()
+    alone = []
+    for w in jobs:
+        print(f"[serving] alone {w.name} n={len(w.seqs)} k={w.k} e={w.e} d={w.distance}", flush=True)
+        alone.append(ctx.extract_all_pairs(w.seqs, serving.job_params(w)))
+    # submit everything at once: jobs are admitted as slots free up and overlap on the GPU
+    handles = [serving.JobHandle(ctx, w.seqs, serving.job_params(w)) for w in jobs]
+    for w, h, (au, av, aoff) in zip(jobs, handles, alone):
+        u, v, off = h.wait()
+        assert np.array_equal(u, au) and np.array_equal(v, av), w.name
+        lat, dev, cells = h.times()
+        assert lat > 0 and dev > 0 and cells > 0
+        h.release()
+
+
+def test_poll_and_release(ctx):
+    w = synth.serving_job(3, scale=0.05)
+    h = serving.JobHandle(ctx, w.seqs, serving.job_params(w))
+    u, v, _ = h.wait()
+    assert h.done()
+    h.release()
+    with pytest.raises(motex_amd.MotexError):
+        h.release()          # unknown id now
+
+
+def test_invalid_job_is_refused(ctx):
+    w = synth.serving_job(1, scale=0.05)
+    with pytest.raises(motex_amd.MotexError):
+        serving.JobHandle(ctx, w.seqs, MotexParams(k=5, e=5))
+
+
+def test_admission_refuses_jobs_over_the_budget():
+    small = motex_amd.Context(0, hbm_budget=1 << 16)
+    try:
+        w = synth.config2(scale=0.05)
+        with pytest.raises(motex_amd.MotexError) as ei:
+            serving.JobHandle(small, w.seqs, serving.job_params(w))
+        assert ei.value.code == -2
+    finally:
+        small.close()
+
+
+def test_stream_replay_reports_latency(ctx):
+    r = serving.run_stream(ctx, njobs=12, warm=4, scale=0.05)
+    assert r["p99_latency_ms"] >= r["p50_latency_ms"] > 0
+    assert r["aggregate_gcups"] > 0
