@@ -1,0 +1,87 @@
+"""The motex command line: option handling, input errors and exit codes of motexCPU
+(motex.cc:96-300, functions.cc:2862-3043).  These cases fail before any GPU work."""
+import os
+import subprocess
+
+import pytest
+
+import motex_amd
+
+BIN = motex_amd.BIN_PATH
+
+
+def run(args, cwd):
+    r = subprocess.run([BIN] + args, cwd=cwd, capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.fixture
+def fa(tmp_path):
+    p = tmp_path / "in.fa"
+    p.write_text(">s1\nACGTACGTAC\n>s2\nacgtnacgta\n")
+    return str(p)
+
+
+BASE = ["-a", "DNA", "-o", "out.motex", "-d", "0", "-k", "4", "-e", "1", "-q", "10"]
+
+
+def test_binary_exists():
+    assert os.access(BIN, os.X_OK)
+
+
+def test_missing_mandatory_prints_usage(tmp_path, fa):
+    rc, out, err = run(["-a", "DNA", "-i", fa, "-o", "x", "-d", "0", "-k", "4", "-e", "1"], tmp_path)
+    assert rc == 1 and "Usage" in out
+
+
+def test_help_prints_usage(tmp_path, fa):
+    rc, out, err = run(["-h"] + BASE + ["-i", fa], tmp_path)
+    assert rc == 1 and "Usage" in out
+
+
+@pytest.mark.parametrize("change,msg", [
+    (["-e", "4"], "error threshold e must be less than"),
+    (["-d", "2"], "distance argument d should be"),
+    (["-q", "101"], "quorum argument q (%) should be less or equal to 100"),
+    (["-Q", "101"], "max-quorum argument Q (%) should be less or equal to 100"),
+    (["-a", "RNA"], "alphabet argument a should be"),
+    (["-u", "un.txt"], "`-u' option must be used with `-b' option"),
+    (["-U", "un.smile"], "`-U' option must be used with `-b' option"),
+    (["-b", "bg.motex"], "`-b' option must be used with `-u' option"),
+])
+def test_validation_errors(tmp_path, fa, change, msg):
+    args = dict(zip(BASE[::2], BASE[1::2]))
+    for k, v in zip(change[::2], change[1::2]):
+        args[k] = v
+    flat = [x for kv in args.items() for x in kv] + ["-i", fa]
+    rc, out, err = run(flat, tmp_path)
+    assert rc == 1 and msg in err
+
+
+def test_background_mode_is_refused(tmp_path, fa):
+    rc, out, err = run(BASE + ["-i", fa, "-b", "bg", "-u", "un"], tmp_path)
+    assert rc == 1 and "not supported" in err
+
+
+def test_missing_input(tmp_path):
+    rc, out, err = run(BASE + ["-i", "nope.fa"], tmp_path)
+    assert rc == 1 and "Cannot open file nope.fa" in err
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("ACGT\n", "is not in FASTA format"),
+    (">s\nACGTX\n", "contains an unexpected character"),
+    (">s\n\nACGT\n", "is not in FASTA format"),      # blank line after a header
+    (">s\r\nACGT\r\n", "contains an unexpected character"),
+])
+def test_fasta_errors(tmp_path, text, msg):
+    p = tmp_path / "bad.fa"
+    p.write_bytes(text.encode())
+    rc, out, err = run(BASE + ["-i", str(p)], tmp_path)
+    assert rc == 1 and msg in err
+
+
+def test_boxes_format_error(tmp_path, fa):
+    (tmp_path / "b.txt").write_text("1\n3\n")
+    rc, out, err = run(BASE + ["-i", fa, "-s", "b.txt"], tmp_path)
+    assert rc == 1 and "is not in the correct format" in err

@@ -1,0 +1,75 @@
+"""End to end: the motex command line on the GPU produces the .motex and SMILE files the
+reference writers would produce from the oracle's u/v (byte-identical apart from the run
+date and timing lines)."""
+import os
+import subprocess
+
+import pytest
+
+import motex_amd
+from motex_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+MASK = ("# Rundate:", "# Run on", "User time:")
+
+
+def masked(path):
+    return [l for l in open(path).read().splitlines() if not l.startswith(MASK)]
+
+
+def run_both(tmp_path, oracle, w, smile=True):
+    gdir = tmp_path / "gpu"
+    odir = tmp_path / "oracle"
+    gdir.mkdir()
+    odir.mkdir()
+    (gdir / "in.fa").write_bytes(w.fasta())
+    args = [motex_amd.BIN_PATH, "-i", "in.fa", "-o", "out.motex"] + w.cli_args()
+    if w.boxes:
+        (gdir / "boxes.txt").write_text(w.boxes_text())
+        args += ["-s", "boxes.txt"]
+    if smile:
+        args += ["-S", "out.smile"]
+    print("[cli] " + " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=gdir, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    u, v, off, _ = oracle.all_pairs(w.seqs, w.distance, w.k, w.e, boxes=w.boxes)
+    cwd = os.getcwd()
+    os.chdir(odir)
+    try:
+        oracle.write("out.motex", w.seqs, u, v, off, alphabet=w.alphabet, d=w.distance, k=w.k,
+                     e=w.e, q=w.q, Q=w.Q, n=w.n, N=w.N, boxes=w.boxes, input_filename="in.fa",
+                     boxes_filename="boxes.txt", smile_path="out.smile" if smile else None)
+    finally:
+        os.chdir(cwd)
+    assert masked(gdir / "out.motex") == masked(odir / "out.motex")
+    if smile:
+        assert masked(gdir / "out.smile") == masked(odir / "out.smile")
+    return masked(gdir / "out.motex")
+
+
+def test_config1(tmp_path, oracle):
+    w = synth.config1()
+    lines = run_both(tmp_path, oracle, w)
+    motifs = [l.split()[0] for l in lines if l and not l.startswith("#")]
+    # the planted 10-mers (in 60% of sequences, q=50) are reported
+    for p in w.planted:
+        assert p.decode() in motifs
+
+
+def test_structured_config4a_subset(tmp_path, oracle):
+    w = synth.config4a(scale=0.03)
+    w.q = 10
+    run_both(tmp_path, oracle, w)
+
+
+def test_protein_edit_subset(tmp_path, oracle):
+    w = synth.config3(scale=0.02)
+    w.q = 5
+    run_both(tmp_path, oracle, w)
+
+
+def test_quorum_zero_prints_every_factor(tmp_path, oracle):
+    w = synth.config1(scale=0.1)
+    w.q, w.N = 0, 2000000000
+    run_both(tmp_path, oracle, w, smile=False)
