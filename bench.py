@@ -44,7 +44,20 @@ def parse():
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving stream (p50/p99 job latency)")
     ap.add_argument("--serving-jobs", type=int, default=256)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES bytes (calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_probe(nbytes: int):
+    import torch
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}))
 
 
 def cpu_baseline(w, gpu_u, gpu_v, off, nx):
@@ -87,6 +100,9 @@ def cpu_baseline_prefix(ctx, w, params, seconds=12.0):
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU's share only; --gpus {a.gpus} is not supported "
               "(sharding is described in DESIGN.md).", file=sys.stderr)

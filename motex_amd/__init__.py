@@ -13,7 +13,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libmotex.so")
+# MOTEX_LIB points at another build of the same library (e.g. an experimental variant).
+LIB_PATH = os.environ.get("MOTEX_LIB", os.path.join(HERE, "lib", "libmotex.so"))
 BIN_PATH = os.path.join(HERE, "bin", "motex")
 
 ALPHA = {"DNA": 0, "PROT": 1, "USR": 2}

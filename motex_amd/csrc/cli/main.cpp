@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../../include/motex.h"
+#include "motex.h"
 #include "cli.h"
 
 using namespace motex_cli;

@@ -50,65 +50,29 @@ __global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n, uint32_t v
     for (; i < n; i += stride) p[i] = val;
 }
 
-// Folds the hit flags of one row into the item's per-row LDS accumulators.  Wave-uniform:
-// all lanes call it together.  Hits come in increasing text position (block, lane, bit)
-// and a whole text lies in one range, so within a row a text is new unless it equals the
-// last one counted.  Ranges that are parts of one long text count it at most once per row.
-template <int B>
-__device__ __forceinline__ void fold_row(const HdArgs &a, const HdItem &I, const HdRange &R, int32_t q0,
-                                         uint32_t px, uint32_t r, const uint32_t (&res)[B],
-                                         uint32_t *acc_v, uint32_t *acc_u)
-{
-    const uint32_t lane = threadIdx.x;
-    uint32_t du = 0, dv = 0;
-    int lasty = -1;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        uint32_t h = res[b];
-        uint32_t cnt = 0, dist = 0;
-        int fy = -1, ly = -1;
-        while (h) {
-            uint32_t t = __builtin_ctz(h);
-            h &= h - 1;
-            int64_t pt = (int64_t)q0 + (int64_t)(kDiagPerBlock * b + 32 * lane + t) + px;
-            if (pt < (int64_t)R.ys || pt >= (int64_t)R.ye) continue;
-            uint32_t y = R.ny == 1 ? R.j0 : find_seq(a.seq_start, R.j0, R.ny, (uint32_t)pt);
-            if ((uint32_t)pt - a.seq_start[y] + 1 < a.k) continue;  // window starts before y
-            cnt++;
-            if ((int)y != ly) { dist++; if (fy < 0) fy = (int)y; ly = (int)y; }
-        }
-        uint64_t mask = __ballot(cnt != 0);
-        while (mask) {
-            uint32_t L = (uint32_t)__builtin_ctzll(mask);
-            mask &= mask - 1;
-            uint32_t lc = __builtin_amdgcn_readlane(cnt, L);
-            uint32_t ld = __builtin_amdgcn_readlane(dist, L);
-            int lf = __builtin_amdgcn_readlane(fy, L);
-            int ll = __builtin_amdgcn_readlane(ly, L);
-            dv += lc;
-            du += ld - (lf == lasty ? 1u : 0u);
-            lasty = ll;
-        }
-    }
-    if (dv && lane == 0) {
-        acc_v[r] += dv;
-        if (I.split != kNoSplit) acc_u[r] = 1;
-        else acc_u[r] += du;
-    }
-}
-
 // Persistent batched kernel: one wave per workgroup pulls items from the queue head.
+//
+// Hit folding is lane-parallel.  A lane with hit flags in a row checks each flag (text
+// range, complete window inside its sequence), finds the flag's sequence among the range's
+// at most kMaxRangeSeqs starts (LDS binary search) and adds its count to the row's v with
+// an LDS atomic add and its sequences to the row's sequence mask with an LDS atomic OR.
+// When a range is done, each row's mask popcount (the distinct sequences of that range
+// with an occurrence) is added to the row's u; a range that is part of a long sequence
+// marks u once.  The item's rows are flushed to global u/v at the end.
 template <int B, int P, int E>
 __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
 {
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x;
     const uint32_t wl = a.wl;
-    // LDS: NE window (sigma + 1 rows, the last all zero) | per-row u, v | x offsets
+    // LDS: NE window (sigma + 1 rows, the last all zero) | per-row v, u, sequence mask |
+    //      range sequence starts | x offsets
     uint32_t *ne_l = smem;
     uint32_t *acc_v = smem + (a.sigma + 1) * wl;
     uint32_t *acc_u = acc_v + a.rows_max;
-    uint16_t *xoff = reinterpret_cast<uint16_t *>(acc_u + a.rows_max);
+    uint32_t *ymask = acc_u + a.rows_max;
+    uint32_t *starts = ymask + a.rows_max;
+    uint16_t *xoff = reinterpret_cast<uint16_t *>(starts + kMaxRangeSeqs + 1);
     const uint32_t zero_row = a.sigma * wl;
     for (uint32_t w = lane; w < wl; w += 64) ne_l[zero_row + w] = 0;
 
@@ -129,6 +93,7 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
             xoff[r] = (uint16_t)(a.codes[I.x_start + I.px_start + r] * wl);
             acc_v[r] = 0;
             acc_u[r] = 0;
+            ymask[r] = 0;
         }
         const uint32_t rep0 = I.r0 > a.k - 1 ? I.r0 : a.k - 1;
         const uint32_t pxs5 = I.px_start >> 5;
@@ -142,6 +107,7 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
                 const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
                 for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
             }
+            if (lane <= R.ny) starts[lane] = a.seq_start[R.j0 + lane];
             __syncthreads();
 
             uint32_t cnt[B][P];
@@ -180,8 +146,45 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
                     res[b] = le_const<P, E>(cnt[b], a.e);
                     any |= res[b];
                 }
-                if (px >= rep0 && __ballot(any != 0))
-                    fold_row<B>(a, I, R, q0, px, r, res, acc_v, acc_u);
+                if (px >= rep0 && __ballot(any != 0) && any) {
+                    uint32_t hits = 0, ym = 0;
+#pragma unroll
+                    for (int b = 0; b < B; ++b) {
+                        uint32_t h = res[b];
+                        while (h) {
+                            uint32_t t = __builtin_ctz(h);
+                            h &= h - 1;
+                            int32_t pt = q0 + (int32_t)(kDiagPerBlock * b + 32 * lane + t + px);
+                            if (pt < (int32_t)R.ys || pt >= (int32_t)R.ye) continue;
+                            uint32_t y = 0;
+                            if (R.ny > 1) {
+                                uint32_t lo = 0, hi = R.ny - 1;
+                                while (lo < hi) {
+                                    uint32_t mid = (lo + hi + 1) >> 1;
+                                    if (starts[mid] <= (uint32_t)pt) lo = mid; else hi = mid - 1;
+                                }
+                                y = lo;
+                            }
+                            if ((uint32_t)pt - starts[y] + 1 < a.k) continue;  // window starts before y
+                            hits++;
+                            ym |= 1u << y;
+                        }
+                    }
+                    if (hits) {
+                        atomicAdd(acc_v + r, hits);
+                        atomicOr(ymask + r, ym);
+                    }
+                }
+            }
+            __syncthreads();
+            // distinct sequences of this range per row
+            for (uint32_t r = lane; r < nrows; r += 64) {
+                uint32_t m = ymask[r];
+                if (m) {
+                    if (I.split != kNoSplit) acc_u[r] = 1;
+                    else acc_u[r] += __builtin_popcount(m);
+                    ymask[r] = 0;
+                }
             }
         }
         __syncthreads();
@@ -239,7 +242,8 @@ static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream
 
 size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max)
 {
-    return (size_t)(sigma + 1) * wl * 4 + (size_t)rows_max * 8 + (size_t)((rows_max + 1) & ~1u) * 2;
+    return (size_t)(sigma + 1) * wl * 4 + (size_t)rows_max * 12 + (kMaxRangeSeqs + 1) * 4 +
+           (size_t)((rows_max + 1) & ~1u) * 2;
 }
 
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)

@@ -11,6 +11,7 @@ constexpr uint32_t kDiagPerBlock = 2048;     // 64 lanes x 32 diagonals
 constexpr uint32_t kMaxHdK = 64;             // Hamming kernels: counters of up to 7 planes
 constexpr uint32_t kMaxEdK = 32;             // edit kernels: bit-vectors stay below 64 bits
 constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
+constexpr uint32_t kMaxRangeSeqs = 32;       // sequences per Hamming text range (one mask word)
 
 // A text range of the Hamming work queue: whole sequences j0 .. j0+ny-1, or one part of a
 // long sequence j0 (ny = 1), covering text positions [ys, ye).

@@ -13,7 +13,7 @@
 #include <tuple>
 #include <vector>
 
-#include "../../include/motex.h"
+#include "motex.h"
 #include "motex_internal.h"
 
 namespace motex {

@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/motex.h"
+#include "motex.h"
 #include "motex_internal.h"
 #include "runtime.h"
 
@@ -181,7 +181,7 @@ void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
             if ((int64_t)ss.len[j] > cap) { add_seq_ranges(j); ++j; continue; }
             HdRange r{ss.start[j], ss.start[j] + ss.len[j], j, 1};
             uint32_t jj = j + 1;
-            while (jj < ss.num && (int64_t)ss.len[jj] <= cap &&
+            while (jj < ss.num && (int64_t)ss.len[jj] <= cap && jj - j < kMaxRangeSeqs &&
                    (int64_t)(ss.start[jj] + ss.len[jj]) - r.ys <= cap) {
                 if (ss.len[jj] >= k) { r.ye = ss.start[jj] + ss.len[jj]; r.ny = jj - j + 1; }
                 ++jj;
