@@ -18,6 +18,8 @@
 // work items).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 #include "motex_internal.h"
 #include "hd_device.h"
 
@@ -50,31 +52,106 @@ __global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n, uint32_t v
     for (; i < n; i += stride) p[i] = val;
 }
 
+// Folds the queued hit words of the current text range into the per-row accumulators.
+// Entry q of the wave's queue holds the tag r | lane << 16 (row r of the item, lane of the
+// wave) and the lane's B hit words of that row; bit t of word b flags text position
+// pt = q0 + 2048 b + 32 lane + t + px.  Each lane takes whole entries.  A word is split
+// at the sequence starts it spans (found once per word by binary search over the range's
+// starts in LDS); within one sequence y the flags that count are those inside the text
+// range whose window lies entirely in y (pt >= start(y) + k - 1).  Their number goes to
+// the row's v (LDS atomic add) and y to the row's sequence mask (LDS atomic OR).
+template <int B>
+__device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, int32_t q0,
+                                           uint32_t px_start, const uint32_t *hq_meta,
+                                           const uint32_t *hq_res, uint32_t nq,
+                                           const uint32_t *starts, uint32_t *acc_v,
+                                           uint32_t *ymask)
+{
+    __syncthreads();  // the queue stores of every lane are visible to the loads below
+    const int32_t ys = (int32_t)R.ys, ye = (int32_t)R.ye, km1 = (int32_t)a.k - 1;
+    for (uint32_t q = threadIdx.x; q < nq; q += 64) {
+        const uint32_t meta = hq_meta[q];
+        uint32_t w[B];
+        if constexpr (B == 4) {
+            const uint4 v4 = *reinterpret_cast<const uint4 *>(hq_res + q * 4);
+            w[0] = v4.x; w[1] = v4.y; w[2] = v4.z; w[3] = v4.w;
+        } else if constexpr (B == 2) {
+            const uint2 v2 = *reinterpret_cast<const uint2 *>(hq_res + q * 2);
+            w[0] = v2.x; w[1] = v2.y;
+        } else {
+            w[0] = hq_res[q];
+        }
+        const uint32_t r = meta & 0xFFFFu;
+        const int32_t base0 = q0 + (int32_t)(32 * (meta >> 16) + px_start + r);
+        uint32_t hits = 0, ym = 0;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (!w[b]) continue;
+            const int32_t base = base0 + (int32_t)kDiagPerBlock * b;
+            const int32_t first = base > ys ? base : ys;
+            uint32_t y = 0;
+            if (R.ny > 1) {
+                uint32_t lo = 0, hi = R.ny - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if ((int32_t)starts[mid] <= first) lo = mid; else hi = mid - 1;
+                }
+                y = lo;
+            }
+            for (;;) {
+                int32_t s1 = (int32_t)starts[y + 1];
+                if (s1 > ye) s1 = ye;
+                int32_t lo = (int32_t)starts[y] + km1;
+                if (lo < first) lo = first;
+                int32_t hi = base + 32 < s1 ? base + 32 : s1;
+                if (lo < hi) {
+                    const uint64_t span = ((1ull << (hi - base)) - 1) & ~((1ull << (lo - base)) - 1);
+                    const uint32_t hv = w[b] & (uint32_t)span;
+                    if (hv) {
+                        hits += __builtin_popcount(hv);
+                        ym |= 1u << y;
+                    }
+                }
+                if (s1 >= base + 32 || ++y >= R.ny) break;
+            }
+        }
+        if (hits) {
+            atomicAdd(acc_v + r, hits);
+            atomicOr(ymask + r, ym);
+        }
+    }
+}
+
 // Persistent batched kernel: one wave per workgroup pulls items from the queue head.
 //
-// Hit folding is lane-parallel.  A lane with hit flags in a row checks each flag (text
-// range, complete window inside its sequence), finds the flag's sequence among the range's
-// at most kMaxRangeSeqs starts (LDS binary search) and adds its count to the row's v with
-// an LDS atomic add and its sequences to the row's sequence mask with an LDS atomic OR.
-// When a range is done, each row's mask popcount (the distinct sequences of that range
-// with an occurrence) is added to the row's u; a range that is part of a long sequence
-// marks u once.  The item's rows are flushed to global u/v at the end.
+// Hit handling is deferred and lane-parallel.  In a row where some lane has hit flags,
+// the lanes with flags append their B hit words to the wave's queue in global memory
+// (slots from the row's ballot, so the append costs a few instructions whatever the
+// number of flags); the queue is drained (drain_hits) when fewer than 64 slots remain and
+// at the end of each text range.  Keeping the queue out of LDS keeps the LDS footprint
+// per wave (which bounds the number of resident waves) to the NE window and the per-row
+// accumulators.  When a range is done, each row's sequence-mask popcount (the distinct
+// sequences of that range with an occurrence) is added to the row's u; a range that is
+// part of a long sequence marks u once.  The item's rows are flushed to global u/v at the
+// end.  The first k rows of an item have no leaving column, so they run a step without it.
 template <int B, int P, int E>
 __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
 {
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x;
     const uint32_t wl = a.wl;
-    // LDS: NE window (sigma + 1 rows, the last all zero) | per-row v, u, sequence mask |
-    //      range sequence starts | x offsets
+    // LDS: NE window (sigma rows; at offset 0 so that the block offsets fit the read
+    //      instructions' immediates) | per-row v, sequence mask | range sequence starts |
+    //      per-row u (16 bit) | x offsets (16 bit)
     uint32_t *ne_l = smem;
-    uint32_t *acc_v = smem + (a.sigma + 1) * wl;
-    uint32_t *acc_u = acc_v + a.rows_max;
-    uint32_t *ymask = acc_u + a.rows_max;
+    uint32_t *acc_v = ne_l + a.sigma * wl;
+    uint32_t *ymask = acc_v + a.rows_max;
     uint32_t *starts = ymask + a.rows_max;
-    uint16_t *xoff = reinterpret_cast<uint16_t *>(starts + kMaxRangeSeqs + 1);
-    const uint32_t zero_row = a.sigma * wl;
-    for (uint32_t w = lane; w < wl; w += 64) ne_l[zero_row + w] = 0;
+    uint16_t *acc_u = reinterpret_cast<uint16_t *>(starts + kMaxRangeSeqs + 1);
+    uint16_t *xoff = acc_u + a.rows_max;
+    // this wave's hit queue: tags, then B words per entry (16-byte aligned)
+    uint32_t *hq_meta = a.hitq + (size_t)blockIdx.x * a.hitq_cap * (B + 1);
+    uint32_t *hq_res = hq_meta + a.hitq_cap;
 
     for (uint32_t round = 0;; ++round) {
         uint32_t it;
@@ -110,79 +187,84 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
             if (lane <= R.ny) starts[lane] = a.seq_start[R.j0 + lane];
             __syncthreads();
 
+            uint32_t nq = 0;  // queued (row, lane) entries, wave-uniform
             uint32_t cnt[B][P];
 #pragma unroll
             for (int b = 0; b < B; ++b)
 #pragma unroll
                 for (int p = 0; p < P; ++p) cnt[b][p] = 0;
 
-            for (uint32_t r = 0; r < nrows; ++r) {
+            // one row: window counters of row r (with the column leaving at r - k when
+            // LEAVE), then the row's hit flags into the queue
+            auto row = [&](uint32_t r, auto leave) {
                 const uint32_t px = I.px_start + r;
                 const uint32_t wn = xoff[r] + (px >> 5) - pxs5 + lane;
-                uint32_t wo, sho;
-                if (r >= a.k) {
+                const uint32_t shn = px & 31;
+                uint32_t wo = 0, sho = 0;
+                if constexpr (decltype(leave)::value) {
                     wo = xoff[r - a.k] + ((px - a.k) >> 5) - pxs5 + lane;
                     sho = (px - a.k) & 31;
-                } else {
-                    wo = zero_row + lane;
-                    sho = 0;
                 }
-                const uint32_t shn = px & 31;
                 uint32_t res[B];
                 uint32_t any = 0;
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
-                    uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 64 * b + 1], ne_l[wn + 64 * b], shn);
-                    uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 64 * b + 1], ne_l[wo + 64 * b], sho);
-                    uint32_t inc = an & ~bo, dec = bo & ~an;
+                    const uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 64 * b + 1], ne_l[wn + 64 * b], shn);
+                    uint32_t inc, dec;
+                    if constexpr (decltype(leave)::value) {
+                        const uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 64 * b + 1], ne_l[wo + 64 * b], sho);
+                        inc = an & ~bo;
+                        dec = bo & ~an;
+                    } else {
+                        inc = an;
+                        dec = 0;
+                    }
 #pragma unroll
                     for (int p = 0; p < P; ++p) {
-                        uint32_t cp = cnt[b][p];
+                        const uint32_t cp = cnt[b][p];
                         cnt[b][p] = cp ^ (inc | dec);
-                        uint32_t ni = cp & inc, nd = ~cp & dec;
+                        const uint32_t ni = cp & inc, nd = ~cp & dec;
                         inc = ni;
                         dec = nd;
                     }
                     res[b] = le_const<P, E>(cnt[b], a.e);
                     any |= res[b];
                 }
-                if (px >= rep0 && __ballot(any != 0) && any) {
-                    uint32_t hits = 0, ym = 0;
-#pragma unroll
-                    for (int b = 0; b < B; ++b) {
-                        uint32_t h = res[b];
-                        while (h) {
-                            uint32_t t = __builtin_ctz(h);
-                            h &= h - 1;
-                            int32_t pt = q0 + (int32_t)(kDiagPerBlock * b + 32 * lane + t + px);
-                            if (pt < (int32_t)R.ys || pt >= (int32_t)R.ye) continue;
-                            uint32_t y = 0;
-                            if (R.ny > 1) {
-                                uint32_t lo = 0, hi = R.ny - 1;
-                                while (lo < hi) {
-                                    uint32_t mid = (lo + hi + 1) >> 1;
-                                    if (starts[mid] <= (uint32_t)pt) lo = mid; else hi = mid - 1;
-                                }
-                                y = lo;
+                if (px >= rep0) {
+                    const uint64_t m = __ballot(any != 0);
+                    if (m) {
+                        if (any) {
+                            const uint32_t slot =
+                                nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                            if constexpr (B == 4) {
+                                *reinterpret_cast<uint4 *>(hq_res + slot * 4) = make_uint4(res[0], res[1], res[2], res[3]);
+                            } else if constexpr (B == 2) {
+                                *reinterpret_cast<uint2 *>(hq_res + slot * 2) = make_uint2(res[0], res[1]);
+                            } else {
+                                hq_res[slot] = res[0];
                             }
-                            if ((uint32_t)pt - starts[y] + 1 < a.k) continue;  // window starts before y
-                            hits++;
-                            ym |= 1u << y;
+                            hq_meta[slot] = r | (lane << 16);
+                        }
+                        nq += (uint32_t)__builtin_popcountll(m);
+                        if (nq > a.hitq_cap - 64) {
+                            drain_hits<B>(a, R, q0, I.px_start, hq_meta, hq_res, nq, starts, acc_v, ymask);
+                            nq = 0;
                         }
                     }
-                    if (hits) {
-                        atomicAdd(acc_v + r, hits);
-                        atomicOr(ymask + r, ym);
-                    }
                 }
-            }
+            };
+            const uint32_t warm = nrows < a.k ? nrows : a.k;
+            for (uint32_t r = 0; r < warm; ++r) row(r, std::false_type{});
+            for (uint32_t r = warm; r < nrows; ++r) row(r, std::true_type{});
+            if (nq) drain_hits<B>(a, R, q0, I.px_start, hq_meta, hq_res, nq, starts, acc_v, ymask);
             __syncthreads();
             // distinct sequences of this range per row
             for (uint32_t r = lane; r < nrows; r += 64) {
-                uint32_t m = ymask[r];
+                const uint32_t m = ymask[r];
                 if (m) {
                     if (I.split != kNoSplit) acc_u[r] = 1;
-                    else acc_u[r] += __builtin_popcount(m);
+                    else acc_u[r] = (uint16_t)(acc_u[r] + __builtin_popcount(m));
                     ymask[r] = 0;
                 }
             }
@@ -190,14 +272,14 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
         __syncthreads();
         // flush the item's rows; a long text split over several items counts once per row
         for (uint32_t r = lane; r < nrows; r += 64) {
-            uint32_t dv = acc_v[r];
+            const uint32_t dv = acc_v[r];
             if (!dv) continue;
             uint32_t du = acc_u[r];
             const uint32_t px = I.px_start + r;
             if (I.split != kNoSplit && du) {
-                uint32_t xf = I.xf_base + px;
-                uint32_t bit = 1u << (xf & 31);
-                uint32_t old = atomicOr(a.split_flags + (size_t)I.split * a.split_words + (xf >> 5), bit);
+                const uint32_t xf = I.xf_base + px;
+                const uint32_t bit = 1u << (xf & 31);
+                const uint32_t old = atomicOr(a.split_flags + (size_t)I.split * a.split_words + (xf >> 5), bit);
                 du = (old & bit) ? 0u : 1u;
             }
             if (du) atomicAdd(a.u + I.uv_base + px, du);
@@ -242,8 +324,9 @@ static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream
 
 size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max)
 {
-    return (size_t)(sigma + 1) * wl * 4 + (size_t)rows_max * 12 + (kMaxRangeSeqs + 1) * 4 +
-           (size_t)((rows_max + 1) & ~1u) * 2;
+    // NE window | v, sequence mask | range starts | u, x offsets (16 bit)
+    return (size_t)sigma * wl * 4 + (size_t)rows_max * 8 + (kMaxRangeSeqs + 1) * 4 +
+           (size_t)rows_max * 4;
 }
 
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)

@@ -12,6 +12,7 @@ constexpr uint32_t kMaxHdK = 64;             // Hamming kernels: counters of up 
 constexpr uint32_t kMaxEdK = 32;             // edit kernels: bit-vectors stay below 64 bits
 constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
 constexpr uint32_t kMaxRangeSeqs = 32;       // sequences per Hamming text range (one mask word)
+constexpr uint32_t kHitQueueCap = 512;       // Hamming hit queue entries per wave (global)
 
 // A text range of the Hamming work queue: whole sequences j0 .. j0+ny-1, or one part of a
 // long sequence j0 (ny = 1), covering text positions [ys, ye).
@@ -43,6 +44,8 @@ struct HdArgs {
     uint32_t *v;
     uint32_t *split_flags;     // split slots x split_words
     uint32_t *queue;           // work-queue head (one counter)
+    uint32_t *hitq;            // per-workgroup hit queues: hitq_cap tags + hitq_cap x B words
+    uint32_t hitq_cap;         // entries per hit queue (a multiple of 4, at least 128)
     uint32_t W;                // words per NE row
     uint32_t nitems;
     uint32_t sigma;

@@ -143,7 +143,7 @@ struct motex_batch {
         if (ev1) hipEventDestroy(ev1);
         if (ev2) hipEventDestroy(ev2);
     }
-    motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2, d_ranges;
+    motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2, d_ranges, d_hitq;
     uint64_t work_items() const { return hd.items.size() + ed_nitems; }
 };
 
@@ -188,6 +188,7 @@ int pair_run(motex_batch *b, uint32_t i, uint32_t j);
 int ed_pair_run(motex_batch *b, uint32_t i, uint32_t j);
 int struct_pair_run(motex_batch *b, uint32_t i, uint32_t j);
 HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t persistent);
+hipError_t hd_alloc_hitq(DevBuf &buf, uint64_t grid, uint32_t blocks);
 void serving_shutdown(motex_ctx *c);
 
 }  // namespace motex

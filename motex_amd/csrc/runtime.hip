@@ -370,11 +370,21 @@ int hd_prepare(motex_batch *b)
     if ((e = launch_ne_build((const uint8_t *)b->d_codes.p, (uint32_t)ss.total, ss.sigma, (uint32_t)W,
                              (uint32_t *)b->d_ne.p, bstream(b))) != hipSuccess)
         return hip_fail(e, "NE build");
+    // hit queues of the persistent grid (one per workgroup)
+    const uint64_t grid = std::min<uint64_t>((uint64_t)c->num_cu * ts.waves_per_cu, b->hd.items.size());
+    if ((e = hd_alloc_hitq(b->d_hitq, grid, ts.blocks)) != hipSuccess) return hip_fail(e, "alloc hit queues");
     b->split_words = (uint32_t)((ss.uv_off[ss.num] + 31) / 32 + 1);
     uint64_t sf = (uint64_t)std::max<uint32_t>(b->hd.split_slots, 1) * b->split_words;
     if ((e = b->d_split.alloc(sf * 4)) != hipSuccess) return hip_fail(e, "alloc split flags");
     if ((e = hipStreamSynchronize(bstream(b))) != hipSuccess) return hip_fail(e, "NE build sync");
     return MOTEX_OK;
+}
+
+hipError_t hd_alloc_hitq(DevBuf &buf, uint64_t grid, uint32_t blocks)
+{
+    const uint64_t need = std::max<uint64_t>(grid, 1) * kHitQueueCap * (blocks + 1) * 4;
+    if (buf.p && buf.bytes >= need) return hipSuccess;
+    return buf.alloc(need);
 }
 
 HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t persistent)
@@ -389,6 +399,8 @@ HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t
     a.v = (uint32_t *)b->d_v.p;
     a.split_flags = (uint32_t *)b->d_split.p;
     a.queue = (uint32_t *)b->d_queue.p;
+    a.hitq = (uint32_t *)b->d_hitq.p;
+    a.hitq_cap = kHitQueueCap;
     a.W = b->hd.W;
     a.nitems = (uint32_t)plan.items.size();
     a.sigma = b->ss.sigma;
@@ -423,6 +435,8 @@ int hd_launch(motex_batch *b)
         HdArgs a = hd_args(b, b->hd, b->d_items, 1);
         uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->num_cu * b->tile.waves_per_cu, b->hd.items.size());
         if (b->grid_cap) grid = std::min(grid, b->grid_cap);
+        if ((uint64_t)grid * kHitQueueCap * (b->tile.blocks + 1) * 4 > b->d_hitq.bytes)
+            return set_error(MOTEX_EINVAL, "internal: Hamming grid of %u beyond its hit queues", grid);
         if ((e = launch_hd(a, b->tile.blocks, grid, s)) != hipSuccess) return hip_fail(e, "launch Hamming kernel");
     }
     return MOTEX_OK;
@@ -500,7 +514,7 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
     if (all.items.empty()) return MOTEX_OK;
     all.rows_max = kPlainRows + k - 1;
     all.wl = hd_lds_words(4, all.rows_max);
-    DevBuf items, ranges, flags;
+    DevBuf items, ranges, flags, hitq;
     uint64_t W = 0;
     int rc = hd_upload_plan(b, all, items, ranges, &W);
     if (rc) return rc;
@@ -508,6 +522,9 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
     if ((e = flags.alloc((uint64_t)std::max<uint32_t>(slots, 1) * b->split_words * 4)) != hipSuccess)
         return hip_fail(e, "alloc plain flags");
     if ((e = launch_fill_u32((uint32_t *)flags.p, flags.bytes / 4, 0, s)) != hipSuccess) return hip_fail(e, "clear flags");
+    uint32_t max_count = 0;
+    for (const Launch &L : launches) max_count = std::max(max_count, L.count);
+    if ((e = hd_alloc_hitq(hitq, max_count, 4)) != hipSuccess) return hip_fail(e, "alloc plain hit queues");
     HdPlan shape;
     shape.rows_max = all.rows_max;
     hipEventRecord(b->ev0, s);
@@ -518,6 +535,7 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
         a.ranges = (const HdRange *)ranges.p;
         a.nitems = L.count;
         a.split_flags = (uint32_t *)flags.p;
+        a.hitq = (uint32_t *)hitq.p;
         if ((e = launch_hd(a, L.blocks, L.count, s)) != hipSuccess) return hip_fail(e, "launch plain kernel");
     }
     hipEventRecord(b->ev1, s);

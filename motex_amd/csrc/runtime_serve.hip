@@ -63,6 +63,7 @@ static size_t footprint(const Job &j, const motex_ctx *c)
     } else if (j.params.distance == 0) {
         bytes += 32 * (T + kPadBits + 16384) / 8;                // NE tables (up to 32 symbols)
         bytes += (T / 256 + 1) * (T / 7000 + 1) * sizeof(HdItem); // work items (upper bound)
+        bytes += grid * 2 * kHitQueueCap * 5 * 4;                 // hit queues (uncapped grid)
     } else {
         bytes += grid * 2 * (maxlen + 128) * 8 + (uint64_t)j.lens.size() * j.lens.size() * sizeof(EdItemDev);
     }
