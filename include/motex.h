@@ -81,6 +81,7 @@ typedef struct motex_stats {
     uint32_t tile_rows;
     uint32_t waves_per_cu;
     uint32_t kernel_variant;
+    uint32_t graph_replay;  /* 1 if the last run replayed the batch's HIP graph */
 } motex_stats;
 
 /* A serving job.  The caller keeps seqs/lens/params and the output arrays alive until
@@ -134,6 +135,24 @@ int motex_batch_clear(motex_batch *b);
 int motex_batch_download(motex_batch *b, uint32_t *const *u, uint32_t *const *v);
 int motex_batch_stats(const motex_batch *b, motex_stats *out);
 void motex_batch_destroy(motex_batch *b);
+
+/* --- tile-shape autotuner (no counterpart in the reference) ------------------------ */
+
+/* Tile shape picked by motex_autotune. */
+typedef struct motex_tile {
+    uint32_t blocks;        /* Hamming: 2048-diagonal blocks per wave */
+    uint32_t rows;          /* Hamming: x rows per work item */
+    uint32_t waves_per_cu;  /* persistent grid = CUs x waves_per_cu */
+    uint32_t candidates;    /* shapes timed */
+    double kernel_ms;       /* batched-kernel time of the winner on the sample */
+} motex_tile;
+
+/* Times the candidate tile shapes of the batched kernel for these parameters on the given
+ * sample of sequences and keeps the fastest in the context: later batches and jobs with
+ * the same (distance/structured, alphabet size, k, e) use it.  Results never depend on the
+ * tile shape, only the speed does. */
+int motex_autotune(motex_ctx *ctx, const char *const *seqs, const uint32_t *lens,
+                   uint32_t num_seqs, const motex_params *params, motex_tile *best);
 
 /* --- serving: continuous batching of extraction jobs ------------------------------- */
 

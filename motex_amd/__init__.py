@@ -39,7 +39,13 @@ class Stats(C.Structure):
     _fields_ = [("cells", C.c_double), ("kernel_ms", C.c_double), ("main_kernel_ms", C.c_double),
                 ("work_items", C.c_uint64), ("pairs", C.c_uint64), ("skipped_pairs", C.c_uint64),
                 ("tile_blocks", C.c_uint32), ("tile_rows", C.c_uint32),
-                ("waves_per_cu", C.c_uint32), ("kernel_variant", C.c_uint32)]
+                ("waves_per_cu", C.c_uint32), ("kernel_variant", C.c_uint32),
+                ("graph_replay", C.c_uint32)]
+
+
+class Tile(C.Structure):
+    _fields_ = [("blocks", C.c_uint32), ("rows", C.c_uint32), ("waves_per_cu", C.c_uint32),
+                ("candidates", C.c_uint32), ("kernel_ms", C.c_double)]
 
 
 class Job(C.Structure):
@@ -64,6 +70,8 @@ SIGNATURES = [
     ("motex_batch_download", C.c_int, [C.c_void_p, C.POINTER(u32p), C.POINTER(u32p)]),
     ("motex_batch_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("motex_batch_destroy", None, [C.c_void_p]),
+    ("motex_autotune", C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), u32p, C.c_uint32,
+                                 C.POINTER(Params), C.POINTER(Tile)]),
     ("motex_submit", C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_uint64)]),
     ("motex_poll", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]),
     ("motex_wait", C.c_int, [C.c_void_p, C.c_uint64]),
@@ -181,6 +189,16 @@ class Context:
         _check(lib().motex_extract_pair(self.h, x, len(x), y, len(y), C.byref(p),
                                         u.ctypes.data_as(u32p), v.ctypes.data_as(u32p)))
         return u[:n], v[:n]
+
+    def autotune(self, seqs: list[bytes], params: MotexParams) -> dict:
+        """Times the tile shapes on this sample; later batches with the same key use the
+        winner.  Returns the winning shape."""
+        sa, lens = _seq_args(seqs)
+        p, keep = params.to_c()
+        t = Tile()
+        _check(lib().motex_autotune(self.h, sa, lens.ctypes.data_as(u32p), len(seqs), C.byref(p),
+                                    C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tile._fields_}
 
     def batch(self, seqs: list[bytes], params: MotexParams) -> "Batch":
         return Batch(self, seqs, params)

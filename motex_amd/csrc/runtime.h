@@ -137,8 +137,11 @@ struct motex_batch {
     motex_stats stats{};
     hipStream_t stream = nullptr;   // overrides ctx->stream when set (serving)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // run start, end, main kernel start
+    hipGraphExec_t graph = nullptr;  // replay of clear + batched kernel(s), captured on run 2
+    uint32_t runs = 0;
     ~motex_batch()
     {
+        if (graph) (void)hipGraphExecDestroy(graph);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (ev2) hipEventDestroy(ev2);
@@ -153,6 +156,8 @@ inline hipStream_t bstream(const motex_batch *b) { return b->stream ? b->stream 
 int batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num,
                  const motex_params *p, hipStream_t stream, motex_batch **out);
 int batch_launch(motex_batch *b);   // enqueue clear + batched kernel, no wait
+int batch_launch_graph(motex_batch *b);  // same, replayed from a HIP graph after the first run
+hipError_t batch_realloc_scratch(motex_batch *b, uint64_t bytes);  // grows d_aux, drops the graph
 int batch_finish(motex_batch *b);   // wait for the last launch, record its timings
 
 std::vector<uint32_t> enumerate_structs(const motex_params *p, uint32_t *nb_structs);

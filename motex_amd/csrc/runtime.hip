@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -442,6 +443,22 @@ int hd_launch(motex_batch *b)
     return MOTEX_OK;
 }
 
+hipError_t batch_realloc_scratch(motex_batch *b, uint64_t bytes)
+{
+    if (b->graph) {  // the captured graph holds the old scratch address
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return b->d_aux.alloc(bytes);
+}
+
+static int batch_kernels(motex_batch *b)
+{
+    if (b->params.nb_gaps) return struct_launch_all(b);
+    if (b->params.distance == 0) return hd_launch(b);
+    return ed_launch_all(b);
+}
+
 int batch_launch(motex_batch *b)
 {
     hipStream_t s = bstream(b);
@@ -451,11 +468,54 @@ int batch_launch(motex_batch *b)
     int rc = batch_clear(b);
     if (rc) return rc;
     if ((e = hipEventRecord(b->ev2, s)) != hipSuccess) return hip_fail(e, "event");
-    if (b->params.nb_gaps) rc = struct_launch_all(b);
-    else if (b->params.distance == 0) rc = hd_launch(b);
-    else rc = ed_launch_all(b);
-    if (rc) return rc;
+    if ((rc = batch_kernels(b))) return rc;
     if ((e = hipEventRecord(b->ev1, s)) != hipSuccess) return hip_fail(e, "event");
+    b->runs++;
+    b->stats.graph_replay = 0;
+    return MOTEX_OK;
+}
+
+// Runs of a batch replay a HIP graph of the run's work: the clears (fill kernels, no
+// memset nodes) and the batched kernel(s).  The graph is captured on the first run; every
+// buffer it names was allocated at batch creation, so capture records no allocation.
+// MOTEX_GRAPH=0 keeps every run eager.
+static bool graphs_enabled()
+{
+    const char *env = getenv("MOTEX_GRAPH");
+    return !(env && env[0] == '0');
+}
+
+int batch_launch_graph(motex_batch *b)
+{
+    if (!graphs_enabled()) return batch_launch(b);
+    hipStream_t s = bstream(b);
+    hipError_t e;
+    if (!b->graph) {
+        AllocStreamGuard guard(b->stream);
+        if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return hip_fail(e, "begin capture");
+        int rc = batch_clear(b);
+        if (!rc) rc = batch_kernels(b);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "end capture");
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->graph = nullptr;
+            return hip_fail(e, "instantiate graph");
+        }
+    }
+    if ((e = hipEventRecord(b->ev0, s)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventRecord(b->ev2, s)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipGraphLaunch(b->graph, s)) != hipSuccess) return hip_fail(e, "graph launch");
+    if ((e = hipEventRecord(b->ev1, s)) != hipSuccess) return hip_fail(e, "event");
+    b->runs++;
+    b->stats.graph_replay = 1;
     return MOTEX_OK;
 }
 
@@ -644,7 +704,7 @@ extern "C" int motex_batch_run(motex_batch *b)
 {
     if (!b) return set_error(MOTEX_EINVAL, "batch is NULL");
     hipSetDevice(b->ctx->device);
-    int rc = batch_launch(b);
+    int rc = batch_launch_graph(b);
     return rc ? rc : batch_finish(b);
 }
 

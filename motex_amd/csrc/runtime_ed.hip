@@ -54,6 +54,19 @@ static uint64_t ed_item_cost(const SeqSet &ss, const EdItemDev &it, bool wide)
     return strips * steps;
 }
 
+// Boundary-stream words per wave (two streams of this many words each).
+static uint32_t ed_stream_words(const motex_batch *b)
+{
+    uint32_t maxn = 0;
+    for (uint32_t j = 0; j < b->ss.num; ++j) maxn = std::max(maxn, b->ss.len[j]);
+    return (maxn + 128) * (ed_wide(b->params) ? 2 : 1);
+}
+
+static uint32_t ed_grid(const motex_batch *b)
+{
+    return (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, b->ed_nitems);
+}
+
 int ed_prepare(motex_batch *b)
 {
     const SeqSet &ss = b->ss;
@@ -79,6 +92,9 @@ int ed_prepare(motex_batch *b)
         (e = hipMemcpyAsync(b->d_items.p, sorted.data(), sorted.size() * sizeof(EdItemDev), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
     b->ed_nitems = (uint32_t)sorted.size();
+    // boundary streams of the persistent grid, allocated here so that runs allocate nothing
+    const uint64_t need = (uint64_t)std::max<uint32_t>(ed_grid(b), 1) * 2 * ed_stream_words(b) * 4;
+    if ((e = batch_realloc_scratch(b, need)) != hipSuccess) return hip_fail(e, "alloc edit streams");
     if ((e = hipStreamSynchronize(bstream(b))) != hipSuccess) return hip_fail(e, "upload items");
     return MOTEX_OK;
 }
@@ -89,11 +105,9 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
     hipError_t e;
     const SeqSet &ss = b->ss;
     bool wide = ed_wide(b->params);
-    uint32_t maxn = 0;
-    for (uint32_t j = 0; j < ss.num; ++j) maxn = std::max(maxn, ss.len[j]);
-    uint32_t stream_words = (maxn + 128) * (wide ? 2 : 1);
+    uint32_t stream_words = ed_stream_words(b);
     uint64_t need = (uint64_t)grid * 2 * stream_words * 4;
-    if (b->d_aux.bytes < need && (e = b->d_aux.alloc(need)) != hipSuccess) return hip_fail(e, "alloc edit streams");
+    if (b->d_aux.bytes < need && (e = batch_realloc_scratch(b, need)) != hipSuccess) return hip_fail(e, "alloc edit streams");
     uint32_t maxm = 0;
     for (uint32_t i = 0; i < ss.num; ++i) maxm = std::max(maxm, ss.len[i]);
     uint32_t lds_cols = std::min(maxm, kEdLdsCols);
@@ -119,7 +133,7 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
 int ed_launch_all(motex_batch *b)
 {
     if (!b->ed_nitems) return MOTEX_OK;
-    uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, b->ed_nitems);
+    uint32_t grid = ed_grid(b);
     if (b->grid_cap) grid = std::min(grid, b->grid_cap);
     return ed_launch(b, b->d_items.p, b->ed_nitems, grid, 1, bstream(b));
 }

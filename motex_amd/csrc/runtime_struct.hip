@@ -13,6 +13,8 @@ hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStre
 
 static const uint32_t kStructHitCap = 4096;
 
+static int struct_alloc_scratch(motex_batch *b);
+
 static size_t struct_lds_bytes(const StructArgs &a)
 {
     return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + 1) * 4;
@@ -111,21 +113,20 @@ int struct_prepare(motex_batch *b)
         for (uint32_t d = 0; d < a.ndist; ++d)
             if (it.m >= a.dlen[d]) cells += (double)(it.m - a.dlen[d] + 1) * it.n;
     b->stats.cells = cells;
+    if ((rc = struct_alloc_scratch(b))) return rc;
     return hipStreamSynchronize(bstream(b)) == hipSuccess ? MOTEX_OK : set_error(MOTEX_EDEVICE, "NE build");
 }
 
-static int struct_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_t grid,
-                         uint32_t persistent, hipStream_t s)
+// Fills every StructArgs field that does not depend on the launch (items, grid).
+static int struct_fill_args(motex_batch *b, StructArgs &a)
 {
     const SeqSet &ss = b->ss;
-    StructArgs a{};
     int rc = struct_shape(b, a);
     if (rc) return rc;
     uint32_t maxlen = 0;
     for (uint32_t i = 0; i < ss.num; ++i) maxlen = std::max(maxlen, ss.len[i]);
     a.ne = (const uint32_t *)b->d_ne.p;
     a.codes = (const uint8_t *)b->d_codes.p;
-    a.items = (const StructItem *)items;
     a.S = (const uint32_t *)b->d_aux2.p;
     a.u = (uint32_t *)b->d_u.p;
     a.v = (uint32_t *)b->d_v.p;
@@ -133,17 +134,27 @@ static int struct_launch(motex_batch *b, const void *items, uint32_t nitems, uin
     a.W = b->hd.W;
     a.sigma = ss.sigma;
     a.wl = b->params.distance == 0 ? b->hd.wl : 0;
-    a.nitems = nitems;
-    a.persistent = persistent;
     a.m_max = std::max<uint32_t>(maxlen, 1);
     a.hcap = kStructHitCap;
     a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
     a.stream_words = (maxlen + 128) * 2;
     a.scratch_words = a.ndist * a.bm_words + 2ull * a.stream_words +
                       ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1;
+    return MOTEX_OK;
+}
+
+static int struct_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_t grid,
+                         uint32_t persistent, hipStream_t s)
+{
+    StructArgs a{};
+    int rc = struct_fill_args(b, a);
+    if (rc) return rc;
+    a.items = (const StructItem *)items;
+    a.nitems = nitems;
+    a.persistent = persistent;
     uint64_t need = (uint64_t)grid * a.scratch_words * 4;
     hipError_t e;
-    if (b->d_aux.bytes < need && (e = b->d_aux.alloc(need)) != hipSuccess) return hip_fail(e, "alloc structured scratch");
+    if (b->d_aux.bytes < need && (e = batch_realloc_scratch(b, need)) != hipSuccess) return hip_fail(e, "alloc structured scratch");
     a.scratch = (uint32_t *)b->d_aux.p;
     size_t lds = struct_lds_bytes(a);
     if (lds > 65536) return set_error(MOTEX_EUNSUPPORTED, "structured kernel LDS window too large");
@@ -162,6 +173,17 @@ static uint32_t struct_grid(motex_batch *b, uint32_t nitems)
     uint64_t cap = std::max<uint64_t>(1, (2ull << 30) / std::max<uint64_t>(per, 1));
     uint64_t g = std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, nitems);
     return (uint32_t)std::max<uint64_t>(1, std::min(g, cap));
+}
+
+// Scratch of the persistent grid, allocated at batch creation so that runs allocate nothing.
+static int struct_alloc_scratch(motex_batch *b)
+{
+    StructArgs a{};
+    int rc = struct_fill_args(b, a);
+    if (rc) return rc;
+    const uint64_t need = (uint64_t)struct_grid(b, std::max<uint32_t>(b->ed_nitems, 1)) * a.scratch_words * 4;
+    hipError_t e = batch_realloc_scratch(b, need);
+    return e == hipSuccess ? MOTEX_OK : hip_fail(e, "alloc structured scratch");
 }
 
 int struct_launch_all(motex_batch *b)

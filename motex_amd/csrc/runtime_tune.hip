@@ -1,19 +1,68 @@
-// Tile-shape selection for the batched kernels.
+// Tile-shape selection for the batched kernels: built-in defaults per (alphabet size,
+// distance), a per-context cache of shapes picked by timing (motex_autotune), and the
+// MOTEX_TILE override for tuning runs.
+//
+// The tunable parameters:
+//   Hamming     blocks (2048-diagonal blocks per wave: 1, 2, 4), rows (x rows per work
+//               item) and waves_per_cu (persistent grid).  blocks x alphabet size sets the
+//               NE window in LDS and rows the per-row accumulators; together they bound the
+//               waves resident per CU, which the Hamming sweep is sensitive to.
+//   edit        waves_per_cu.
+//   structured  waves_per_cu.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include "runtime.h"
 
 namespace motex {
 
-TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss)
+// Shape the autotuner is timing on this thread (overrides the cache and the defaults).
+static thread_local const TileShape *t_force_tile = nullptr;
+
+static std::tuple<uint32_t, uint32_t, uint32_t, uint32_t> tune_key(const motex_params &p, const SeqSet &ss)
+{
+    const uint32_t kind = p.nb_gaps ? 2u : p.distance;
+    return {kind, ss.sigma, p.k, p.e};
+}
+
+// Defaults from sweeps on MI355X: the Hamming kernel runs best with the smallest LDS
+// footprint that still amortises the k-1 warm-up rows of an item (128 rows), at full
+// occupancy (32 waves per CU), with the NE window narrowed for large alphabets.
+static TileShape default_tile(const motex_params &p, const SeqSet &ss)
 {
     TileShape t;
-    t.blocks = ss.sigma <= 8 ? 4 : 2;
-    t.rows = 256;
-    t.waves_per_cu = 24;
+    if (!p.nb_gaps && p.distance == 0) {
+        t.blocks = ss.sigma <= 8 ? 4 : (ss.sigma <= 24 ? 2 : 1);
+        t.rows = 128;
+        t.waves_per_cu = 32;
+    } else {
+        t.blocks = 4;
+        t.rows = 256;
+        t.waves_per_cu = 24;
+    }
+    return t;
+}
+
+static bool hd_fits(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows)
+{
+    const uint32_t rows_max = rows + k;
+    const uint32_t wl = hd_lds_words(blocks, rows_max);
+    return hd_lds_bytes(ss.sigma, wl, rows_max) <= 65536 && (uint64_t)ss.sigma * wl < 65536;
+}
+
+TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss)
+{
+    TileShape t = default_tile(p, ss);
+    {
+        std::lock_guard<std::mutex> lk(c->tune_mu);
+        auto it = c->tune_cache.find(tune_key(p, ss));
+        if (it != c->tune_cache.end()) t = it->second;
+    }
+    if (t_force_tile) t = *t_force_tile;
     if (const char *env = getenv("MOTEX_TILE")) {  // "blocks,rows,waves_per_cu" (tuning runs)
         unsigned b = 0, r = 0, w = 0;
         if (sscanf(env, "%u,%u,%u", &b, &r, &w) == 3 && (b == 1 || b == 2 || b == 4) && r >= 32 && w >= 1) {
@@ -23,14 +72,88 @@ TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss)
         }
     }
     // keep the Hamming kernel's LDS window under 64 KB
-    while (t.blocks > 1 &&
-           hd_lds_bytes(ss.sigma, hd_lds_words(t.blocks, t.rows + p.k), t.rows + p.k) > 65536)
-        t.blocks /= 2;
-    while (t.rows > 64 &&
-           hd_lds_bytes(ss.sigma, hd_lds_words(t.blocks, t.rows + p.k), t.rows + p.k) > 65536)
-        t.rows /= 2;
-    (void)c;
+    while (t.blocks > 1 && !hd_fits(ss, p.k, t.blocks, t.rows)) t.blocks /= 2;
+    while (t.rows > 64 && !hd_fits(ss, p.k, t.blocks, t.rows)) t.rows /= 2;
     return t;
 }
 
+// Times one candidate shape on the caller's sequences: one untimed run, then the best of
+// two timed runs of the batched kernel (HIP events on the context stream).
+static int time_shape(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num,
+                      const motex_params *p, const TileShape &shape, double *ms)
+{
+    t_force_tile = &shape;
+    motex_batch *b = nullptr;
+    int rc = batch_create(c, seqs, lens, num, p, nullptr, &b);
+    t_force_tile = nullptr;
+    if (rc) return rc;
+    double best = 1e30;
+    for (int rep = 0; rep < 3 && !rc; ++rep) {
+        rc = batch_launch(b);
+        if (!rc) rc = batch_finish(b);
+        if (!rc && rep > 0) best = std::min(best, b->stats.main_kernel_ms);
+    }
+    motex_batch_destroy(b);
+    *ms = best;
+    return rc;
+}
+
 }  // namespace motex
+
+using namespace motex;
+
+extern "C" int motex_autotune(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num_seqs,
+                              const motex_params *p, motex_tile *best)
+{
+    if (!c) return set_error(MOTEX_EINVAL, "ctx is NULL");
+    if (!seqs || !lens || !num_seqs) return set_error(MOTEX_EINVAL, "autotune needs a sample of sequences");
+    int rc = validate_params(p);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    uint32_t nb_structs = 1;
+    enumerate_structs(p, &nb_structs);
+    SeqSet ss;
+    if ((rc = ss.build(seqs, lens, num_seqs, nb_structs))) return rc;
+    std::vector<TileShape> cand;
+    if (!p->nb_gaps && p->distance == 0) {
+        for (uint32_t blocks : {1u, 2u, 4u})
+            for (uint32_t rows : {64u, 128u, 192u, 256u})
+                for (uint32_t waves : {24u, 32u}) {
+                    if (!hd_fits(ss, p->k, blocks, rows)) continue;
+                    TileShape t;
+                    t.blocks = blocks;
+                    t.rows = rows;
+                    t.waves_per_cu = waves;
+                    cand.push_back(t);
+                }
+    } else {
+        for (uint32_t waves : {8u, 16u, 24u, 32u}) {
+            TileShape t = default_tile(*p, ss);
+            t.waves_per_cu = waves;
+            cand.push_back(t);
+        }
+    }
+    if (cand.empty()) return set_error(MOTEX_EUNSUPPORTED, "no tile shape fits this alphabet");
+    TileShape win = cand[0];
+    double win_ms = 1e30;
+    for (const TileShape &t : cand) {
+        double ms = 0;
+        if ((rc = time_shape(c, seqs, lens, num_seqs, p, t, &ms))) return rc;
+        if (ms < win_ms) {
+            win_ms = ms;
+            win = t;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(c->tune_mu);
+        c->tune_cache[tune_key(*p, ss)] = win;
+    }
+    if (best) {
+        best->blocks = win.blocks;
+        best->rows = win.rows;
+        best->waves_per_cu = win.waves_per_cu;
+        best->candidates = (uint32_t)cand.size();
+        best->kernel_ms = win_ms;
+    }
+    return MOTEX_OK;
+}
