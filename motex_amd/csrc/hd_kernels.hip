@@ -45,6 +45,42 @@ __global__ void ne_build_kernel(const uint8_t *__restrict__ codes, uint32_t tota
     }
 }
 
+// u of the text side of pair-symmetric runs: for every text position p, the number of x
+// sequences whose bit (x, p) is set.  One thread per 32-position word; the 32 counts are
+// kept as bit-sliced planes (plane q = bit q of each count), so adding one x's word is a
+// ripple-carry increment of the planes.
+__global__ void fold_fbits_kernel(const uint32_t *__restrict__ fbits, uint32_t nx, uint32_t f_words,
+                                  uint32_t total, uint32_t *__restrict__ u)
+{
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= f_words) return;
+    uint32_t plane[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) plane[q] = 0;
+    for (uint32_t x = 0; x < nx; ++x) {
+        uint32_t carry = fbits[(size_t)x * f_words + w];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t t = plane[q] & carry;
+            plane[q] ^= carry;
+            carry = t;
+        }
+    }
+    uint32_t any = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) any |= plane[q];
+    while (any) {
+        const uint32_t b = __builtin_ctz(any);
+        any &= any - 1;
+        const uint32_t p = w * 32 + b;
+        if (p >= total) break;
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) cnt |= ((plane[q] >> b) & 1u) << q;
+        u[p] += cnt;
+    }
+}
+
 __global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n, uint32_t val)
 {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -60,11 +96,16 @@ __global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n, uint32_t v
 // starts in LDS); within one sequence y the flags that count are those inside the text
 // range whose window lies entirely in y (pt >= start(y) + k - 1).  Their number goes to
 // the row's v (LDS atomic add) and y to the row's sequence mask (LDS atomic OR).
+//
+// Pair-symmetric items (sym): texts j < xi are dropped (their pairs are computed with the
+// roles swapped), and an occurrence in a text j > xi is also credited to the text side:
+// v of (j, text position) += 1 and the (xi, text position) bit of fbits is set, from which
+// hd_fold_fbits adds the distinct-x count to u after the kernel.
 template <int B>
 __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, int32_t q0,
-                                           uint32_t px_start, const uint32_t *hq_meta,
-                                           const uint32_t *hq_res, uint32_t nq,
-                                           const uint32_t *starts, uint32_t *acc_v,
+                                           uint32_t px_start, uint32_t xi, bool sym,
+                                           const uint32_t *hq_meta, const uint32_t *hq_res,
+                                           uint32_t nq, const uint32_t *starts, uint32_t *acc_v,
                                            uint32_t *ymask)
 {
     __syncthreads();  // the queue stores of every lane are visible to the loads below
@@ -104,12 +145,30 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
                 int32_t lo = (int32_t)starts[y] + km1;
                 if (lo < first) lo = first;
                 int32_t hi = base + 32 < s1 ? base + 32 : s1;
-                if (lo < hi) {
+                const uint32_t j = R.j0 + y;
+                if (lo < hi && (!sym || j >= xi)) {
                     const uint64_t span = ((1ull << (hi - base)) - 1) & ~((1ull << (lo - base)) - 1);
                     const uint32_t hv = w[b] & (uint32_t)span;
                     if (hv) {
                         hits += __builtin_popcount(hv);
                         ym |= 1u << y;
+                        if (sym && j > xi) {
+                            // text side: one v per occurrence, one bit per (xi, position)
+                            for (uint32_t h2 = hv; h2;) {
+                                const uint32_t t = __builtin_ctz(h2);
+                                h2 &= h2 - 1;
+                                atomicAdd(a.v + (uint32_t)(base + (int32_t)t), 1u);
+                            }
+                            // the flags' positions (all >= 0) span words w0 and w0 + 1 of
+                            // fbits; w0 is -1 when base < 0, and then holds none of them
+                            const int32_t w0 = base >> 5;
+                            const uint32_t sh = (uint32_t)base & 31;
+                            const uint32_t lo_bits = hv << sh;
+                            const uint32_t hi_bits = sh ? hv >> (32 - sh) : 0u;
+                            uint32_t *frow = a.fbits + (size_t)xi * a.f_words;
+                            if (lo_bits) atomicOr(frow + w0, lo_bits);
+                            if (hi_bits) atomicOr(frow + (w0 + 1), hi_bits);
+                        }
                     }
                 }
                 if (s1 >= base + 32 || ++y >= R.ny) break;
@@ -248,7 +307,7 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
                         }
                         nq += (uint32_t)__builtin_popcountll(m);
                         if (nq > a.hitq_cap - 64) {
-                            drain_hits<B>(a, R, q0, I.px_start, hq_meta, hq_res, nq, starts, acc_v, ymask);
+                            drain_hits<B>(a, R, q0, I.px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask);
                             nq = 0;
                         }
                     }
@@ -257,7 +316,7 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
             const uint32_t warm = nrows < a.k ? nrows : a.k;
             for (uint32_t r = 0; r < warm; ++r) row(r, std::false_type{});
             for (uint32_t r = warm; r < nrows; ++r) row(r, std::true_type{});
-            if (nq) drain_hits<B>(a, R, q0, I.px_start, hq_meta, hq_res, nq, starts, acc_v, ymask);
+            if (nq) drain_hits<B>(a, R, q0, I.px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask);
             __syncthreads();
             // distinct sequences of this range per row
             for (uint32_t r = lane; r < nrows; r += 64) {
@@ -339,6 +398,15 @@ hipError_t launch_ne_build(const uint8_t *codes, uint32_t total, uint32_t sigma,
 {
     hipLaunchKernelGGL(ne_build_kernel, dim3((W + 255) / 256), dim3(256), 0, s, codes, total,
                        sigma, W, ne);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_fbits(const uint32_t *fbits, uint32_t nx, uint32_t f_words, uint32_t total,
+                             uint32_t *u, hipStream_t s)
+{
+    if (nx > 65535) return hipErrorInvalidValue;  // counts are 16-bit planes
+    hipLaunchKernelGGL(fold_fbits_kernel, dim3((f_words + 255) / 256), dim3(256), 0, s, fbits, nx, f_words,
+                       total, u);
     return hipGetLastError();
 }
 

@@ -31,7 +31,10 @@ struct HdItem {
     uint32_t rb, re;    // text ranges
     uint32_t split;     // kNoSplit (whole texts only) or the split slot of the one long text
                         // every range of the item is a part of
-    uint32_t pad[3];
+    uint32_t xi;        // sequence index of x
+    uint32_t sym;       // 1: pair-symmetric item (texts j < xi dropped, j > xi also
+                        //    credited to the text side)
+    uint32_t pad;
 };
 
 struct HdArgs {
@@ -44,6 +47,9 @@ struct HdArgs {
     uint32_t *v;
     uint32_t *split_flags;     // split slots x split_words
     uint32_t *queue;           // work-queue head (one counter)
+    uint32_t *fbits;           // pair-symmetric runs: bit (xi, text position) = text position
+                               // has an occurrence of some factor of x_xi (f_words per x)
+    uint32_t f_words;
     uint32_t *hitq;            // per-workgroup hit queues: hitq_cap tags + hitq_cap x B words
     uint32_t hitq_cap;         // entries per hit queue (a multiple of 4, at least 128)
     uint32_t W;                // words per NE row

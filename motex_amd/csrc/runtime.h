@@ -147,6 +147,8 @@ struct motex_batch {
         if (ev2) hipEventDestroy(ev2);
     }
     motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2, d_ranges, d_hitq;
+    motex::DevBuf d_fbits;           // pair-symmetric Hamming runs: x-by-text occurrence bits
+    uint32_t f_words = 0;
     uint64_t work_items() const { return hd.items.size() + ed_nitems; }
 };
 
@@ -164,7 +166,7 @@ std::vector<uint32_t> enumerate_structs(const motex_params *p, uint32_t *nb_stru
 uint32_t total_motif_length(const motex_params *p);
 void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
                    const std::vector<uint32_t> &xs, const std::vector<int32_t> *only_y,
-                   HdPlan &plan, uint32_t group_max);
+                   HdPlan &plan, uint32_t group_max, bool sym = false);
 TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss);
 
 // kernels (hd_kernels.hip, ed_kernels.hip)
@@ -174,6 +176,8 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t val, hipStream_t s)
 hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, size_t lds, hipStream_t s);
 uint32_t ed_strip_cols(bool wide);
 hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_t s);
+hipError_t launch_fold_fbits(const uint32_t *fbits, uint32_t nx, uint32_t f_words, uint32_t total,
+                             uint32_t *u, hipStream_t s);
 size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max);
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max);
 

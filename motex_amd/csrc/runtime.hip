@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <cmath>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -147,7 +148,7 @@ static int64_t floor32(int64_t v) { return v >= 0 ? (v & ~31LL) : -(((-v) + 31) 
 
 void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
                    const std::vector<uint32_t> &xs, const std::vector<int32_t> *only_y,
-                   HdPlan &plan, uint32_t group_max)
+                   HdPlan &plan, uint32_t group_max, bool sym)
 {
     const int64_t cap = (int64_t)kDiagPerBlock * blocks - rows - 32;
     plan.items.clear();
@@ -205,9 +206,16 @@ void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
     for (uint32_t i : xs) {
         uint32_t m = ss.len[i];
         if (m < k) continue;
+        // pair-symmetric plan: only the ranges from the one holding text i onwards
+        uint32_t r_first = 0;
+        if (sym)
+            while (r_first < plan.ranges.size() &&
+                   plan.ranges[r_first].j0 + plan.ranges[r_first].ny - 1 < i)
+                ++r_first;
         for (uint32_t r0 = k - 1; r0 < m; r0 += rows) {
             uint32_t r1 = std::min(m, r0 + rows);
             for (const Group &g : groups) {
+                if (g.re <= r_first) continue;
                 HdItem it{};
                 it.x_start = ss.start[i];
                 it.uv_base = (uint32_t)ss.uv_off[i];
@@ -215,9 +223,11 @@ void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
                 it.r0 = r0;
                 it.r1 = r1;
                 it.px_start = r0 - (k - 1);
-                it.rb = g.rb;
+                it.rb = std::max(g.rb, r_first);
                 it.re = g.re;
                 it.split = g.split;
+                it.xi = i;
+                it.sym = sym ? 1u : 0u;
                 plan.items.push_back(it);
             }
         }
@@ -329,6 +339,34 @@ int batch_upload_common(motex_batch *b)
     return MOTEX_OK;
 }
 
+// Pair symmetry of the Hamming path: the occurrences of ordered pair (j, i) are those of
+// (i, j) with the roles of factor and text position swapped, so each unordered pair can be
+// computed once and credited to both sides.  The text side's u needs, per (x sequence,
+// text position), whether any factor of x occurs there: one bit each (d_fbits), folded into
+// u by hd_fold_fbits after the kernel.  Worth it when occurrences are rare (each costs two
+// global atomics on the text side), the bitmap is small and no text is split across work
+// items.  MOTEX_SYM=0 / 1 forces it off / on (where allowed).
+static bool hd_symmetric(motex_batch *b, const HdPlan &plan)
+{
+    const SeqSet &ss = b->ss;
+    if (plan.split_slots || b->nb_structs != 1 || ss.num < 2 || ss.num > 65535) return false;
+    const uint64_t f_words = (ss.total + 31) / 32 + 1;
+    if ((uint64_t)ss.num * f_words * 4 > (4ull << 30)) return false;
+    if (const char *env = getenv("MOTEX_SYM")) return env[0] == '1';
+    // expected occurrences per cell from the symbol frequencies of the text
+    std::vector<uint64_t> freq(ss.sigma, 0);
+    for (uint8_t c : ss.codes) freq[c]++;
+    double pm = 0;
+    for (uint64_t f : freq) pm += ((double)f / ss.total) * ((double)f / ss.total);
+    const uint32_t k = b->params.k, e = b->params.e;
+    double p = 0, binom = 1;
+    for (uint32_t d = 0; d <= e; ++d) {
+        p += binom * std::pow(1 - pm, d) * std::pow(pm, k - d);
+        binom = binom * (k - d) / (d + 1);
+    }
+    return p < 1e-4;
+}
+
 int hd_prepare(motex_batch *b)
 {
     motex_ctx *c = b->ctx;
@@ -339,16 +377,23 @@ int hd_prepare(motex_batch *b)
     for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
     // group size: aim at ~16 items per resident wave so the tail of the queue is short
     plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, 1u << 30);
+    const bool sym = hd_symmetric(b, b->hd);
     uint64_t pairs_units = 0;  // tiles x ranges
     {
-        uint64_t ntiles = 0;
-        for (uint32_t i = 0; i < ss.num; ++i)
-            if (ss.len[i] >= b->params.k) ntiles += (ss.len[i] - b->params.k + 1 + ts.rows - 1) / ts.rows;
-        pairs_units = ntiles * b->hd.ranges.size();
+        size_t r_first = 0;
+        for (uint32_t i = 0; i < ss.num; ++i) {
+            if (ss.len[i] < b->params.k) continue;
+            uint64_t ntiles = (ss.len[i] - b->params.k + 1 + ts.rows - 1) / ts.rows;
+            if (sym)
+                while (r_first < b->hd.ranges.size() &&
+                       b->hd.ranges[r_first].j0 + b->hd.ranges[r_first].ny - 1 < i)
+                    ++r_first;
+            pairs_units += ntiles * (b->hd.ranges.size() - r_first);
+        }
     }
     uint64_t target_items = (uint64_t)c->num_cu * ts.waves_per_cu * 16;
     uint32_t gmax = (uint32_t)std::max<uint64_t>(1, pairs_units / std::max<uint64_t>(target_items, 1));
-    plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, gmax);
+    plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, gmax, sym);
     // longest items first: they set the tail of the persistent schedule
     std::vector<uint64_t> span(b->hd.ranges.size() + 1, 0);
     for (size_t r = 0; r < b->hd.ranges.size(); ++r) span[r + 1] = span[r] + (b->hd.ranges[r].ye - b->hd.ranges[r].ys);
@@ -374,6 +419,11 @@ int hd_prepare(motex_batch *b)
     // hit queues of the persistent grid (one per workgroup)
     const uint64_t grid = std::min<uint64_t>((uint64_t)c->num_cu * ts.waves_per_cu, b->hd.items.size());
     if ((e = hd_alloc_hitq(b->d_hitq, grid, ts.blocks)) != hipSuccess) return hip_fail(e, "alloc hit queues");
+    b->stats.symmetric = sym ? 1u : 0u;
+    if (sym) {
+        b->f_words = (uint32_t)((ss.total + 31) / 32 + 1);
+        if ((e = b->d_fbits.alloc((uint64_t)ss.num * b->f_words * 4)) != hipSuccess) return hip_fail(e, "alloc text bits");
+    }
     b->split_words = (uint32_t)((ss.uv_off[ss.num] + 31) / 32 + 1);
     uint64_t sf = (uint64_t)std::max<uint32_t>(b->hd.split_slots, 1) * b->split_words;
     if ((e = b->d_split.alloc(sf * 4)) != hipSuccess) return hip_fail(e, "alloc split flags");
@@ -402,6 +452,8 @@ HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t
     a.queue = (uint32_t *)b->d_queue.p;
     a.hitq = (uint32_t *)b->d_hitq.p;
     a.hitq_cap = kHitQueueCap;
+    a.fbits = (uint32_t *)b->d_fbits.p;
+    a.f_words = b->f_words;
     a.W = b->hd.W;
     a.nitems = (uint32_t)plan.items.size();
     a.sigma = b->ss.sigma;
@@ -424,6 +476,8 @@ int batch_clear(motex_batch *b)
     if (b->d_split.p && (e = launch_fill_u32((uint32_t *)b->d_split.p, b->d_split.bytes / 4, 0, s)) != hipSuccess)
         return hip_fail(e, "clear flags");
     if ((e = launch_fill_u32((uint32_t *)b->d_queue.p, 64, 0, s)) != hipSuccess) return hip_fail(e, "clear queue");
+    if (b->d_fbits.p && (e = launch_fill_u32((uint32_t *)b->d_fbits.p, b->d_fbits.bytes / 4, 0, s)) != hipSuccess)
+        return hip_fail(e, "clear text bits");
     return MOTEX_OK;
 }
 
@@ -439,6 +493,10 @@ int hd_launch(motex_batch *b)
         if ((uint64_t)grid * kHitQueueCap * (b->tile.blocks + 1) * 4 > b->d_hitq.bytes)
             return set_error(MOTEX_EINVAL, "internal: Hamming grid of %u beyond its hit queues", grid);
         if ((e = launch_hd(a, b->tile.blocks, grid, s)) != hipSuccess) return hip_fail(e, "launch Hamming kernel");
+        if (b->d_fbits.p &&
+            (e = launch_fold_fbits((const uint32_t *)b->d_fbits.p, b->ss.num, b->f_words, (uint32_t)b->ss.total,
+                                   (uint32_t *)b->d_u.p, s)) != hipSuccess)
+            return hip_fail(e, "fold text bits");
     }
     return MOTEX_OK;
 }

@@ -13,9 +13,23 @@ def rand_seqs(rng, n, lo, hi, alpha):
     return [a[rng.integers(0, len(a), rng.integers(lo, hi + 1))].tobytes() for _ in range(n)]
 
 
-def check(ctx, oracle, seqs, k, e, alphabet="DNA"):
-    print(f"[hd parity] n={len(seqs)} total={sum(map(len, seqs))} k={k} e={e}", flush=True)
-    u, v, off = ctx.extract_all_pairs(seqs, MotexParams(k=k, e=e, distance=0, alphabet=alphabet))
+@pytest.fixture(params=["0", "1"], ids=["ordered", "symmetric"])
+def sym(request, monkeypatch):
+    """Runs a test with the pair-symmetric Hamming plan forced off, then on (where the
+    input allows it: every text fits one work item's range)."""
+    monkeypatch.setenv("MOTEX_SYM", request.param)
+    return request.param == "1"
+
+
+def check(ctx, oracle, seqs, k, e, alphabet="DNA", sym=None):
+    print(f"[hd parity] n={len(seqs)} total={sum(map(len, seqs))} k={k} e={e} sym={sym}", flush=True)
+    p = MotexParams(k=k, e=e, distance=0, alphabet=alphabet)
+    b = ctx.batch(seqs, p)
+    st = b.run()
+    u, v, off = b.download()
+    b.close()
+    if sym is not None and len(seqs) >= 2:
+        assert st["symmetric"] == int(sym)
     ou, ov, ooff, _ = oracle.all_pairs(seqs, 0, k, e)
     assert np.array_equal(off[:-1], ooff)
     bad = np.nonzero((u != ou) | (v != ov))[0]
@@ -23,15 +37,26 @@ def check(ctx, oracle, seqs, k, e, alphabet="DNA"):
     return u, v
 
 
-def test_config1_full(ctx, oracle):
+def test_config1_full(ctx, oracle, sym):
     w = synth.config1()
-    u, v = check(ctx, oracle, w.seqs, w.k, w.e)
+    u, v = check(ctx, oracle, w.seqs, w.k, w.e, sym=sym)
     assert v.sum() > 0
 
 
-def test_config2_subset(ctx, oracle):
+def test_config2_subset(ctx, oracle, sym):
     w = synth.config2(scale=0.05)  # 60 sequences of 1000 nt
-    check(ctx, oracle, w.seqs, w.k, w.e)
+    check(ctx, oracle, w.seqs, w.k, w.e, sym=sym)
+
+
+def test_symmetric_is_the_default_for_rare_occurrences(ctx):
+    w = synth.config2(scale=0.05)
+    b = ctx.batch(w.seqs, MotexParams(k=w.k, e=w.e))
+    assert b.stats()["symmetric"] == 1
+    b.close()
+    # k=5, e=1 over ACGT: occurrences too dense for the text-side atomics to pay off
+    b = ctx.batch(w.seqs, MotexParams(k=5, e=1))
+    assert b.stats()["symmetric"] == 0
+    b.close()
 
 
 @pytest.mark.parametrize("seed,k,e,alpha,n,lo,hi", [
@@ -45,10 +70,10 @@ def test_config2_subset(ctx, oracle):
     (8, 10, 2, b"ARNDCQEGHILKMFPSTWYV", 10, 100, 900),  # protein alphabet
     (9, 6, 1, b"0123456789", 10, 50, 300),  # USR alphabet
 ])
-def test_random_edges(ctx, oracle, seed, k, e, alpha, n, lo, hi):
+def test_random_edges(ctx, oracle, sym, seed, k, e, alpha, n, lo, hi):
     rng = np.random.default_rng(seed)
     alphabet = "PROT" if len(alpha) == 20 else ("USR" if alpha[0:1] == b"0" else "DNA")
-    check(ctx, oracle, rand_seqs(rng, n, lo, hi, alpha), k, e, alphabet)
+    check(ctx, oracle, rand_seqs(rng, n, lo, hi, alpha), k, e, alphabet, sym=sym)
 
 
 def test_long_sequence_split(ctx, oracle):
