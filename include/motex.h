@@ -84,6 +84,8 @@ typedef struct motex_stats {
     uint32_t graph_replay;  /* 1 if the last run replayed the batch's HIP graph */
     uint32_t symmetric;     /* 1 if each unordered pair is computed once and credited to
                                both orders (Hamming; cells still count every ordered pair) */
+    double swept_cells;     /* cells the kernel actually sweeps (Hamming: wave rows x
+                               diagonals of their windows, warm-up and padding included) */
 } motex_stats;
 
 /* A serving job.  The caller keeps seqs/lens/params and the output arrays alive until

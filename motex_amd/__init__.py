@@ -40,7 +40,8 @@ class Stats(C.Structure):
                 ("work_items", C.c_uint64), ("pairs", C.c_uint64), ("skipped_pairs", C.c_uint64),
                 ("tile_blocks", C.c_uint32), ("tile_rows", C.c_uint32),
                 ("waves_per_cu", C.c_uint32), ("kernel_variant", C.c_uint32),
-                ("graph_replay", C.c_uint32), ("symmetric", C.c_uint32)]
+                ("graph_replay", C.c_uint32), ("symmetric", C.c_uint32),
+                ("swept_cells", C.c_double)]
 
 
 class Tile(C.Structure):

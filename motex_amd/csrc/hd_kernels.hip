@@ -46,24 +46,47 @@ __global__ void ne_build_kernel(const uint8_t *__restrict__ codes, uint32_t tota
 }
 
 // u of the text side of pair-symmetric runs: for every text position p, the number of x
-// sequences whose bit (x, p) is set.  One thread per 32-position word; the 32 counts are
-// kept as bit-sliced planes (plane q = bit q of each count), so adding one x's word is a
-// ripple-carry increment of the planes.
-__global__ void fold_fbits_kernel(const uint32_t *__restrict__ fbits, uint32_t nx, uint32_t f_words,
-                                  uint32_t total, uint32_t *__restrict__ u)
+// sequences whose bit (x, p) is set.  A workgroup takes 64 words of positions; its four
+// waves each count a quarter of the x sequences, keeping the 32 counts of a word as
+// bit-sliced planes (plane q = bit q of each count; adding one x's word is a ripple-carry
+// increment that stops when the carry dies out), and the quarters are summed with a
+// bit-sliced adder through LDS.
+__global__ __launch_bounds__(256) void fold_fbits_kernel(const uint32_t *__restrict__ fbits, uint32_t nx,
+                                                         uint32_t f_words, uint32_t total,
+                                                         uint32_t *__restrict__ u)
 {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= f_words) return;
+    __shared__ uint32_t part[3][16][64];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t w = blockIdx.x * 64 + lane;
     uint32_t plane[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) plane[q] = 0;
-    for (uint32_t x = 0; x < nx; ++x) {
-        uint32_t carry = fbits[(size_t)x * f_words + w];
+    if (w < f_words) {
+        const uint32_t x0 = (uint32_t)((uint64_t)nx * wv / 4), x1 = (uint32_t)((uint64_t)nx * (wv + 1) / 4);
+        for (uint32_t x = x0; x < x1; ++x) {
+            uint32_t carry = fbits[(size_t)x * f_words + w];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                if (!carry) break;
+                const uint32_t t = plane[q] & carry;
+                plane[q] ^= carry;
+                carry = t;
+            }
+        }
+    }
+    if (wv) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) part[wv - 1][q][lane] = plane[q];
+    }
+    __syncthreads();
+    if (wv || w >= f_words) return;
+    for (int o = 0; o < 3; ++o) {
+        uint32_t c = 0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            const uint32_t t = plane[q] & carry;
-            plane[q] ^= carry;
-            carry = t;
+            const uint32_t a = plane[q], b = part[o][q][lane];
+            plane[q] = a ^ b ^ c;
+            c = (a & b) | (c & (a ^ b));
         }
     }
     uint32_t any = 0;
@@ -314,8 +337,19 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
                 }
             };
             const uint32_t warm = nrows < a.k ? nrows : a.k;
-            for (uint32_t r = 0; r < warm; ++r) row(r, std::false_type{});
-            for (uint32_t r = warm; r < nrows; ++r) row(r, std::true_type{});
+            // two rows per trip: the counter planes then stay in place across the back edge
+            // (one row per trip leaves register copies of the planes in the loop)
+            uint32_t r = 0;
+            for (; r + 1 < warm; r += 2) {
+                row(r, std::false_type{});
+                row(r + 1, std::false_type{});
+            }
+            if (r < warm) row(r++, std::false_type{});
+            for (; r + 1 < nrows; r += 2) {
+                row(r, std::true_type{});
+                row(r + 1, std::true_type{});
+            }
+            if (r < nrows) row(r, std::true_type{});
             if (nq) drain_hits<B>(a, R, q0, I.px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask);
             __syncthreads();
             // distinct sequences of this range per row
@@ -405,7 +439,7 @@ hipError_t launch_fold_fbits(const uint32_t *fbits, uint32_t nx, uint32_t f_word
                              uint32_t *u, hipStream_t s)
 {
     if (nx > 65535) return hipErrorInvalidValue;  // counts are 16-bit planes
-    hipLaunchKernelGGL(fold_fbits_kernel, dim3((f_words + 255) / 256), dim3(256), 0, s, fbits, nx, f_words,
+    hipLaunchKernelGGL(fold_fbits_kernel, dim3((f_words + 63) / 64), dim3(256), 0, s, fbits, nx, f_words,
                        total, u);
     return hipGetLastError();
 }

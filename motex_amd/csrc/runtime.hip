@@ -420,6 +420,9 @@ int hd_prepare(motex_batch *b)
     const uint64_t grid = std::min<uint64_t>((uint64_t)c->num_cu * ts.waves_per_cu, b->hd.items.size());
     if ((e = hd_alloc_hitq(b->d_hitq, grid, ts.blocks)) != hipSuccess) return hip_fail(e, "alloc hit queues");
     b->stats.symmetric = sym ? 1u : 0u;
+    b->stats.swept_cells = 0;
+    for (const HdItem &it : b->hd.items)
+        b->stats.swept_cells += (double)(it.r1 - it.px_start) * (it.re - it.rb) * kDiagPerBlock * ts.blocks;
     if (sym) {
         b->f_words = (uint32_t)((ss.total + 31) / 32 + 1);
         if ((e = b->d_fbits.alloc((uint64_t)ss.num * b->f_words * 4)) != hipSuccess) return hip_fail(e, "alloc text bits");
@@ -750,6 +753,7 @@ int motex::batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *l
     else rc = ed_prepare(b.get());
     if (rc) return rc;
     if ((e = hipStreamSynchronize(bstream(b.get()))) != hipSuccess) return hip_fail(e, "batch upload");
+    if (b->stats.swept_cells == 0) b->stats.swept_cells = b->stats.cells;  // edit, structured
     b->stats.tile_blocks = b->tile.blocks;
     b->stats.tile_rows = b->tile.rows;
     b->stats.waves_per_cu = b->tile.waves_per_cu;
