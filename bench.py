@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 VALU_INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # 39.3 T lane-ops/s
 OPS_PER_CELL = {0: 8.0, 1: 22.0}                  # algorithmic int32 ops per cell (HD, ED)
+# Issue-rate ceilings measured on MI355X with a dependency-free micro-benchmark of one
+# instruction kind at 32 waves per CU (T lane-ops/s); see profiles/valu_issue_rate.md.
+VALU_CEILINGS_TOPS = {"v_xor_b32": 61.9, "v_bitop3_b32": 59.3, "v_alignbit_b32": 37.9}
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def parse():
@@ -58,6 +62,28 @@ def copy_probe(nbytes: int):
     y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": nbytes}))
+
+
+def traffic_from_profile(kernel: str, workload_tag: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter reads for a
+    plain device copy of known size).  None when no summary covers this workload."""
+    path = os.path.join(PROFILES, f"{kernel}_{workload_tag}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            key, val = line.split(":", 1)
+            kv[key.strip()] = val.strip()
+    try:
+        cb = float(kv["copy_bytes"])
+        fetch = float(kv["fetch_kib"]) * 1024 * cb / (float(kv["copy_fetch_kib"]) * 1024)
+        write = float(kv["write_kib"]) * 1024 * cb / (float(kv["copy_write_kib"]) * 1024)
+    except (KeyError, ValueError):
+        return None
+    return {"bytes": round(fetch + write), "fetch_bytes": round(fetch), "write_bytes": round(write),
+            "source": os.path.relpath(path, ROOT)}
 
 
 def cpu_baseline(w, gpu_u, gpu_v, off, nx):
@@ -152,14 +178,26 @@ def main():
     kms = float(np.mean(kernel_ms))
     ops = OPS_PER_CELL[w.distance]
     achieved = cells * ops / (kms / 1e3) / 1e12
+    kernel = ("struct_kernel" if w.boxes else
+              "hd_batched_kernel" if w.distance == 0 else "ed_kernel")
+    tr = traffic_from_profile(kernel, f"config{a.config}") if a.scale == 1.0 else None
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_INT32_PEAK_TOPS,
                 "unit": "TOP/s", "frac": round(achieved / VALU_INT32_PEAK_TOPS, 4),
-                "traffic": None,
-                "kernel": "hd_batched_kernel" if w.distance == 0 else "ed_kernel",
+                "traffic": tr["bytes"] if tr else None,
+                "traffic_detail": tr,
+                "kernel": kernel,
                 "kernel_ms": round(kms, 4),
                 "ops_per_cell": ops,
-                "note": "algorithmic int32 ops (per-cell MaxShift count) / kernel time; the "
-                        "bit-sliced kernel issues fewer ops per cell, so frac can exceed 1"}
+                "cells_per_s": round(cells / (kms / 1e3), 1),
+                "swept_cells_per_s": round(st["swept_cells"] / (kms / 1e3), 1),
+                "symmetric": bool(st["symmetric"]),
+                "graph_replay": bool(st["graph_replay"]),
+                "valu_ceilings_measured": VALU_CEILINGS_TOPS,
+                "note": "achieved = algorithmic int32 ops (the reference's per-cell MaxShift "
+                        "count x every ordered pair's cells) / kernel time (HIP events).  The "
+                        "kernel is bit-sliced (32 diagonals per lane op) and, for Hamming, "
+                        "computes each unordered pair once, so frac exceeds 1; the issued VALU "
+                        "rate against the measured ceilings is in profiles/"}
 
     out = {
         "metric": "GCUPS batched vs plain-per-pair on 1 GPU at k=12,e=2; p50/p99 job latency",

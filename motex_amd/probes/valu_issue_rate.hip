@@ -1,0 +1,67 @@
+// VALU issue-rate probe (calibrates the Hamming kernel's roofline): 12 independent chains
+// of one instruction kind (v_xor_b32, v_bitop3_b32, v_alignbit_b32 -- the instructions of
+// the bit-sliced sweep), at 8, 16 and 32 waves per CU.  No memory traffic apart from one
+// store per thread at the end.  Prints wave-instructions per CU-cycle at 2.4 GHz and
+// lane-ops/s.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdint>
+#include <vector>
+
+#define CHAINS 12
+template <int KIND>
+__global__ __launch_bounds__(256) void kern(uint32_t *out, uint32_t n, uint32_t seed)
+{
+    uint32_t a[CHAINS];
+    for (int c = 0; c < CHAINS; ++c) a[c] = threadIdx.x * 2654435761u + c * 40503u + seed;
+    uint32_t b = seed ^ 0x5bd1e995u, d = seed * 7u + 3u;
+    for (uint32_t i = 0; i < n; ++i) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+#pragma unroll
+            for (int c = 0; c < CHAINS; ++c) {
+                if (KIND == 0) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(a[c]) : "v"(b));
+                else if (KIND == 1) asm volatile("v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96" : "+v"(a[c]) : "v"(b), "v"(d));
+                else asm volatile("v_alignbit_b32 %0, %0, %1, %2" : "+v"(a[c]) : "v"(b), "v"(d));
+            }
+        }
+    }
+    uint32_t x = 0;
+    for (int c = 0; c < CHAINS; ++c) x ^= a[c];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
+int main()
+{
+    int dev = 0, ncu = 0, clk = 0;
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    hipDeviceGetAttribute(&clk, hipDeviceAttributeClockRate, dev);
+    printf("cus %d clock_khz %d\n", ncu, clk);
+    const uint32_t n = 4096;
+    for (int wpc : {8, 16, 32}) {
+        const uint32_t grid = ncu * wpc / 4;  // 4 waves per block
+        uint32_t *out;
+        if (hipMalloc(&out, (size_t)grid * 256 * 4) != hipSuccess) return 1;
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        for (int kind = 0; kind < 3; ++kind) {
+            for (int rep = 0; rep < 2; ++rep) {
+                hipEventRecord(e0, 0);
+                if (kind == 0) hipLaunchKernelGGL(kern<0>, dim3(grid), dim3(256), 0, 0, out, n, 1u);
+                else if (kind == 1) hipLaunchKernelGGL(kern<1>, dim3(grid), dim3(256), 0, 0, out, n, 1u);
+                else hipLaunchKernelGGL(kern<2>, dim3(grid), dim3(256), 0, 0, out, n, 1u);
+                hipEventRecord(e1, 0);
+                if (hipEventSynchronize(e1) != hipSuccess) return 2;
+                float ms = 0;
+                hipEventElapsedTime(&ms, e0, e1);
+                double winstr = (double)grid * 4 * n * 8 * CHAINS;
+                if (rep) printf("waves/cu %d kind %d ms %.3f wave-instr/s %.3e per CU-cycle@2.4GHz %.3f lane-ops/s %.3e\n",
+                                wpc, kind, ms, winstr / (ms * 1e-3), winstr / (ms * 1e-3) / (ncu * 2.4e9),
+                                winstr * 64 / (ms * 1e-3));
+            }
+        }
+        hipFree(out);
+    }
+    return 0;
+}
