@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace motex {
 
 template <bool WIDE> struct EdWord;
@@ -140,8 +142,8 @@ __device__ __forceinline__ void ed_step(EdState<C, WIDE> &st, uint32_t s, uint32
 template <int C, bool WIDE, bool BITMAP>
 __device__ void ed_pair(const uint8_t *__restrict__ xcodes, uint32_t m,
                         const uint8_t *__restrict__ text, uint32_t n, uint32_t k, uint32_t e,
-                        uint32_t *buf0, uint32_t *buf1, uint32_t *acc_u, uint32_t *acc_v,
-                        uint32_t *gu, uint32_t *gv, const EdOut &out)
+                        uint32_t *buf0, uint32_t *buf1, uint32_t *gu, uint32_t *gv,
+                        const EdOut &out)
 {
     typedef typename EdWord<WIDE>::T T;
     const uint32_t lane = threadIdx.x;
@@ -179,7 +181,8 @@ __device__ void ed_pair(const uint8_t *__restrict__ xcodes, uint32_t m,
         const uint32_t steps = n + 63;
         uint32_t vinT = 0;
         T vinD = 0;
-        for (uint32_t s = 1; s <= steps; ++s) {
+        // one wavefront step s; CHECK: fill/drain form (rows outside 1..n keep their state)
+        auto step = [&](uint32_t s, auto check) {
             const uint32_t r0 = s - 1;  // 0-based row lane 0 starts this step
             if ((r0 & 63) == 0) {
                 uint32_t rr = r0 + lane;
@@ -202,10 +205,7 @@ __device__ void ed_pair(const uint8_t *__restrict__ xcodes, uint32_t m,
             } else {
                 inD = __builtin_amdgcn_readlane(vinD, sl);
             }
-            if (s >= 64 && s <= n)
-                ed_step<C, WIDE, false, BITMAP>(st, s, n, e, inT, inD, out, col0);
-            else
-                ed_step<C, WIDE, true, BITMAP>(st, s, n, e, inT, inD, out, col0);
+            ed_step<C, WIDE, decltype(check)::value, BITMAP>(st, s, n, e, inT, inD, out, col0);
             if (!last && s >= 64) {
                 // collect lane 63's last-column word of row s-63
                 if constexpr (WIDE) {
@@ -232,25 +232,34 @@ __device__ void ed_pair(const uint8_t *__restrict__ xcodes, uint32_t m,
                     coll_n = 0;
                 }
             }
+        };
+        // fill (s < 64), steady state (64 <= s <= n, two steps per trip: with one, the
+        // column words are copied between registers on every back edge), drain (s > n)
+        uint32_t s = 1;
+        const uint32_t fill_end = n < 63 ? n : 63;
+        for (; s <= fill_end; ++s) step(s, std::true_type{});
+        for (; s + 1 <= n; s += 2) {
+            step(s, std::false_type{});
+            step(s + 1, std::false_type{});
         }
+        if (s <= n) {  // here s >= 64: a steady-state step
+            step(s, std::false_type{});
+            ++s;
+        }
+        for (; s <= steps; ++s) step(s, std::true_type{});
         if constexpr (!BITMAP) {
             // fold this strip's counts for the current text
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 uint32_t j = col0 + c + 1;
                 uint32_t anyb = st.any[c] ? 1u : 0u;
-                if (j >= k && j <= m && (anyb || st.cnt[c])) {
-                    if (acc_u) {
-                        acc_u[j - 1] += anyb;
-                        acc_v[j - 1] += st.cnt[c];
-                    } else {
-                        if (anyb) atomicAdd(gu + j - 1, anyb);
-                        atomicAdd(gv + j - 1, st.cnt[c]);
-                    }
+                if (j >= k && j <= m && anyb) {
+                    atomicAdd(gu + j - 1, 1u);
+                    atomicAdd(gv + j - 1, st.cnt[c]);
                 }
             }
         }
-        __threadfence();  // boundary stream visible to the next strip's loads
+        __threadfence_block();  // boundary stream visible to the next strip's loads (same wave)
     }
 }
 

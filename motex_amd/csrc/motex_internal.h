@@ -81,7 +81,6 @@ struct EdArgs {
     uint32_t stream_words;
     uint32_t nitems;
     uint32_t k, e;
-    uint32_t lds_cols;        // columns held in the LDS accumulators
     uint32_t persistent;      // 1: pull items from *queue; 0: item = blockIdx.x
 };
 
@@ -104,7 +103,8 @@ struct StructArgs {
     const uint32_t *S;        // nb_structs x nb_gaps gap lengths (last gap fastest)
     uint32_t *u, *v;
     uint32_t *queue;
-    uint32_t *scratch;        // per workgroup: box bitmaps, then two edit boundary streams
+    uint32_t *scratch;        // per workgroup: box bitmaps, two edit boundary streams, the
+                              // counted bits, then (Hamming) one occurrence list per shape
     uint64_t scratch_words;   // per workgroup
     uint64_t bm_words;        // one box bitmap: ceil(n_max/32) * m_max words
     uint32_t stream_words;
@@ -118,6 +118,7 @@ struct StructArgs {
     uint32_t dist_of[kMaxBoxes];                // box b -> distinct shape
     uint32_t dlen[kMaxBoxes], derr[kMaxBoxes];
     uint32_t m_max, hcap;
+    uint32_t lcap;                              // Hamming boxes: occurrence list entries per shape
 };
 
 }  // namespace motex

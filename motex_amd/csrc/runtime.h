@@ -173,7 +173,7 @@ TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss);
 hipError_t launch_ne_build(const uint8_t *codes, uint32_t total, uint32_t sigma, uint32_t W,
                            uint32_t *ne, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t val, hipStream_t s);
-hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, hipStream_t s);
 uint32_t ed_strip_cols(bool wide);
 hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_t s);
 hipError_t launch_fold_fbits(const uint32_t *fbits, uint32_t nx, uint32_t f_words, uint32_t total,

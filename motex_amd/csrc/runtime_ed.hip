@@ -8,10 +8,9 @@
 
 namespace motex {
 
-hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, hipStream_t s);
 uint32_t ed_strip_cols(bool wide);
 
-static const uint32_t kEdLdsCols = 8000;      // x longer than this accumulates in global memory
 static const uint64_t kEdTextPerItem = 8192;  // text symbols per work item
 
 static bool ed_wide(const motex_params &p)
@@ -108,9 +107,6 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
     uint32_t stream_words = ed_stream_words(b);
     uint64_t need = (uint64_t)grid * 2 * stream_words * 4;
     if (b->d_aux.bytes < need && (e = batch_realloc_scratch(b, need)) != hipSuccess) return hip_fail(e, "alloc edit streams");
-    uint32_t maxm = 0;
-    for (uint32_t i = 0; i < ss.num; ++i) maxm = std::max(maxm, ss.len[i]);
-    uint32_t lds_cols = std::min(maxm, kEdLdsCols);
     EdArgs a{};
     a.codes = (const uint8_t *)b->d_codes.p;
     a.seq_start = (const uint32_t *)b->d_seq_start.p;
@@ -123,10 +119,8 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
     a.nitems = nitems;
     a.k = b->params.k;
     a.e = b->params.e;
-    a.lds_cols = lds_cols;
     a.persistent = persistent;
-    size_t lds = (size_t)std::max<uint32_t>(lds_cols, 1) * 8;
-    if ((e = launch_ed(a, wide, grid, lds, s)) != hipSuccess) return hip_fail(e, "launch edit kernel");
+    if ((e = launch_ed(a, wide, grid, s)) != hipSuccess) return hip_fail(e, "launch edit kernel");
     return MOTEX_OK;
 }
 

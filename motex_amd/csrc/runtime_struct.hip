@@ -11,13 +11,14 @@ namespace motex {
 
 hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStream_t s);
 
-static const uint32_t kStructHitCap = 4096;
+static const uint32_t kStructHitCap = 2048;  // >= one chunk: 64 lanes x 32 bits
+static const uint32_t kStructListCap = 8192; // Hamming occurrences listed per pair and box shape
 
 static int struct_alloc_scratch(motex_batch *b);
 
 static size_t struct_lds_bytes(const StructArgs &a)
 {
-    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + 1) * 4;
+    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + 1 + kMaxBoxes) * 4;
 }
 
 // Fills the shape fields of StructArgs from the parameters.
@@ -138,8 +139,9 @@ static int struct_fill_args(motex_batch *b, StructArgs &a)
     a.hcap = kStructHitCap;
     a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
     a.stream_words = (maxlen + 128) * 2;
+    a.lcap = b->params.distance == 0 ? kStructListCap : 0;
     a.scratch_words = a.ndist * a.bm_words + 2ull * a.stream_words +
-                      ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1;
+                      ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1 + (uint64_t)a.ndist * a.lcap;
     return MOTEX_OK;
 }
 
@@ -169,7 +171,8 @@ static uint32_t struct_grid(motex_batch *b, uint32_t nitems)
     uint32_t maxlen = 0;
     for (uint32_t i = 0; i < ss.num; ++i) maxlen = std::max(maxlen, ss.len[i]);
     uint64_t per = ((uint64_t)(maxlen + 31) / 32 * maxlen * (b->params.nb_gaps + 1) + 2ull * (maxlen + 128) * 2 +
-                    ((uint64_t)b->nb_structs * maxlen + 31) / 32 + 1) * 4;
+                    ((uint64_t)b->nb_structs * maxlen + 31) / 32 + 1 +
+                    (uint64_t)(b->params.nb_gaps + 1) * kStructListCap) * 4;
     uint64_t cap = std::max<uint64_t>(1, (2ull << 30) / std::max<uint64_t>(per, 1));
     uint64_t g = std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, nitems);
     return (uint32_t)std::max<uint64_t>(1, std::min(g, cap));

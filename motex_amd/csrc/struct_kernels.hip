@@ -9,8 +9,11 @@
 //      wavefront of the edit kernel in bitmap mode (every word written once).  A box whose
 //      error bound is at least m passes everywhere, as the reference's B matrices, which
 //      hold m where FLASM returned no tuple (functions.cc:597-598), do.
-//   2. the merge of functions.cc:647-690: every box-0 hit (pos_t, pos_x) is gathered into
-//      an LDS list; for every x-side gap combination kx the y-side combinations that hit
+//      Hamming hits also go to a per-shape occurrence list in scratch, so that the merge
+//      needs no scan of the bitmaps and the pair clears only the words it set.
+//   2. the merge of functions.cc:647-690: every box-0 hit (pos_t, pos_x) is taken from its
+//      list (or, for edit boxes and overflowing lists, gathered from the bitmap into an LDS
+//      list); for every x-side gap combination kx the y-side combinations that hit
 //      every box are counted -- explicit loops over all but the last gap, a popcount of a
 //      bit window of the last box's column for the last one.  v[kx*m + pos_x] adds the
 //      count; u[kx*m + pos_x] adds 1 once per pair, deduplicated with a per-pair bitmap of
@@ -100,7 +103,8 @@ __device__ uint32_t struct_walk(const StructArgs &a, uint32_t *const bm[kMaxBoxe
 // Hamming hits of one box shape for the pair, by the diagonal sweep, into bm.
 template <int P>
 __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t len, uint32_t err,
-                              uint32_t *bm, uint32_t *ne_l, const uint16_t *xoff)
+                              uint32_t *bm, uint32_t *ne_l, const uint16_t *xoff, uint32_t *list,
+                              uint32_t *lcount)
 {
     const uint32_t lane = threadIdx.x;
     const uint32_t m = I.m, n = I.n, wl = a.wl;
@@ -149,6 +153,8 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
                     int64_t pt = qb + 32 * lane + t + px - (int64_t)I.y_start;
                     if (pt + 1 < (int64_t)len || pt >= (int64_t)n) continue;
                     atomicOr(bm + (size_t)((uint32_t)pt >> 5) * m + px, 1u << ((uint32_t)pt & 31));
+                    const uint32_t slot = atomicAdd(lcount, 1u);
+                    if (slot < a.lcap) list[slot] = (px << 16) | (uint32_t)pt;
                 }
             }
         }
@@ -166,12 +172,18 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
     uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + (a.sigma + 1) * wl);
     uint32_t *hits = smem + (a.sigma + 1) * wl + (a.m_max + 1) / 2;
     uint32_t *hcount = hits + a.hcap;
+    uint32_t *lcount = hcount + 1;  // per box shape: occurrences listed this pair
     for (uint32_t w = lane; w < wl; w += 64) ne_l[a.sigma * wl + w] = 0;
     // scratch: box bitmaps | two edit boundary streams | counted (kx, pos_x) bits
     uint32_t *scr = a.scratch + (size_t)blockIdx.x * a.scratch_words;
     uint32_t *buf0 = scr + (size_t)a.ndist * a.bm_words;
     uint32_t *buf1 = buf0 + a.stream_words;
     uint32_t *counted = buf1 + a.stream_words;
+    uint32_t *lists = counted + ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1;
+    // Hamming boxes: the bitmaps are all zero between pairs (each pair clears the words it
+    // set, through the occurrence lists, or all of them when a list overflowed); the first
+    // pair of a launch clears them in full.
+    bool clean = false;
 
     for (uint32_t round = 0;; ++round) {
         uint32_t it;
@@ -195,6 +207,12 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             for (uint32_t r = lane; r < m; r += 64)
                 xoff[r] = (uint16_t)(a.codes[I.x_start + r] * wl);
         }
+        if (a.distance == 0 && !clean) {
+            for (size_t w = lane; w < (size_t)a.ndist * a.bm_words; w += 64) scr[w] = 0;
+            clean = true;
+        }
+        if (lane < a.ndist) lcount[lane] = 0;
+        bool listed = a.distance == 0;  // every shape's occurrences are in its list
         // 1. box bitmaps
         for (uint32_t d = 0; d < a.ndist; ++d) {
             const uint32_t len = a.dlen[d], err = a.derr[d];
@@ -206,58 +224,99 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
                     uint32_t rem = n - 32 * word;
                     out[w] = rem >= 32 ? ~0u : ((1u << rem) - 1);
                 }
+                listed = false;
                 continue;
             }
             if (a.distance == 0) {
-                for (size_t w = lane; w < (size_t)wn * m; w += 64) out[w] = 0;
-                __threadfence();
                 __syncthreads();
-                struct_hd_box<P>(a, I, len, err, out, ne_l, xoff);
+                struct_hd_box<P>(a, I, len, err, out, ne_l, xoff, lists + (size_t)d * a.lcap, lcount + d);
             } else {
                 EdOut eo{out, m};
                 if (len <= 14)
                     ed_pair<4, false, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
-                                            buf0, buf1, nullptr, nullptr, nullptr, nullptr, eo);
+                                            buf0, buf1, nullptr, nullptr, eo);
                 else
                     ed_pair<2, true, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
-                                           buf0, buf1, nullptr, nullptr, nullptr, nullptr, eo);
+                                           buf0, buf1, nullptr, nullptr, eo);
             }
         }
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         // 2. merge: gather box-0 hits word row by word row, then count per x combination
         const uint32_t cwords = (a.nb_structs * m + 31) / 32;
         for (uint32_t w = lane; w < cwords; w += 64) counted[w] = 0;
-        __threadfence();
+        __threadfence_block();
         if (lane == 0) *hcount = 0;
         __syncthreads();
         const uint32_t *b0 = bm[0];
-        const uint32_t chunks = (m + 63) / 64;
-        for (uint32_t wc = 0; wc <= wn * chunks; ++wc) {
-            // flush before a chunk that could overflow the list, and at the end
-            uint32_t have = __builtin_amdgcn_readfirstlane(*hcount);
-            bool flush = (wc == wn * chunks) ? have > 0 : have + 64 * 32 > a.hcap;
-            if (flush) {
-                for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
-                    for (uint32_t h = lane; h < have; h += 64) {
-                        uint32_t e = hits[h];
-                        uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
-                        uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
-                        if (c) {
-                            uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
-                            uint32_t old = atomicOr(counted + (idx >> 5), bit);
-                            size_t o = (size_t)I.uv_base + idx;
-                            if (!(old & bit)) atomicAdd(a.u + o, 1u);
-                            atomicAdd(a.v + o, c);
-                        }
+        for (uint32_t d = 0; d < a.ndist && listed; ++d)
+            if (__builtin_amdgcn_readfirstlane(lcount[d]) > a.lcap) listed = false;
+        if (listed) {
+            // walk box 0's listed occurrences, then clear the bitmap words of every list
+            const uint32_t d0 = a.dist_of[0];
+            const uint32_t *l0 = lists + (size_t)d0 * a.lcap;
+            const uint32_t c0 = __builtin_amdgcn_readfirstlane(lcount[d0]);
+            for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
+                for (uint32_t h = lane; h < c0; h += 64) {
+                    const uint32_t e = l0[h];
+                    const uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
+                    const uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
+                    if (c) {
+                        const uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
+                        const uint32_t old = atomicOr(counted + (idx >> 5), bit);
+                        const size_t o = (size_t)I.uv_base + idx;
+                        if (!(old & bit)) atomicAdd(a.u + o, 1u);
+                        atomicAdd(a.v + o, c);
                     }
                 }
-                __syncthreads();
-                if (lane == 0) *hcount = 0;
-                __syncthreads();
             }
-            if (wc == wn * chunks) break;
-            const uint32_t w = wc / chunks, px = (wc % chunks) * 64 + lane;
+            __threadfence_block();
+            __syncthreads();
+            for (uint32_t d = 0; d < a.ndist; ++d) {
+                const uint32_t *ld = lists + (size_t)d * a.lcap;
+                const uint32_t cd = __builtin_amdgcn_readfirstlane(lcount[d]);
+                for (uint32_t h = lane; h < cd; h += 64) {
+                    const uint32_t e = ld[h];
+                    bmd[d][(size_t)((e & 0xFFFFu) >> 5) * m + (e >> 16)] = 0;
+                }
+            }
+            __threadfence_block();
+            continue;
+        }
+        // the bitmaps are scanned; Hamming ones are cleared in full after the merge
+        clean = a.distance != 0;
+        // box-0 occurrences of the pair: count them first (every lane streams its share of
+        // the bitmap with no barrier in between); when they fit the LDS list, gather them in
+        // one more pass and walk them once.  Otherwise fall back to gathering chunk by
+        // chunk, walking whenever the next chunk could overflow the list.
+        uint32_t mine = 0;
+        for (uint32_t w = 0; w < wn; ++w)
+            for (uint32_t px = lane; px < m; px += 64) mine += __builtin_popcount(b0[(size_t)w * m + px]);
+        if (lane == 0) *hcount = 0;
+        __syncthreads();
+        if (mine) atomicAdd(hcount, mine);
+        __syncthreads();
+        const uint32_t total = __builtin_amdgcn_readfirstlane(*hcount);
+        __syncthreads();
+        if (lane == 0) *hcount = 0;
+        __syncthreads();
+        auto walk_list = [&](uint32_t have) {
+            for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
+                for (uint32_t h = lane; h < have; h += 64) {
+                    uint32_t e = hits[h];
+                    uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
+                    uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
+                    if (c) {
+                        uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
+                        uint32_t old = atomicOr(counted + (idx >> 5), bit);
+                        size_t o = (size_t)I.uv_base + idx;
+                        if (!(old & bit)) atomicAdd(a.u + o, 1u);
+                        atomicAdd(a.v + o, c);
+                    }
+                }
+            }
+        };
+        auto append = [&](uint32_t w, uint32_t px) {
             uint32_t word = px < m ? b0[(size_t)w * m + px] : 0u;
             if (word) {
                 uint32_t off = atomicAdd(hcount, (uint32_t)__builtin_popcount(word));
@@ -267,6 +326,29 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
                     hits[off++] = (px << 16) | (32 * w + t);
                 }
             }
+        };
+        if (total <= a.hcap) {
+            if (total) {
+                for (uint32_t w = 0; w < wn; ++w)
+                    for (uint32_t px0 = 0; px0 < m; px0 += 64) append(w, px0 + lane);
+                __syncthreads();
+                walk_list(total);
+            }
+            continue;
+        }
+        const uint32_t chunks = (m + 63) / 64;
+        for (uint32_t wc = 0; wc <= wn * chunks; ++wc) {
+            // walk before a chunk that could overflow the list, and at the end
+            uint32_t have = __builtin_amdgcn_readfirstlane(*hcount);
+            bool flush = (wc == wn * chunks) ? have > 0 : have + 64 * 32 > a.hcap;
+            if (flush) {
+                walk_list(have);
+                __syncthreads();
+                if (lane == 0) *hcount = 0;
+                __syncthreads();
+            }
+            if (wc == wn * chunks) break;
+            append(wc / chunks, (wc % chunks) * 64 + lane);
             __syncthreads();
         }
     }
