@@ -1,6 +1,7 @@
-// VALU issue-rate probe (calibrates the Hamming kernel's roofline): 12 independent chains
-// of one instruction kind (v_xor_b32, v_bitop3_b32, v_alignbit_b32 -- the instructions of
-// the bit-sliced sweep), at 8, 16 and 32 waves per CU.  No memory traffic apart from one
+// VALU issue-rate probe (calibrates the kernels' rooflines): 12 independent chains of one
+// instruction kind -- v_xor_b32, v_bitop3_b32, v_alignbit_b32 (the bit-sliced Hamming
+// sweep), v_bcnt_u32_b32, v_min3_u32, v_lshl_or_b32, v_add_u32 (the edit recurrence's
+// min* keys) -- at 8, 16 and 32 waves per CU.  No memory traffic apart from one
 // store per thread at the end.  Prints wave-instructions per CU-cycle at 2.4 GHz and
 // lane-ops/s.
 #include <hip/hip_runtime.h>
@@ -22,7 +23,11 @@ __global__ __launch_bounds__(256) void kern(uint32_t *out, uint32_t n, uint32_t 
             for (int c = 0; c < CHAINS; ++c) {
                 if (KIND == 0) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(a[c]) : "v"(b));
                 else if (KIND == 1) asm volatile("v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96" : "+v"(a[c]) : "v"(b), "v"(d));
-                else asm volatile("v_alignbit_b32 %0, %0, %1, %2" : "+v"(a[c]) : "v"(b), "v"(d));
+                else if (KIND == 2) asm volatile("v_alignbit_b32 %0, %0, %1, %2" : "+v"(a[c]) : "v"(b), "v"(d));
+                else if (KIND == 3) asm volatile("v_bcnt_u32_b32 %0, %0, %1" : "+v"(a[c]) : "v"(b));
+                else if (KIND == 4) asm volatile("v_min3_u32 %0, %0, %1, %2" : "+v"(a[c]) : "v"(b), "v"(d));
+                else if (KIND == 5) asm volatile("v_lshl_or_b32 %0, %0, 3, %1" : "+v"(a[c]) : "v"(b));
+                else asm volatile("v_add_u32 %0, %0, %1" : "+v"(a[c]) : "v"(b));
             }
         }
     }
@@ -45,19 +50,27 @@ int main()
         hipEvent_t e0, e1;
         hipEventCreate(&e0);
         hipEventCreate(&e1);
-        for (int kind = 0; kind < 3; ++kind) {
+        const char *names[] = {"v_xor_b32", "v_bitop3_b32", "v_alignbit_b32", "v_bcnt_u32_b32",
+                               "v_min3_u32", "v_lshl_or_b32", "v_add_u32"};
+        for (int kind = 0; kind < 7; ++kind) {
             for (int rep = 0; rep < 2; ++rep) {
                 hipEventRecord(e0, 0);
-                if (kind == 0) hipLaunchKernelGGL(kern<0>, dim3(grid), dim3(256), 0, 0, out, n, 1u);
-                else if (kind == 1) hipLaunchKernelGGL(kern<1>, dim3(grid), dim3(256), 0, 0, out, n, 1u);
-                else hipLaunchKernelGGL(kern<2>, dim3(grid), dim3(256), 0, 0, out, n, 1u);
+                switch (kind) {
+                case 0: hipLaunchKernelGGL(kern<0>, dim3(grid), dim3(256), 0, 0, out, n, 1u); break;
+                case 1: hipLaunchKernelGGL(kern<1>, dim3(grid), dim3(256), 0, 0, out, n, 1u); break;
+                case 2: hipLaunchKernelGGL(kern<2>, dim3(grid), dim3(256), 0, 0, out, n, 1u); break;
+                case 3: hipLaunchKernelGGL(kern<3>, dim3(grid), dim3(256), 0, 0, out, n, 1u); break;
+                case 4: hipLaunchKernelGGL(kern<4>, dim3(grid), dim3(256), 0, 0, out, n, 1u); break;
+                case 5: hipLaunchKernelGGL(kern<5>, dim3(grid), dim3(256), 0, 0, out, n, 1u); break;
+                default: hipLaunchKernelGGL(kern<6>, dim3(grid), dim3(256), 0, 0, out, n, 1u); break;
+                }
                 hipEventRecord(e1, 0);
                 if (hipEventSynchronize(e1) != hipSuccess) return 2;
                 float ms = 0;
                 hipEventElapsedTime(&ms, e0, e1);
                 double winstr = (double)grid * 4 * n * 8 * CHAINS;
-                if (rep) printf("waves/cu %d kind %d ms %.3f wave-instr/s %.3e per CU-cycle@2.4GHz %.3f lane-ops/s %.3e\n",
-                                wpc, kind, ms, winstr / (ms * 1e-3), winstr / (ms * 1e-3) / (ncu * 2.4e9),
+                if (rep) printf("waves/cu %d %-15s ms %.3f wave-instr/s %.3e per CU-cycle@2.4GHz %.3f lane-ops/s %.3e\n",
+                                wpc, names[kind], ms, winstr / (ms * 1e-3), winstr / (ms * 1e-3) / (ncu * 2.4e9),
                                 winstr * 64 / (ms * 1e-3));
             }
         }
