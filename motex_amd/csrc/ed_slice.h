@@ -1,0 +1,192 @@
+// Text-parallel ("sliced") MaxShift edit recurrence: one cell of 32 texts at once.
+//
+// The scalar recurrence (functions.cc:437-447, bitminmax functions.cc:923-967) keeps one
+// word D[i][j] per cell (row i of the text, column j of x):
+//     j <= K: D = min*(shift(up)|1, shift(left)|1, shift(diag)|delta)
+//     j >  K: D = min*(shift(up)|1, shiftc(left)|1, shiftc(diag)|delta)
+// with shift(a) = a << 1, shiftc(a) = (a & (2^(K-1)-1)) << 1 and min* = fewest set bits,
+// largest word on ties.  Here the same cell of 32 different texts (same x, same i and j) is
+// computed at once: plane p of a Slice holds bit p of the 32 words, one text per bit, so a
+// shift is a renaming of planes and every operation is a bitwise op over 32 texts.  Each
+// Slice also carries its popcount pc and the popcount pm of its low K-1 bits (the bits
+// shiftc keeps), as 5 bit-sliced planes each; the candidates' popcounts then follow from
+// their sources by a bit-sliced increment, and only pm of the selected word is counted.
+//
+// Selected words stay below 2K+1 bits (width <= K + popcount, popcount <= K), so W = 2K
+// planes hold every word; the candidates carry one more plane (shift(up) of a full-width
+// up) for the comparison, which is never selected.  The functions are plain C++ on 32-bit
+// masks, shared by the GPU kernel and the host check of this file against the scalar
+// recurrence (tests/slice_check.cpp).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MOTEX_HD __host__ __device__ __forceinline__
+#else
+#define MOTEX_HD inline
+#endif
+
+namespace motex {
+
+constexpr int kSlicePc = 5;  // popcount planes (counts up to 31)
+
+template <int K> struct SliceW { static constexpr int W = 2 * K; };
+
+template <int K>
+struct Slice {
+    uint32_t d[SliceW<K>::W];
+    uint32_t pc[kSlicePc];  // popcount of the word
+    uint32_t pm[kSlicePc];  // popcount of bits 0 .. K-2
+};
+
+// out = in + carry (carry: one 0/1 per text)
+MOTEX_HD void slice_inc(const uint32_t in[kSlicePc], uint32_t carry, uint32_t out[kSlicePc])
+{
+    for (int q = 0; q < kSlicePc; ++q) {
+        out[q] = in[q] ^ carry;
+        carry = in[q] & carry;
+    }
+}
+
+// Bit-sliced popcount of N planes into kSlicePc planes: full adders (3 planes -> sum and
+// carry, two 3-input logic ops each) at weight 1, then at weight 2, the weight-4 carries
+// rippled into the top planes.
+MOTEX_HD void slice_csa(uint32_t &s, uint32_t a, uint32_t b, uint32_t &carry)
+{
+    carry = (s & a) | (s & b) | (a & b);
+    s ^= a ^ b;
+}
+
+MOTEX_HD void slice_ripple(uint32_t *h, int n, uint32_t c)
+{
+    for (int q = 0; q < n; ++q) {
+        const uint32_t t = h[q] & c;
+        h[q] ^= c;
+        c = t;
+    }
+}
+
+template <int N>
+MOTEX_HD void slice_popcount(const uint32_t *x, uint32_t out[kSlicePc])
+{
+    uint32_t ones = 0, twos = 0, hi[kSlicePc - 2] = {0, 0, 0};
+    uint32_t c2 = 0;          // a pending weight-2 carry
+    bool have_c2 = false;
+    for (int i = 0; i < N; i += 2) {
+        uint32_t c;
+        slice_csa(ones, x[i], i + 1 < N ? x[i + 1] : 0u, c);
+        if (have_c2) {
+            uint32_t c4;
+            slice_csa(twos, c2, c, c4);
+            slice_ripple(hi, kSlicePc - 2, c4);
+            have_c2 = false;
+        } else {
+            c2 = c;
+            have_c2 = true;
+        }
+    }
+    if (have_c2) {
+        const uint32_t c4 = twos & c2;
+        twos ^= c2;
+        slice_ripple(hi, kSlicePc - 2, c4);
+    }
+    out[0] = ones;
+    out[1] = twos;
+    for (int q = 2; q < kSlicePc; ++q) out[q] = hi[q - 2];
+}
+
+// Texts where (pa, va) is strictly better than (pb, vb) under min*: smaller popcount, or
+// equal popcount and larger word.  va / vb have NV planes, compared from the top.
+template <int NV>
+MOTEX_HD uint32_t slice_better(const uint32_t pa[kSlicePc], const uint32_t *va,
+                               const uint32_t pb[kSlicePc], const uint32_t *vb)
+{
+    uint32_t lt = 0, eq = ~0u;  // popcount a < b, a == b so far (from the top bit)
+    for (int q = kSlicePc - 1; q >= 0; --q) {
+        lt |= eq & ~pa[q] & pb[q];
+        eq &= ~(pa[q] ^ pb[q]);
+    }
+    uint32_t gt = 0, veq = ~0u;  // word a > b, a == b so far
+    for (int p = NV - 1; p >= 0; --p) {
+        gt |= veq & va[p] & ~vb[p];
+        veq &= ~(va[p] ^ vb[p]);
+    }
+    return lt | (eq & gt);
+}
+
+// "count <= e" over kSlicePc planes.
+MOTEX_HD uint32_t slice_le(const uint32_t c[kSlicePc], uint32_t e)
+{
+    uint32_t res = ~0u;  // c <= e, deciding from the low bits up
+    for (int q = 0; q < kSlicePc; ++q) {
+        const uint32_t eb = ((e >> q) & 1) ? ~0u : 0u;
+        res = (~c[q] & (res | eb)) | (res & eb);
+    }
+    return res;
+}
+
+// One cell of 32 texts.  U = D[i-1][j], L = D[i][j-1], G = D[i-1][j-1]; delta = texts whose
+// symbol at row i differs from x[j]; full = all ones when j <= K (no truncation) else 0.
+// Returns D[i][j].
+template <int K>
+MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G, uint32_t delta,
+                         uint32_t full, Slice<K> &D, uint32_t *dropped = nullptr)
+{
+    constexpr int W = SliceW<K>::W;
+    // candidates, W+1 planes each (plane W only for the comparison)
+    uint32_t a[W + 1], b[W + 1], c[W + 1];
+    uint32_t pa[kSlicePc], pb[kSlicePc], pcc[kSlicePc];
+    a[0] = ~0u;
+    b[0] = ~0u;
+    c[0] = delta;
+    for (int p = 1; p <= W; ++p) {
+        a[p] = U.d[p - 1];
+        // shiftc keeps bits 0 .. K-2 of the source: planes 1 .. K-1 of the candidate
+        const uint32_t keep = p <= K - 1 ? ~0u : full;
+        b[p] = L.d[p - 1] & keep;
+        c[p] = G.d[p - 1] & keep;
+    }
+    uint32_t lsrc[kSlicePc], gsrc[kSlicePc];
+    for (int q = 0; q < kSlicePc; ++q) {
+        lsrc[q] = (L.pc[q] & full) | (L.pm[q] & ~full);
+        gsrc[q] = (G.pc[q] & full) | (G.pm[q] & ~full);
+    }
+    slice_inc(U.pc, ~0u, pa);
+    slice_inc(lsrc, ~0u, pb);
+    slice_inc(gsrc, delta, pcc);
+    // min*: start from a, replace where b then c is strictly better
+    uint32_t sel = slice_better<W + 1>(pb, b, pa, a);
+    uint32_t d[W + 1], pd[kSlicePc];
+    for (int p = 0; p <= W; ++p) d[p] = (b[p] & sel) | (a[p] & ~sel);
+    for (int q = 0; q < kSlicePc; ++q) pd[q] = (pb[q] & sel) | (pa[q] & ~sel);
+    sel = slice_better<W + 1>(pcc, c, pd, d);
+    for (int p = 0; p < W; ++p) D.d[p] = (c[p] & sel) | (d[p] & ~sel);
+    if (dropped) *dropped |= (c[W] & sel) | (d[W] & ~sel);  // host check of the width bound
+    for (int q = 0; q < kSlicePc; ++q) D.pc[q] = (pcc[q] & sel) | (pd[q] & ~sel);
+    slice_popcount<K - 1>(D.d, D.pm);
+}
+
+// Row 0 of column j (1-based): min(j, K) ones in every text.
+template <int K>
+MOTEX_HD void slice_row0(uint32_t j, Slice<K> &D)
+{
+    constexpr int W = SliceW<K>::W;
+    const uint32_t r = j < (uint32_t)K ? j : (uint32_t)K;
+    for (int p = 0; p < W; ++p) D.d[p] = (uint32_t)p < r ? ~0u : 0u;
+    const uint32_t rm = r < (uint32_t)(K - 1) ? r : (uint32_t)(K - 1);
+    for (int q = 0; q < kSlicePc; ++q) {
+        D.pc[q] = ((r >> q) & 1) ? ~0u : 0u;
+        D.pm[q] = ((rm >> q) & 1) ? ~0u : 0u;
+    }
+}
+
+// Column 0: the zero word.
+template <int K>
+MOTEX_HD void slice_zero(Slice<K> &D)
+{
+    for (int p = 0; p < SliceW<K>::W; ++p) D.d[p] = 0;
+    for (int q = 0; q < kSlicePc; ++q) D.pc[q] = D.pm[q] = 0;
+}
+
+}  // namespace motex

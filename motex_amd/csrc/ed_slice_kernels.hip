@@ -1,0 +1,185 @@
+// Text-parallel edit-distance kernel for gfx950: x against 32 texts at once.
+//
+// What is computed is the edit recurrence of ed_kernels.hip (motifs_extraction_ed,
+// functions.cc:734-768, MaxShift functions.cc:437-447), for one x and a group of up to 32
+// texts: bit t of every word belongs to text t of the group, and ed_slice.h advances one
+// cell of all 32 texts with bitwise operations (a word is a stack of bit planes).
+//
+// Layout.  A wave owns a strip of 64 columns of x, one per lane, and streams the group's
+// rows through it as a skewed wavefront: at step s lane L works on row s-L.  The left
+// neighbour's cell (all its planes) and the row's text planes arrive from lane L-1 by one
+// DPP wave shift per word; lane 0 takes them from an LDS chunk of 64 rows holding the
+// previous strip's last column (the boundary stream, in the wave's scratch) and the
+// group's text planes.  Lane 63 writes its cells to the next strip's boundary stream.
+// Per column, the texts with an occurrence (OR of the hit masks over rows) and the number
+// of occurrences (popcount of the hit masks) go to u and v with one atomic each per strip.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ed_slice.h"
+#include "motex_internal.h"
+
+namespace motex {
+
+// wave_shr:1 -- lane L receives lane L-1's x; lane 0 receives in0
+__device__ __forceinline__ uint32_t shr1_from(uint32_t in0, uint32_t x)
+{
+    return __builtin_amdgcn_update_dpp(in0, x, 0x138, 0xF, 0xF, false);
+}
+
+template <int K>
+struct SliceLayout {
+    static constexpr int W = SliceW<K>::W;
+    static constexpr int SW = W + 2 * kSlicePc;         // words of one cell
+    static constexpr int SWS = (SW + 3) & ~3;           // boundary-stream row (16-byte aligned)
+    static constexpr int LW = (SW + 6 + 3) & ~3;        // LDS chunk row: cell + 6 text words
+};
+
+template <int K>
+__global__ __launch_bounds__(64) void ed_slice_kernel(EdSliceArgs a)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr int W = Lay::W, SW = Lay::SW, SWS = Lay::SWS, LW = Lay::LW;
+    __shared__ uint32_t chunk[64 * LW];
+    const uint32_t lane = threadIdx.x;
+    uint32_t *st0 = a.scratch + (size_t)blockIdx.x * 2 * a.stream_words;
+    uint32_t *st1 = st0 + a.stream_words;
+
+    for (uint32_t round = 0;; ++round) {
+        uint32_t it;
+        if (a.persistent) {
+            it = 0;
+            if (lane == 0) it = atomicAdd(a.queue, 1u);
+            it = __builtin_amdgcn_readfirstlane(it);
+        } else {
+            it = round == 0 ? blockIdx.x : a.nitems;
+        }
+        if (it >= a.nitems) break;
+        const EdSliceItem I = a.items[it];
+        const uint32_t m = I.m;
+        const uint32_t g0 = a.goff[I.group];
+        const uint32_t nrows = a.goff[I.group + 1] - g0;
+        const uint32_t *gp = a.gplanes + (size_t)g0 * 6;
+        const uint32_t nstrips = (m + 63) / 64;
+        for (uint32_t strip = 0; strip < nstrips; ++strip) {
+            const uint32_t *sin = (strip & 1) ? st1 : st0;
+            uint32_t *sout = (strip & 1) ? st0 : st1;
+            const bool last = strip + 1 == nstrips;
+            const uint32_t j = strip * 64 + lane + 1;  // 1-based column
+            const uint32_t code = j <= m ? a.codes[I.x_start + j - 1] : 0u;
+            uint32_t xb[5];
+#pragma unroll
+            for (int b = 0; b < 5; ++b) xb[b] = ((code >> b) & 1) ? ~0u : 0u;
+            const uint32_t full = j <= (uint32_t)K ? ~0u : 0u;
+            const bool report = j <= m && j >= (uint32_t)K;
+            Slice<K> U, G;  // D[i-1][j] (also what lane L+1 reads), D[i-1][j-1]
+            slice_row0<K>(j, U);
+            slice_row0<K>(j - 1, G);
+            uint32_t tpub[6] = {0, 0, 0, 0, 0, 0};
+            uint32_t vcnt = 0, anyh = 0;
+            const uint32_t steps = nrows + 63;
+            for (uint32_t s = 1; s <= steps; ++s) {
+                const uint32_t r0 = s - 1;  // 0-based row lane 0 starts this step
+                if ((r0 & 63) == 0) {
+                    __syncthreads();  // the previous chunk is no longer read
+                    for (uint32_t w = lane; w < 64u * LW; w += 64) {
+                        const uint32_t rr = w / LW, c = w % LW, row = r0 + rr;
+                        uint32_t val = 0;
+                        if (row < nrows) {
+                            if (c < (uint32_t)SW) val = strip ? sin[(size_t)row * SWS + c] : 0u;
+                            else if (c < (uint32_t)SW + 6) val = gp[(size_t)row * 6 + (c - SW)];
+                        }
+                        chunk[w] = val;
+                    }
+                    __syncthreads();
+                }
+                const uint32_t *cr = chunk + (r0 & 63) * LW;
+                Slice<K> L;  // D[i][j-1]
+#pragma unroll
+                for (int p = 0; p < W; ++p) L.d[p] = shr1_from(cr[p], U.d[p]);
+#pragma unroll
+                for (int q = 0; q < kSlicePc; ++q) {
+                    L.pc[q] = shr1_from(cr[W + q], U.pc[q]);
+                    L.pm[q] = shr1_from(cr[W + kSlicePc + q], U.pm[q]);
+                }
+                uint32_t t[6];
+#pragma unroll
+                for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[SW + b], tpub[b]);
+                const bool active = s > lane && s - lane <= nrows;
+                if (active) {
+                    uint32_t ne = 0;
+#pragma unroll
+                    for (int b = 0; b < 5; ++b) ne |= t[b] ^ xb[b];
+                    Slice<K> D;
+                    slice_cell<K>(U, L, G, ne, full, D);
+                    if (report) {
+                        const uint32_t hit = slice_le(D.pc, a.e) & t[5];
+                        vcnt += __builtin_popcount(hit);
+                        anyh |= hit;
+                    }
+                    G = L;
+                    U = D;
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) tpub[b] = t[b];
+                    if (!last && lane == 63) {
+                        // row s-63 (0-based s-64) of this strip's last column
+                        uint32_t w[SWS];
+#pragma unroll
+                        for (int p = 0; p < W; ++p) w[p] = D.d[p];
+#pragma unroll
+                        for (int q = 0; q < kSlicePc; ++q) {
+                            w[W + q] = D.pc[q];
+                            w[W + kSlicePc + q] = D.pm[q];
+                        }
+#pragma unroll
+                        for (int p = SW; p < SWS; ++p) w[p] = 0;
+                        uint4 *dst = reinterpret_cast<uint4 *>(sout + (size_t)(s - 64) * SWS);
+#pragma unroll
+                        for (int p = 0; p < SWS / 4; ++p)
+                            dst[p] = make_uint4(w[4 * p], w[4 * p + 1], w[4 * p + 2], w[4 * p + 3]);
+                    }
+                }
+            }
+            if (report) {
+                if (anyh) atomicAdd(a.u + I.uv_base + j - 1, (uint32_t)__builtin_popcount(anyh));
+                if (vcnt) atomicAdd(a.v + I.uv_base + j - 1, vcnt);
+            }
+            __threadfence_block();  // the boundary stream is read by this wave's next strip
+            __syncthreads();
+        }
+    }
+}
+
+template <int K>
+static uint32_t stream_row_words() { return SliceLayout<K>::SWS; }
+
+uint32_t ed_slice_stream_row_words(uint32_t k)
+{
+    switch (k) {
+    case 4: return stream_row_words<4>();
+    case 5: return stream_row_words<5>();
+    case 6: return stream_row_words<6>();
+    case 7: return stream_row_words<7>();
+    case 8: return stream_row_words<8>();
+    case 9: return stream_row_words<9>();
+    case 10: return stream_row_words<10>();
+    default: return 0;
+    }
+}
+
+hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, hipStream_t s)
+{
+    switch (a.k) {
+    case 4: hipLaunchKernelGGL(ed_slice_kernel<4>, dim3(grid), dim3(64), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(ed_slice_kernel<5>, dim3(grid), dim3(64), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(ed_slice_kernel<6>, dim3(grid), dim3(64), 0, s, a); break;
+    case 7: hipLaunchKernelGGL(ed_slice_kernel<7>, dim3(grid), dim3(64), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(ed_slice_kernel<8>, dim3(grid), dim3(64), 0, s, a); break;
+    case 9: hipLaunchKernelGGL(ed_slice_kernel<9>, dim3(grid), dim3(64), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(ed_slice_kernel<10>, dim3(grid), dim3(64), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace motex

@@ -71,6 +71,27 @@ struct EdItemDev {
     uint32_t pad;
 };
 
+// Text-parallel edit kernel: one x against a group of up to 32 texts (one bit each).
+struct EdSliceItem {
+    uint32_t x_start;   // text position of x
+    uint32_t m;         // length of x
+    uint32_t uv_base;   // u/v index of x's pos_x = 0
+    uint32_t group;     // text group
+};
+struct EdSliceArgs {
+    const uint8_t *codes;
+    const EdSliceItem *items;
+    const uint32_t *gplanes;  // per group row: 5 symbol-code planes + the valid-text mask
+    const uint32_t *goff;     // groups + 1 entries: first row of each group in gplanes
+    uint32_t *u, *v;
+    uint32_t *queue;
+    uint32_t *scratch;        // per workgroup: two boundary streams of stream_words words
+    uint32_t stream_words;
+    uint32_t nitems;
+    uint32_t k, e;
+    uint32_t persistent;
+};
+
 struct EdArgs {
     const uint8_t *codes;
     const uint32_t *seq_start;

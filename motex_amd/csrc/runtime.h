@@ -149,6 +149,9 @@ struct motex_batch {
     motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2, d_ranges, d_hitq;
     motex::DevBuf d_fbits;           // pair-symmetric Hamming runs: x-by-text occurrence bits
     uint32_t f_words = 0;
+    bool ed_slice = false;           // edit runs on the text-parallel kernel
+    motex::DevBuf d_gplanes, d_goff; // its text groups
+    uint32_t ed_slice_stream_words = 0;
     uint64_t work_items() const { return hd.items.size() + ed_nitems; }
 };
 

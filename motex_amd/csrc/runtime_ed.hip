@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "runtime.h"
@@ -9,6 +10,8 @@
 namespace motex {
 
 hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, hipStream_t s);
+hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, hipStream_t s);
+uint32_t ed_slice_stream_row_words(uint32_t k);
 uint32_t ed_strip_cols(bool wide);
 
 static const uint64_t kEdTextPerItem = 8192;  // text symbols per work item
@@ -66,8 +69,85 @@ static uint32_t ed_grid(const motex_batch *b)
     return (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, b->ed_nitems);
 }
 
+// Text-parallel plan (ed_slice_kernels.hip): the texts that can hold an occurrence
+// (length >= k), longest first, in groups of 32; per group row, the 5 bit planes of the 32
+// texts' symbol codes and the mask of texts still running.  Items are (x, group), most
+// expensive first.  Used for k in 4..10 and alphabets of at most 32 symbols (5 planes);
+// MOTEX_ED_SLICE=0 keeps the one-text-per-lane kernel.
+static bool ed_slice_ok(const motex_batch *b)
+{
+    const char *env = getenv("MOTEX_ED_SLICE");
+    if (env && env[0] == '0') return false;
+    return b->params.k >= 4 && b->params.k <= 10 && b->ss.sigma <= 32 &&
+           ed_slice_stream_row_words(b->params.k) != 0;
+}
+
+static int ed_prepare_slice(motex_batch *b)
+{
+    const SeqSet &ss = b->ss;
+    const uint32_t k = b->params.k;
+    std::vector<uint32_t> texts;
+    for (uint32_t j = 0; j < ss.num; ++j)
+        if (ss.len[j] >= k) texts.push_back(j);
+    std::stable_sort(texts.begin(), texts.end(), [&](uint32_t x, uint32_t z) { return ss.len[x] > ss.len[z]; });
+    const uint32_t ngroups = (uint32_t)((texts.size() + 31) / 32);
+    std::vector<uint32_t> goff(ngroups + 1, 0);
+    for (uint32_t g = 0; g < ngroups; ++g) goff[g + 1] = goff[g] + ss.len[texts[32 * g]];
+    std::vector<uint32_t> planes((size_t)goff[ngroups] * 6, 0);
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        for (uint32_t t = 0; t < 32 && 32 * g + t < texts.size(); ++t) {
+            const uint32_t j = texts[32 * g + t];
+            const uint8_t *code = ss.codes.data() + ss.start[j];
+            for (uint32_t r = 0; r < ss.len[j]; ++r) {
+                uint32_t *row = planes.data() + (size_t)(goff[g] + r) * 6;
+                for (int bit = 0; bit < 5; ++bit) row[bit] |= (uint32_t)((code[r] >> bit) & 1) << t;
+                row[5] |= 1u << t;
+            }
+        }
+    }
+    std::vector<EdSliceItem> items;
+    for (uint32_t i = 0; i < ss.num; ++i) {
+        if (ss.len[i] < k) continue;
+        for (uint32_t g = 0; g < ngroups; ++g)
+            items.push_back({ss.start[i], ss.len[i], (uint32_t)ss.uv_off[i], g});
+    }
+    auto cost = [&](const EdSliceItem &it) {
+        return (uint64_t)((it.m + 63) / 64) * (goff[it.group + 1] - goff[it.group] + 63);
+    };
+    std::stable_sort(items.begin(), items.end(), [&](const EdSliceItem &x, const EdSliceItem &z) {
+        return cost(x) > cost(z);
+    });
+    b->tile = choose_tile(b->ctx, b->params, ss);
+    b->tile.variant = 2;
+    b->ed_slice = true;
+    b->ed_nitems = (uint32_t)items.size();
+    uint32_t rows_max = 0;
+    for (uint32_t g = 0; g < ngroups; ++g) rows_max = std::max(rows_max, goff[g + 1] - goff[g]);
+    b->ed_slice_stream_words = std::max<uint32_t>(rows_max, 1) * ed_slice_stream_row_words(k);
+    hipError_t e;
+    hipStream_t s = bstream(b);
+    if ((e = b->d_items.alloc(std::max<size_t>(items.size(), 1) * sizeof(EdSliceItem))) != hipSuccess)
+        return hip_fail(e, "alloc items");
+    if (!items.empty() &&
+        (e = hipMemcpyAsync(b->d_items.p, items.data(), items.size() * sizeof(EdSliceItem), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(e, "copy items");
+    if ((e = b->d_gplanes.alloc(std::max<size_t>(planes.size(), 1) * 4)) != hipSuccess) return hip_fail(e, "alloc text planes");
+    if (!planes.empty() &&
+        (e = hipMemcpyAsync(b->d_gplanes.p, planes.data(), planes.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(e, "copy text planes");
+    if ((e = b->d_goff.alloc(goff.size() * 4)) != hipSuccess) return hip_fail(e, "alloc group offsets");
+    if ((e = hipMemcpyAsync(b->d_goff.p, goff.data(), goff.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(e, "copy group offsets");
+    // boundary streams of the persistent grid, allocated here so that runs allocate nothing
+    const uint64_t need = (uint64_t)std::max<uint32_t>(ed_grid(b), 1) * 2 * b->ed_slice_stream_words * 4;
+    if ((e = batch_realloc_scratch(b, need)) != hipSuccess) return hip_fail(e, "alloc edit streams");
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "upload items");
+    return MOTEX_OK;
+}
+
 int ed_prepare(motex_batch *b)
 {
+    if (ed_slice_ok(b)) return ed_prepare_slice(b);
     const SeqSet &ss = b->ss;
     std::vector<uint32_t> xs(ss.num);
     for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
@@ -129,7 +209,25 @@ int ed_launch_all(motex_batch *b)
     if (!b->ed_nitems) return MOTEX_OK;
     uint32_t grid = ed_grid(b);
     if (b->grid_cap) grid = std::min(grid, b->grid_cap);
-    return ed_launch(b, b->d_items.p, b->ed_nitems, grid, 1, bstream(b));
+    if (!b->ed_slice) return ed_launch(b, b->d_items.p, b->ed_nitems, grid, 1, bstream(b));
+    if ((uint64_t)grid * 2 * b->ed_slice_stream_words * 4 > b->d_aux.bytes)
+        return set_error(MOTEX_EINVAL, "internal: edit grid of %u beyond its boundary streams", grid);
+    EdSliceArgs a{};
+    a.codes = (const uint8_t *)b->d_codes.p;
+    a.items = (const EdSliceItem *)b->d_items.p;
+    a.gplanes = (const uint32_t *)b->d_gplanes.p;
+    a.goff = (const uint32_t *)b->d_goff.p;
+    a.u = (uint32_t *)b->d_u.p;
+    a.v = (uint32_t *)b->d_v.p;
+    a.queue = (uint32_t *)b->d_queue.p;
+    a.scratch = (uint32_t *)b->d_aux.p;
+    a.stream_words = b->ed_slice_stream_words;
+    a.nitems = b->ed_nitems;
+    a.k = b->params.k;
+    a.e = b->params.e;
+    a.persistent = 1;
+    hipError_t e = launch_ed_slice(a, grid, bstream(b));
+    return e == hipSuccess ? MOTEX_OK : hip_fail(e, "launch text-parallel edit kernel");
 }
 
 // Plain form: one launch per pair (one work item, one wave).

@@ -65,7 +65,9 @@ static size_t footprint(const Job &j, const motex_ctx *c)
         bytes += (T / 256 + 1) * (T / 7000 + 1) * sizeof(HdItem); // work items (upper bound)
         bytes += grid * 2 * kHitQueueCap * 5 * 4;                 // hit queues (uncapped grid)
     } else {
-        bytes += grid * 2 * (maxlen + 128) * 8 + (uint64_t)j.lens.size() * j.lens.size() * sizeof(EdItemDev);
+        // boundary streams (text-parallel kernel: up to 32 words per row, grid of 12 waves/CU)
+        bytes += std::max<uint64_t>(grid * 2 * (maxlen + 128) * 8, (uint64_t)c->num_cu * 12 * 2 * maxlen * 32 * 4);
+        bytes += (uint64_t)j.lens.size() * j.lens.size() * sizeof(EdItemDev) + T * 24;
     }
     return (size_t)(bytes * 1.25);
 }

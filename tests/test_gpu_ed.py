@@ -12,19 +12,41 @@ def rand_seqs(rng, n, lo, hi, alpha):
     return [a[rng.integers(0, len(a), rng.integers(lo, hi + 1))].tobytes() for _ in range(n)]
 
 
-def check(ctx, oracle, seqs, k, e, alphabet="DNA"):
-    print(f"[ed parity] n={len(seqs)} total={sum(map(len, seqs))} k={k} e={e}", flush=True)
-    u, v, off = ctx.extract_all_pairs(seqs, MotexParams(k=k, e=e, distance=1, alphabet=alphabet))
+@pytest.fixture(params=["1", "0"], ids=["text_parallel", "one_text_per_lane"])
+def kernel(request, monkeypatch):
+    """Runs a test on the text-parallel edit kernel (32 texts per lane word; used for k in
+    4..10) and again with it switched off."""
+    monkeypatch.setenv("MOTEX_ED_SLICE", request.param)
+    return request.param
+
+
+def check(ctx, oracle, seqs, k, e, alphabet="DNA", kernel=None):
+    print(f"[ed parity] n={len(seqs)} total={sum(map(len, seqs))} k={k} e={e} kernel={kernel}", flush=True)
+    p = MotexParams(k=k, e=e, distance=1, alphabet=alphabet)
+    b = ctx.batch(seqs, p)
+    st = b.run()
+    u, v, off = b.download()
+    b.close()
+    if kernel is not None:
+        sliced = kernel == "1" and 4 <= k <= 10
+        assert (st["kernel_variant"] == 2) == sliced
     ou, ov, ooff, _ = oracle.all_pairs(seqs, 1, k, e)
     bad = np.nonzero((u != ou) | (v != ov))[0]
     assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: gpu {u[bad[:5]]},{v[bad[:5]]} oracle {ou[bad[:5]]},{ov[bad[:5]]}"
     return u, v
 
 
-def test_config3_subset(ctx, oracle):
+def test_config3_subset(ctx, oracle, kernel):
     w = synth.config3(scale=0.03)  # 15 protein sequences, lengths 200..2000
-    u, v = check(ctx, oracle, w.seqs, w.k, w.e, "PROT")
+    u, v = check(ctx, oracle, w.seqs, w.k, w.e, "PROT", kernel)
     assert v.sum() > 0
+
+
+def test_text_groups_of_mixed_lengths(ctx, oracle):
+    # 70 texts: three groups of 32 texts sorted by length, ragged ends, short texts skipped
+    rng = np.random.default_rng(77)
+    seqs = rand_seqs(rng, 70, 2, 900, b"ACGT")
+    check(ctx, oracle, seqs, 7, 2, kernel="1")
 
 
 @pytest.mark.parametrize("seed,k,e,alpha,n,lo,hi", [
@@ -36,10 +58,12 @@ def test_config3_subset(ctx, oracle):
     (6, 20, 4, b"ACGT", 5, 100, 400),      # 64-bit words
     (7, 32, 6, b"AC", 4, 100, 300),        # k = 32
     (8, 3, 2, b"ACGT", 8, 2, 70),          # e = k-1, texts shorter than one wave
+    (9, 10, 9, b"A", 5, 10, 200),          # one symbol: widest words, every tie
+    (10, 4, 1, b"ARNDCQEGHILKMFPSTWYV", 40, 4, 300),  # protein, more than one text group
 ])
-def test_random_edges(ctx, oracle, seed, k, e, alpha, n, lo, hi):
+def test_random_edges(ctx, oracle, kernel, seed, k, e, alpha, n, lo, hi):
     rng = np.random.default_rng(100 + seed)
-    check(ctx, oracle, rand_seqs(rng, n, lo, hi, alpha), k, e)
+    check(ctx, oracle, rand_seqs(rng, n, lo, hi, alpha), k, e, kernel=kernel)
 
 
 def test_plain_form_equals_batched(ctx):
