@@ -40,6 +40,27 @@ struct Slice {
     uint32_t pm[kSlicePc];  // popcount of bits 0 .. K-2
 };
 
+// sel ? t : f, per bit (one 3-input logic op on gfx950: the compiler's own choice for this
+// pattern, v_bfi_b32, issues at about 60% of the v_bitop3_b32 rate there)
+MOTEX_HD uint32_t slice_mux(uint32_t sel, uint32_t t, uint32_t f)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(t, f, sel, 0xE4);
+#else
+    return (t & sel) | (f & ~sel);
+#endif
+}
+
+// x & m for a mask m that is all ones or all zeros per lane (kept a logic op, not a select)
+MOTEX_HD uint32_t slice_and(uint32_t x, uint32_t m)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(x, m, x, 0xC0);
+#else
+    return x & m;
+#endif
+}
+
 // out = in + carry (carry: one 0/1 per text)
 MOTEX_HD void slice_inc(const uint32_t in[kSlicePc], uint32_t carry, uint32_t out[kSlicePc])
 {
@@ -96,23 +117,36 @@ MOTEX_HD void slice_popcount(const uint32_t *x, uint32_t out[kSlicePc])
     for (int q = 2; q < kSlicePc; ++q) out[q] = hi[q - 2];
 }
 
+// Texts where a > b for numbers of N planes (plane N-1 most significant).  The planes are
+// combined pairwise in a tree (greater, equal) so the dependency chain is log2(N) deep.
+template <int N>
+MOTEX_HD void slice_cmp(const uint32_t *a, const uint32_t *b, uint32_t &gt, uint32_t &eq)
+{
+    uint32_t g[N], e[N];
+    for (int p = 0; p < N; ++p) {
+        g[p] = a[p] & ~b[p];
+        e[p] = ~(a[p] ^ b[p]);
+    }
+    for (int step = 1; step < N; step *= 2)
+        for (int p = 0; p + step < N; p += 2 * step) {
+            g[p] = g[p + step] | (e[p + step] & g[p]);
+            e[p] = e[p + step] & e[p];
+        }
+    gt = g[0];
+    eq = e[0];
+}
+
 // Texts where (pa, va) is strictly better than (pb, vb) under min*: smaller popcount, or
-// equal popcount and larger word.  va / vb have NV planes, compared from the top.
+// equal popcount and larger word.  va / vb have NV planes.
 template <int NV>
 MOTEX_HD uint32_t slice_better(const uint32_t pa[kSlicePc], const uint32_t *va,
                                const uint32_t pb[kSlicePc], const uint32_t *vb)
 {
-    uint32_t lt = 0, eq = ~0u;  // popcount a < b, a == b so far (from the top bit)
-    for (int q = kSlicePc - 1; q >= 0; --q) {
-        lt |= eq & ~pa[q] & pb[q];
-        eq &= ~(pa[q] ^ pb[q]);
-    }
-    uint32_t gt = 0, veq = ~0u;  // word a > b, a == b so far
-    for (int p = NV - 1; p >= 0; --p) {
-        gt |= veq & va[p] & ~vb[p];
-        veq &= ~(va[p] ^ vb[p]);
-    }
-    return lt | (eq & gt);
+    uint32_t pgt, peq, vgt, veq;
+    slice_cmp<kSlicePc>(pb, pa, pgt, peq);  // pb > pa: a has the smaller popcount
+    slice_cmp<NV>(va, vb, vgt, veq);
+    (void)veq;
+    return pgt | (peq & vgt);
 }
 
 // "count <= e" over kSlicePc planes.
@@ -143,14 +177,13 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
     for (int p = 1; p <= W; ++p) {
         a[p] = U.d[p - 1];
         // shiftc keeps bits 0 .. K-2 of the source: planes 1 .. K-1 of the candidate
-        const uint32_t keep = p <= K - 1 ? ~0u : full;
-        b[p] = L.d[p - 1] & keep;
-        c[p] = G.d[p - 1] & keep;
+        b[p] = p <= K - 1 ? L.d[p - 1] : slice_and(L.d[p - 1], full);
+        c[p] = p <= K - 1 ? G.d[p - 1] : slice_and(G.d[p - 1], full);
     }
     uint32_t lsrc[kSlicePc], gsrc[kSlicePc];
     for (int q = 0; q < kSlicePc; ++q) {
-        lsrc[q] = (L.pc[q] & full) | (L.pm[q] & ~full);
-        gsrc[q] = (G.pc[q] & full) | (G.pm[q] & ~full);
+        lsrc[q] = slice_mux(full, L.pc[q], L.pm[q]);
+        gsrc[q] = slice_mux(full, G.pc[q], G.pm[q]);
     }
     slice_inc(U.pc, ~0u, pa);
     slice_inc(lsrc, ~0u, pb);
@@ -158,12 +191,12 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
     // min*: start from a, replace where b then c is strictly better
     uint32_t sel = slice_better<W + 1>(pb, b, pa, a);
     uint32_t d[W + 1], pd[kSlicePc];
-    for (int p = 0; p <= W; ++p) d[p] = (b[p] & sel) | (a[p] & ~sel);
-    for (int q = 0; q < kSlicePc; ++q) pd[q] = (pb[q] & sel) | (pa[q] & ~sel);
+    for (int p = 0; p <= W; ++p) d[p] = slice_mux(sel, b[p], a[p]);
+    for (int q = 0; q < kSlicePc; ++q) pd[q] = slice_mux(sel, pb[q], pa[q]);
     sel = slice_better<W + 1>(pcc, c, pd, d);
-    for (int p = 0; p < W; ++p) D.d[p] = (c[p] & sel) | (d[p] & ~sel);
-    if (dropped) *dropped |= (c[W] & sel) | (d[W] & ~sel);  // host check of the width bound
-    for (int q = 0; q < kSlicePc; ++q) D.pc[q] = (pcc[q] & sel) | (pd[q] & ~sel);
+    for (int p = 0; p < W; ++p) D.d[p] = slice_mux(sel, c[p], d[p]);
+    if (dropped) *dropped |= slice_mux(sel, c[W], d[W]);  // host check of the width bound
+    for (int q = 0; q < kSlicePc; ++q) D.pc[q] = slice_mux(sel, pcc[q], pd[q]);
     slice_popcount<K - 1>(D.d, D.pm);
 }
 

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "ed_slice.h"
 #include "motex_internal.h"
 
@@ -78,7 +80,8 @@ __global__ __launch_bounds__(64) void ed_slice_kernel(EdSliceArgs a)
             uint32_t tpub[6] = {0, 0, 0, 0, 0, 0};
             uint32_t vcnt = 0, anyh = 0;
             const uint32_t steps = nrows + 63;
-            for (uint32_t s = 1; s <= steps; ++s) {
+            // one wavefront step; CHECK: fill/drain form (lanes outside rows 1..nrows idle)
+            auto step = [&](uint32_t s, auto check) {
                 const uint32_t r0 = s - 1;  // 0-based row lane 0 starts this step
                 if ((r0 & 63) == 0) {
                     __syncthreads();  // the previous chunk is no longer read
@@ -105,7 +108,8 @@ __global__ __launch_bounds__(64) void ed_slice_kernel(EdSliceArgs a)
                 uint32_t t[6];
 #pragma unroll
                 for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[SW + b], tpub[b]);
-                const bool active = s > lane && s - lane <= nrows;
+                bool active = true;
+                if constexpr (decltype(check)::value) active = s > lane && s - lane <= nrows;
                 if (active) {
                     uint32_t ne = 0;
 #pragma unroll
@@ -139,7 +143,22 @@ __global__ __launch_bounds__(64) void ed_slice_kernel(EdSliceArgs a)
                             dst[p] = make_uint4(w[4 * p], w[4 * p + 1], w[4 * p + 2], w[4 * p + 3]);
                     }
                 }
+            };
+            // fill (s < 64), steady state (64 <= s <= nrows: every lane has a row; two steps
+            // per trip, so the cell planes are not copied between registers on each back
+            // edge), drain (s > nrows)
+            uint32_t s = 1;
+            const uint32_t fill_end = nrows < 63 ? nrows : 63;
+            for (; s <= fill_end; ++s) step(s, std::true_type{});
+            for (; s + 1 <= nrows; s += 2) {
+                step(s, std::false_type{});
+                step(s + 1, std::false_type{});
             }
+            if (s <= nrows) {  // here s >= 64
+                step(s, std::false_type{});
+                ++s;
+            }
+            for (; s <= steps; ++s) step(s, std::true_type{});
             if (report) {
                 if (anyh) atomicAdd(a.u + I.uv_base + j - 1, (uint32_t)__builtin_popcount(anyh));
                 if (vcnt) atomicAdd(a.v + I.uv_base + j - 1, vcnt);
