@@ -200,6 +200,46 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
     slice_popcount<K - 1>(D.d, D.pm);
 }
 
+// The cell for columns j > K, where left and diag enter truncated: their candidates have no
+// plane at or above K, so only pm of left and diag and pc of up are needed, and the result's
+// pm is left to the reader (slice_pm).  up: d, pc; left, diag: d, pm.  Returns d, pc.
+template <int K>
+MOTEX_HD void slice_cell_masked(const uint32_t *ud, const uint32_t *upc, const uint32_t *ld,
+                                const uint32_t *lpm, const uint32_t *gd, const uint32_t *gpm,
+                                uint32_t delta, uint32_t *dd, uint32_t *dpc,
+                                uint32_t *dropped = nullptr)
+{
+    constexpr int W = SliceW<K>::W;
+    uint32_t a[W + 1], b[W + 1], c[W + 1];
+    uint32_t pa[kSlicePc], pb[kSlicePc], pcc[kSlicePc];
+    a[0] = ~0u;
+    b[0] = ~0u;
+    c[0] = delta;
+    for (int p = 1; p <= W; ++p) {
+        a[p] = ud[p - 1];
+        b[p] = p <= K - 1 ? ld[p - 1] : 0u;
+        c[p] = p <= K - 1 ? gd[p - 1] : 0u;
+    }
+    slice_inc(upc, ~0u, pa);
+    slice_inc(lpm, ~0u, pb);
+    slice_inc(gpm, delta, pcc);
+    uint32_t sel = slice_better<W + 1>(pb, b, pa, a);
+    uint32_t d[W + 1], pd[kSlicePc];
+    for (int p = 0; p <= W; ++p) d[p] = slice_mux(sel, b[p], a[p]);
+    for (int q = 0; q < kSlicePc; ++q) pd[q] = slice_mux(sel, pb[q], pa[q]);
+    sel = slice_better<W + 1>(pcc, c, pd, d);
+    for (int p = 0; p < W; ++p) dd[p] = slice_mux(sel, c[p], d[p]);
+    if (dropped) *dropped |= slice_mux(sel, c[W], d[W]);
+    for (int q = 0; q < kSlicePc; ++q) dpc[q] = slice_mux(sel, pcc[q], pd[q]);
+}
+
+// pm of a word: the popcount of its low K-1 planes.
+template <int K>
+MOTEX_HD void slice_pm(const uint32_t *d, uint32_t pm[kSlicePc])
+{
+    slice_popcount<K - 1>(d, pm);
+}
+
 // Row 0 of column j (1-based): min(j, K) ones in every text.
 template <int K>
 MOTEX_HD void slice_row0(uint32_t j, Slice<K> &D)

@@ -7,10 +7,12 @@
 //
 // Layout.  A wave owns a strip of 64 columns of x, one per lane, and streams the group's
 // rows through it as a skewed wavefront: at step s lane L works on row s-L.  The left
-// neighbour's cell (all its planes) and the row's text planes arrive from lane L-1 by one
-// DPP wave shift per word; lane 0 takes them from an LDS chunk of 64 rows holding the
-// previous strip's last column (the boundary stream, in the wave's scratch) and the
-// group's text planes.  Lane 63 writes its cells to the next strip's boundary stream.
+// neighbour's cell and the row's text planes arrive from lane L-1 by one DPP wave shift per
+// word; lane 0 takes them from an LDS chunk of 64 rows holding the previous strip's last
+// column (the boundary stream, in the wave's scratch) and the group's text planes.  Lane 63
+// writes its cells to the next strip's boundary stream.  Past the first strip every column
+// has j > K, so only the word's planes travel and the receiver recounts the popcount of its
+// low K-1 planes (slice_cell_masked).
 // Per column, the texts with an occurrence (OR of the hit masks over rows) and the number
 // of occurrences (popcount of the hit masks) go to u and v with one atomic each per strip.
 #include <hip/hip_runtime.h>
@@ -32,17 +34,134 @@ __device__ __forceinline__ uint32_t shr1_from(uint32_t in0, uint32_t x)
 template <int K>
 struct SliceLayout {
     static constexpr int W = SliceW<K>::W;
-    static constexpr int SW = W + 2 * kSlicePc;         // words of one cell
-    static constexpr int SWS = (SW + 3) & ~3;           // boundary-stream row (16-byte aligned)
-    static constexpr int LW = (SW + 6 + 3) & ~3;        // LDS chunk row: cell + 6 text words
+    static constexpr int SWS = (W + 3) & ~3;      // boundary-stream row: the word's planes
+    static constexpr int LW = (W + 6 + 3) & ~3;   // LDS chunk row: stream row + 6 text words
 };
+
+// Loads the LDS chunk of 64 rows from r0: the previous strip's boundary cells (strip > 0)
+// and the group's text planes.
+template <int K>
+__device__ __forceinline__ void slice_chunk(uint32_t *chunk, uint32_t r0, uint32_t nrows, bool from_stream,
+                                            const uint32_t *sin, const uint32_t *gp)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr int W = Lay::W, SWS = Lay::SWS, LW = Lay::LW;
+    __syncthreads();  // the previous chunk is no longer read
+    for (uint32_t w = threadIdx.x; w < 64u * LW; w += 64) {
+        const uint32_t rr = w / LW, c = w % LW, row = r0 + rr;
+        uint32_t val = 0;
+        if (row < nrows) {
+            if (c < (uint32_t)W) val = from_stream ? sin[(size_t)row * SWS + c] : 0u;
+            else if (c < (uint32_t)W + 6) val = gp[(size_t)row * 6 + (c - W)];
+        }
+        chunk[w] = val;
+    }
+    __syncthreads();
+}
+
+// Lane 63 writes its cell of row s-63 (0-based s-64) to the next strip's boundary stream.
+template <int K>
+__device__ __forceinline__ void slice_emit(uint32_t *sout, uint32_t s, const uint32_t *d)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr int W = Lay::W, SWS = Lay::SWS;
+    uint32_t w[SWS];
+#pragma unroll
+    for (int p = 0; p < SWS; ++p) w[p] = p < W ? d[p] : 0u;
+    uint4 *dst = reinterpret_cast<uint4 *>(sout + (size_t)(s - 64) * SWS);
+#pragma unroll
+    for (int p = 0; p < SWS / 4; ++p) dst[p] = make_uint4(w[4 * p], w[4 * p + 1], w[4 * p + 2], w[4 * p + 3]);
+}
+
+// One strip of 64 columns over all rows of the group.  Only a cell's planes travel between
+// lanes; the receiver recounts its popcounts.  FIRST: the first strip, whose left edge is
+// column 0 and whose first K columns enter left and diag untruncated (slice_cell, needing
+// pc and pm of left).  Later strips have only columns j > K (slice_cell_masked, needing pm).
+template <int K, bool FIRST>
+__device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceItem &I, uint32_t strip,
+                                            uint32_t nrows, const uint32_t *gp, const uint32_t *sin,
+                                            uint32_t *sout, bool last, uint32_t *chunk)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr int W = Lay::W, LW = Lay::LW;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t m = I.m;
+    const uint32_t j = strip * 64 + lane + 1;  // 1-based column
+    const uint32_t code = j <= m ? a.codes[I.x_start + j - 1] : 0u;
+    uint32_t xb[5];
+#pragma unroll
+    for (int b = 0; b < 5; ++b) xb[b] = ((code >> b) & 1) ? ~0u : 0u;
+    const bool report = j <= m && j >= (uint32_t)K;
+    const uint32_t full = j <= (uint32_t)K ? ~0u : 0u;
+    // state: D[i-1][j] (what lane L+1 receives next) and D[i-1][j-1]
+    Slice<K> U, G;
+    slice_row0<K>(j, U);
+    slice_row0<K>(j - 1, G);
+    uint32_t tpub[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t vcnt = 0, anyh = 0;
+    const uint32_t steps = nrows + 63;
+    auto step = [&](uint32_t s, auto check) {
+        const uint32_t r0 = s - 1;  // 0-based row lane 0 starts this step
+        if ((r0 & 63) == 0) slice_chunk<K>(chunk, r0, nrows, !FIRST, sin, gp);
+        const uint32_t *cr = chunk + (r0 & 63) * LW;
+        Slice<K> L;  // D[i][j-1]; only its planes travel, the receiver recounts pc and pm
+#pragma unroll
+        for (int p = 0; p < W; ++p) L.d[p] = shr1_from(FIRST ? 0u : cr[p], U.d[p]);
+        if constexpr (FIRST) slice_popcount<W>(L.d, L.pc);
+        slice_pm<K>(L.d, L.pm);
+        uint32_t t[6];
+#pragma unroll
+        for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[W + b], tpub[b]);
+        bool active = true;
+        if constexpr (decltype(check)::value) active = s > lane && s - lane <= nrows;
+        if (active) {
+            uint32_t ne = 0;
+#pragma unroll
+            for (int b = 0; b < 5; ++b) ne |= t[b] ^ xb[b];
+            Slice<K> D;
+            if constexpr (FIRST) {
+                slice_cell<K>(U, L, G, ne, full, D);
+            } else {
+                slice_cell_masked<K>(U.d, U.pc, L.d, L.pm, G.d, G.pm, ne, D.d, D.pc);
+            }
+            if (report) {
+                const uint32_t hit = slice_le(D.pc, a.e) & t[5];
+                vcnt += __builtin_popcount(hit);
+                anyh |= hit;
+            }
+            G = L;
+            U = D;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) tpub[b] = t[b];
+            if (!last && lane == 63) slice_emit<K>(sout, s, D.d);
+        }
+    };
+    // fill (s < 64), steady state (64 <= s <= nrows: every lane has a row; two steps per
+    // trip, so the cell planes are not copied between registers on each back edge), drain
+    uint32_t s = 1;
+    const uint32_t fill_end = nrows < 63 ? nrows : 63;
+    for (; s <= fill_end; ++s) step(s, std::true_type{});
+    for (; s + 1 <= nrows; s += 2) {
+        step(s, std::false_type{});
+        step(s + 1, std::false_type{});
+    }
+    if (s <= nrows) {  // here s >= 64
+        step(s, std::false_type{});
+        ++s;
+    }
+    for (; s <= steps; ++s) step(s, std::true_type{});
+    if (report) {
+        if (anyh) atomicAdd(a.u + I.uv_base + j - 1, (uint32_t)__builtin_popcount(anyh));
+        if (vcnt) atomicAdd(a.v + I.uv_base + j - 1, vcnt);
+    }
+    __threadfence_block();  // the boundary stream is read by this wave's next strip
+    __syncthreads();
+}
 
 template <int K>
 __global__ __launch_bounds__(64) void ed_slice_kernel(EdSliceArgs a)
 {
-    typedef SliceLayout<K> Lay;
-    constexpr int W = Lay::W, SW = Lay::SW, SWS = Lay::SWS, LW = Lay::LW;
-    __shared__ uint32_t chunk[64 * LW];
+    __shared__ uint32_t chunk[64 * SliceLayout<K>::LW];
     const uint32_t lane = threadIdx.x;
     uint32_t *st0 = a.scratch + (size_t)blockIdx.x * 2 * a.stream_words;
     uint32_t *st1 = st0 + a.stream_words;
@@ -58,113 +177,15 @@ __global__ __launch_bounds__(64) void ed_slice_kernel(EdSliceArgs a)
         }
         if (it >= a.nitems) break;
         const EdSliceItem I = a.items[it];
-        const uint32_t m = I.m;
         const uint32_t g0 = a.goff[I.group];
         const uint32_t nrows = a.goff[I.group + 1] - g0;
         const uint32_t *gp = a.gplanes + (size_t)g0 * 6;
-        const uint32_t nstrips = (m + 63) / 64;
-        for (uint32_t strip = 0; strip < nstrips; ++strip) {
-            const uint32_t *sin = (strip & 1) ? st1 : st0;
-            uint32_t *sout = (strip & 1) ? st0 : st1;
-            const bool last = strip + 1 == nstrips;
-            const uint32_t j = strip * 64 + lane + 1;  // 1-based column
-            const uint32_t code = j <= m ? a.codes[I.x_start + j - 1] : 0u;
-            uint32_t xb[5];
-#pragma unroll
-            for (int b = 0; b < 5; ++b) xb[b] = ((code >> b) & 1) ? ~0u : 0u;
-            const uint32_t full = j <= (uint32_t)K ? ~0u : 0u;
-            const bool report = j <= m && j >= (uint32_t)K;
-            Slice<K> U, G;  // D[i-1][j] (also what lane L+1 reads), D[i-1][j-1]
-            slice_row0<K>(j, U);
-            slice_row0<K>(j - 1, G);
-            uint32_t tpub[6] = {0, 0, 0, 0, 0, 0};
-            uint32_t vcnt = 0, anyh = 0;
-            const uint32_t steps = nrows + 63;
-            // one wavefront step; CHECK: fill/drain form (lanes outside rows 1..nrows idle)
-            auto step = [&](uint32_t s, auto check) {
-                const uint32_t r0 = s - 1;  // 0-based row lane 0 starts this step
-                if ((r0 & 63) == 0) {
-                    __syncthreads();  // the previous chunk is no longer read
-                    for (uint32_t w = lane; w < 64u * LW; w += 64) {
-                        const uint32_t rr = w / LW, c = w % LW, row = r0 + rr;
-                        uint32_t val = 0;
-                        if (row < nrows) {
-                            if (c < (uint32_t)SW) val = strip ? sin[(size_t)row * SWS + c] : 0u;
-                            else if (c < (uint32_t)SW + 6) val = gp[(size_t)row * 6 + (c - SW)];
-                        }
-                        chunk[w] = val;
-                    }
-                    __syncthreads();
-                }
-                const uint32_t *cr = chunk + (r0 & 63) * LW;
-                Slice<K> L;  // D[i][j-1]
-#pragma unroll
-                for (int p = 0; p < W; ++p) L.d[p] = shr1_from(cr[p], U.d[p]);
-#pragma unroll
-                for (int q = 0; q < kSlicePc; ++q) {
-                    L.pc[q] = shr1_from(cr[W + q], U.pc[q]);
-                    L.pm[q] = shr1_from(cr[W + kSlicePc + q], U.pm[q]);
-                }
-                uint32_t t[6];
-#pragma unroll
-                for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[SW + b], tpub[b]);
-                bool active = true;
-                if constexpr (decltype(check)::value) active = s > lane && s - lane <= nrows;
-                if (active) {
-                    uint32_t ne = 0;
-#pragma unroll
-                    for (int b = 0; b < 5; ++b) ne |= t[b] ^ xb[b];
-                    Slice<K> D;
-                    slice_cell<K>(U, L, G, ne, full, D);
-                    if (report) {
-                        const uint32_t hit = slice_le(D.pc, a.e) & t[5];
-                        vcnt += __builtin_popcount(hit);
-                        anyh |= hit;
-                    }
-                    G = L;
-                    U = D;
-#pragma unroll
-                    for (int b = 0; b < 6; ++b) tpub[b] = t[b];
-                    if (!last && lane == 63) {
-                        // row s-63 (0-based s-64) of this strip's last column
-                        uint32_t w[SWS];
-#pragma unroll
-                        for (int p = 0; p < W; ++p) w[p] = D.d[p];
-#pragma unroll
-                        for (int q = 0; q < kSlicePc; ++q) {
-                            w[W + q] = D.pc[q];
-                            w[W + kSlicePc + q] = D.pm[q];
-                        }
-#pragma unroll
-                        for (int p = SW; p < SWS; ++p) w[p] = 0;
-                        uint4 *dst = reinterpret_cast<uint4 *>(sout + (size_t)(s - 64) * SWS);
-#pragma unroll
-                        for (int p = 0; p < SWS / 4; ++p)
-                            dst[p] = make_uint4(w[4 * p], w[4 * p + 1], w[4 * p + 2], w[4 * p + 3]);
-                    }
-                }
-            };
-            // fill (s < 64), steady state (64 <= s <= nrows: every lane has a row; two steps
-            // per trip, so the cell planes are not copied between registers on each back
-            // edge), drain (s > nrows)
-            uint32_t s = 1;
-            const uint32_t fill_end = nrows < 63 ? nrows : 63;
-            for (; s <= fill_end; ++s) step(s, std::true_type{});
-            for (; s + 1 <= nrows; s += 2) {
-                step(s, std::false_type{});
-                step(s + 1, std::false_type{});
-            }
-            if (s <= nrows) {  // here s >= 64
-                step(s, std::false_type{});
-                ++s;
-            }
-            for (; s <= steps; ++s) step(s, std::true_type{});
-            if (report) {
-                if (anyh) atomicAdd(a.u + I.uv_base + j - 1, (uint32_t)__builtin_popcount(anyh));
-                if (vcnt) atomicAdd(a.v + I.uv_base + j - 1, vcnt);
-            }
-            __threadfence_block();  // the boundary stream is read by this wave's next strip
-            __syncthreads();
+        const uint32_t nstrips = (I.m + 63) / 64;
+        slice_strip<K, true>(a, I, 0, nrows, gp, st1, st0, nstrips == 1, chunk);
+        for (uint32_t strip = 1; strip < nstrips; ++strip) {
+            const uint32_t *sin = (strip & 1) ? st0 : st1;
+            uint32_t *sout = (strip & 1) ? st1 : st0;
+            slice_strip<K, false>(a, I, strip, nrows, gp, sin, sout, strip + 1 == nstrips, chunk);
         }
     }
 }

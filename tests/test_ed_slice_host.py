@@ -2,7 +2,8 @@
 edit kernel runs for 32 texts at once) against the oracle's scalar per-pair edit operator
 (orc_ed_pair, functions.cc:734-768 with MaxShift functions.cc:437-447).  The check program
 tests/slice_check.cpp is built with g++ and run on random and low-complexity inputs; it also
-asserts that no selected word needs more than 2k bits."""
+asserts that no selected word needs more than 2k bits and that the truncated-column form
+(slice_cell_masked + slice_pm, used for columns j > k) gives the same cells."""
 import os
 import subprocess
 
@@ -45,3 +46,4 @@ def test_sliced_step_matches_oracle(checker, case):
     r = subprocess.run([checker, *map(str, case)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "mismatched_columns=0" in r.stdout and "wide_state=0" in r.stdout
+    assert "masked_form_differs=0" in r.stdout
