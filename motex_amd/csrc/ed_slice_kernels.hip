@@ -203,6 +203,10 @@ uint32_t ed_slice_stream_row_words(uint32_t k)
     case 8: return stream_row_words<8>();
     case 9: return stream_row_words<9>();
     case 10: return stream_row_words<10>();
+    case 11: return stream_row_words<11>();
+    case 12: return stream_row_words<12>();
+    case 13: return stream_row_words<13>();
+    case 14: return stream_row_words<14>();
     default: return 0;
     }
 }
@@ -217,6 +221,10 @@ hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, hipStream_t s)
     case 8: hipLaunchKernelGGL(ed_slice_kernel<8>, dim3(grid), dim3(64), 0, s, a); break;
     case 9: hipLaunchKernelGGL(ed_slice_kernel<9>, dim3(grid), dim3(64), 0, s, a); break;
     case 10: hipLaunchKernelGGL(ed_slice_kernel<10>, dim3(grid), dim3(64), 0, s, a); break;
+    case 11: hipLaunchKernelGGL(ed_slice_kernel<11>, dim3(grid), dim3(64), 0, s, a); break;
+    case 12: hipLaunchKernelGGL(ed_slice_kernel<12>, dim3(grid), dim3(64), 0, s, a); break;
+    case 13: hipLaunchKernelGGL(ed_slice_kernel<13>, dim3(grid), dim3(64), 0, s, a); break;
+    case 14: hipLaunchKernelGGL(ed_slice_kernel<14>, dim3(grid), dim3(64), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

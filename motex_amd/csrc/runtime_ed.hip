@@ -72,14 +72,22 @@ static uint32_t ed_grid(const motex_batch *b)
 // Text-parallel plan (ed_slice_kernels.hip): the texts that can hold an occurrence
 // (length >= k), longest first, in groups of 32; per group row, the 5 bit planes of the 32
 // texts' symbol codes and the mask of texts still running.  Items are (x, group), most
-// expensive first.  Used for k in 4..10 and alphabets of at most 32 symbols (5 planes);
+// expensive first.  Used for k in 4..14 and alphabets of at most 32 symbols (5 planes);
 // MOTEX_ED_SLICE=0 keeps the one-text-per-lane kernel.
 static bool ed_slice_ok(const motex_batch *b)
 {
     const char *env = getenv("MOTEX_ED_SLICE");
     if (env && env[0] == '0') return false;
-    return b->params.k >= 4 && b->params.k <= 10 && b->ss.sigma <= 32 &&
-           ed_slice_stream_row_words(b->params.k) != 0;
+    const uint32_t k = b->params.k;
+    if (k < 4 || k > 14 || b->ss.sigma > 32 || ed_slice_stream_row_words(k) == 0) return false;
+    if (env && env[0] == '1') return true;
+    // An item (x, group of 32 texts) is one wave's sequential sweep; with too few of them
+    // the device is not filled, and for k > 8 the one-text-per-lane kernel (many smaller
+    // items) is then faster (sweep over 200 vs 500 protein sequences of U[200,2000]).
+    uint64_t nx = 0;
+    for (uint32_t i = 0; i < b->ss.num; ++i) nx += b->ss.len[i] >= k;
+    const uint64_t items = nx * ((nx + 31) / 32);
+    return k <= 8 || items >= (uint64_t)b->ctx->num_cu * 6;
 }
 
 static int ed_prepare_slice(motex_batch *b)

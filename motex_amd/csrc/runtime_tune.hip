@@ -39,8 +39,8 @@ static TileShape default_tile(const motex_params &p, const SeqSet &ss)
         t.blocks = ss.sigma <= 8 ? 4 : (ss.sigma <= 24 ? 2 : 1);
         t.rows = 128;
         t.waves_per_cu = 32;
-    } else if (!p.nb_gaps && p.k >= 4 && p.k <= 10 && ss.sigma <= 32) {
-        // text-parallel edit kernel: ~150 VGPRs, so 3 waves per SIMD are resident
+    } else if (!p.nb_gaps && p.k >= 4 && p.k <= 14 && ss.sigma <= 32) {
+        // text-parallel edit kernel: 130-230 VGPRs, so 2-3 waves per SIMD are resident
         t.waves_per_cu = 12;
     } else {
         t.blocks = 4;

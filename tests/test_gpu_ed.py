@@ -15,7 +15,7 @@ def rand_seqs(rng, n, lo, hi, alpha):
 @pytest.fixture(params=["1", "0"], ids=["text_parallel", "one_text_per_lane"])
 def kernel(request, monkeypatch):
     """Runs a test on the text-parallel edit kernel (32 texts per lane word; used for k in
-    4..10) and again with it switched off."""
+    4..14) and again with it switched off."""
     monkeypatch.setenv("MOTEX_ED_SLICE", request.param)
     return request.param
 
@@ -28,7 +28,7 @@ def check(ctx, oracle, seqs, k, e, alphabet="DNA", kernel=None):
     u, v, off = b.download()
     b.close()
     if kernel is not None:
-        sliced = kernel == "1" and 4 <= k <= 10
+        sliced = kernel == "1" and 4 <= k <= 14
         assert (st["kernel_variant"] == 2) == sliced
     ou, ov, ooff, _ = oracle.all_pairs(seqs, 1, k, e)
     bad = np.nonzero((u != ou) | (v != ov))[0]
