@@ -252,18 +252,28 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
         for (uint32_t d = 0; d < a.ndist && listed; ++d)
             if (__builtin_amdgcn_readfirstlane(lcount[d]) > a.lcap) listed = false;
         if (listed) {
-            // walk box 0's listed occurrences, then clear the bitmap words of every list
+            // walk box 0's listed occurrences (one per lane, all its x-side combinations in
+            // turn, so a lane's bitmap reads for different combinations overlap), then clear
+            // the bitmap words of every list.  The (combination, pos_x) counted bits live in
+            // LDS (in the hit-list area, unused here) when they fit.
             const uint32_t d0 = a.dist_of[0];
             const uint32_t *l0 = lists + (size_t)d0 * a.lcap;
             const uint32_t c0 = __builtin_amdgcn_readfirstlane(lcount[d0]);
-            for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
-                for (uint32_t h = lane; h < c0; h += 64) {
-                    const uint32_t e = l0[h];
-                    const uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
+            const bool lds_counted = (uint64_t)a.nb_structs * m <= (uint64_t)a.hcap * 32;
+            uint32_t *cnt_bits = lds_counted ? hits : counted;
+            if (lds_counted) {
+                for (uint32_t w = lane; w < cwords; w += 64) hits[w] = 0;
+                __syncthreads();
+            }
+            for (uint32_t h = lane; h < c0; h += 64) {
+                const uint32_t e = l0[h];
+                const uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
+#pragma unroll 4
+                for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
                     const uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
                     if (c) {
                         const uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
-                        const uint32_t old = atomicOr(counted + (idx >> 5), bit);
+                        const uint32_t old = atomicOr(cnt_bits + (idx >> 5), bit);
                         const size_t o = (size_t)I.uv_base + idx;
                         if (!(old & bit)) atomicAdd(a.u + o, 1u);
                         atomicAdd(a.v + o, c);
