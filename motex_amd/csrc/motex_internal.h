@@ -114,7 +114,8 @@ struct StructItem {
     uint32_t x_start, m;    // x: text position and length
     uint32_t y_start, n;    // y: text position and length
     uint32_t uv_base;       // u/v index of (structure 0, pos_x 0) of x
-    uint32_t pad;
+    uint32_t uv_base_t;     // kNoSplit, or (pair-symmetric Hamming item) u/v index of
+                            // (structure 0, pos 0) of y: the pair (y, x) is credited too
 };
 
 struct StructArgs {

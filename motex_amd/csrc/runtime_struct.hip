@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "runtime.h"
@@ -52,20 +53,33 @@ struct StructPlan {
     std::vector<StructItem> items;
 };
 
+// Items: ordered pairs, or (sym) unordered pairs i <= j whose box DPs serve both (i, j) and
+// (j, i): Hamming boxes are symmetric, so the transposed bitmaps are those of (j, i).
 static void plan_pairs(const SeqSet &ss, uint32_t total_mot, const std::vector<std::pair<uint32_t, uint32_t>> *only,
-                       std::vector<StructItem> &out)
+                       std::vector<StructItem> &out, bool sym = false)
 {
     out.clear();
-    auto add = [&](uint32_t i, uint32_t j) {
+    auto add = [&](uint32_t i, uint32_t j, uint32_t uvt) {
         if (ss.len[i] < total_mot || ss.len[j] < total_mot) return;
-        out.push_back({ss.start[i], ss.len[i], ss.start[j], ss.len[j], (uint32_t)ss.uv_off[i], 0});
+        out.push_back({ss.start[i], ss.len[i], ss.start[j], ss.len[j], (uint32_t)ss.uv_off[i], uvt});
     };
     if (only) {
-        for (auto &pr : *only) add(pr.first, pr.second);
+        for (auto &pr : *only) add(pr.first, pr.second, kNoSplit);
+    } else if (sym) {
+        for (uint32_t i = 0; i < ss.num; ++i)
+            for (uint32_t j = i; j < ss.num; ++j) add(i, j, j == i ? kNoSplit : (uint32_t)ss.uv_off[j]);
     } else {
         for (uint32_t i = 0; i < ss.num; ++i)
-            for (uint32_t j = 0; j < ss.num; ++j) add(i, j);
+            for (uint32_t j = 0; j < ss.num; ++j) add(i, j, kNoSplit);
     }
+}
+
+// Pair-symmetric structured plan: Hamming boxes only; MOTEX_SYM=0 turns it off.
+static bool struct_symmetric(const motex_batch *b)
+{
+    const char *env = getenv("MOTEX_SYM");
+    if (env && env[0] == '0') return false;
+    return b->params.distance == 0;
 }
 
 int struct_prepare(motex_batch *b)
@@ -93,7 +107,8 @@ int struct_prepare(motex_batch *b)
             return hip_fail(e, "NE build");
     }
     std::vector<StructItem> items;
-    plan_pairs(ss, total_motif_length(&b->params), nullptr, items);
+    b->stats.symmetric = struct_symmetric(b) ? 1u : 0u;
+    plan_pairs(ss, total_motif_length(&b->params), nullptr, items, b->stats.symmetric != 0);
     std::stable_sort(items.begin(), items.end(), [](const StructItem &x, const StructItem &z) {
         return (uint64_t)x.m * x.n > (uint64_t)z.m * z.n;
     });
@@ -111,8 +126,10 @@ int struct_prepare(motex_batch *b)
     // cells: one DP per distinct box shape and pair
     double cells = 0;
     for (const StructItem &it : items)
-        for (uint32_t d = 0; d < a.ndist; ++d)
+        for (uint32_t d = 0; d < a.ndist; ++d) {
             if (it.m >= a.dlen[d]) cells += (double)(it.m - a.dlen[d] + 1) * it.n;
+            if (it.uv_base_t != kNoSplit && it.n >= a.dlen[d]) cells += (double)(it.n - a.dlen[d] + 1) * it.m;
+        }
     b->stats.cells = cells;
     if ((rc = struct_alloc_scratch(b))) return rc;
     return hipStreamSynchronize(bstream(b)) == hipSuccess ? MOTEX_OK : set_error(MOTEX_EDEVICE, "NE build");
@@ -140,7 +157,8 @@ static int struct_fill_args(motex_batch *b, StructArgs &a)
     a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
     a.stream_words = (maxlen + 128) * 2;
     a.lcap = b->params.distance == 0 ? kStructListCap : 0;
-    a.scratch_words = a.ndist * a.bm_words + 2ull * a.stream_words +
+    // bitmaps (Hamming: and their transposes), edit streams, counted bits, lists
+    a.scratch_words = (b->params.distance == 0 ? 2ull : 1ull) * a.ndist * a.bm_words + 2ull * a.stream_words +
                       ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1 + (uint64_t)a.ndist * a.lcap;
     return MOTEX_OK;
 }
@@ -170,7 +188,7 @@ static uint32_t struct_grid(motex_batch *b, uint32_t nitems)
     const SeqSet &ss = b->ss;
     uint32_t maxlen = 0;
     for (uint32_t i = 0; i < ss.num; ++i) maxlen = std::max(maxlen, ss.len[i]);
-    uint64_t per = ((uint64_t)(maxlen + 31) / 32 * maxlen * (b->params.nb_gaps + 1) + 2ull * (maxlen + 128) * 2 +
+    uint64_t per = ((uint64_t)(maxlen + 31) / 32 * maxlen * (b->params.nb_gaps + 1) * 2 + 2ull * (maxlen + 128) * 2 +
                     ((uint64_t)b->nb_structs * maxlen + 31) / 32 + 1 +
                     (uint64_t)(b->params.nb_gaps + 1) * kStructListCap) * 4;
     uint64_t cap = std::max<uint64_t>(1, (2ull << 30) / std::max<uint64_t>(per, 1));

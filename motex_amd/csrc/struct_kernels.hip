@@ -103,8 +103,8 @@ __device__ uint32_t struct_walk(const StructArgs &a, uint32_t *const bm[kMaxBoxe
 // Hamming hits of one box shape for the pair, by the diagonal sweep, into bm.
 template <int P>
 __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t len, uint32_t err,
-                              uint32_t *bm, uint32_t *ne_l, const uint16_t *xoff, uint32_t *list,
-                              uint32_t *lcount)
+                              uint32_t *bm, uint32_t *bmT, uint32_t *ne_l, const uint16_t *xoff,
+                              uint32_t *list, uint32_t *lcount)
 {
     const uint32_t lane = threadIdx.x;
     const uint32_t m = I.m, n = I.n, wl = a.wl;
@@ -153,6 +153,7 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
                     int64_t pt = qb + 32 * lane + t + px - (int64_t)I.y_start;
                     if (pt + 1 < (int64_t)len || pt >= (int64_t)n) continue;
                     atomicOr(bm + (size_t)((uint32_t)pt >> 5) * m + px, 1u << ((uint32_t)pt & 31));
+                    if (bmT) atomicOr(bmT + (size_t)(px >> 5) * n + (uint32_t)pt, 1u << (px & 31));
                     const uint32_t slot = atomicAdd(lcount, 1u);
                     if (slot < a.lcap) list[slot] = (px << 16) | (uint32_t)pt;
                 }
@@ -174,9 +175,11 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
     uint32_t *hcount = hits + a.hcap;
     uint32_t *lcount = hcount + 1;  // per box shape: occurrences listed this pair
     for (uint32_t w = lane; w < wl; w += 64) ne_l[a.sigma * wl + w] = 0;
-    // scratch: box bitmaps | two edit boundary streams | counted (kx, pos_x) bits
+    // scratch: box bitmaps | (Hamming) their transposes | two edit boundary streams |
+    //          counted (kx, pos_x) bits | (Hamming) occurrence lists
     uint32_t *scr = a.scratch + (size_t)blockIdx.x * a.scratch_words;
-    uint32_t *buf0 = scr + (size_t)a.ndist * a.bm_words;
+    uint32_t *scrT = scr + (size_t)a.ndist * a.bm_words;
+    uint32_t *buf0 = scrT + (a.distance == 0 ? (size_t)a.ndist * a.bm_words : 0);
     uint32_t *buf1 = buf0 + a.stream_words;
     uint32_t *counted = buf1 + a.stream_words;
     uint32_t *lists = counted + ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1;
@@ -195,171 +198,203 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             it = round == 0 ? blockIdx.x : a.nitems;
         }
         if (it >= a.nitems) break;
-        const StructItem I = a.items[it];
-        const uint32_t m = I.m, n = I.n;
-        const uint32_t wn = (n + 31) / 32;
-        uint32_t *bmd[kMaxBoxes];
-        for (uint32_t d = 0; d < a.ndist; ++d) bmd[d] = scr + (size_t)d * wn * m;
-        uint32_t *bm[kMaxBoxes];
-        for (uint32_t b = 0; b <= a.nb_gaps; ++b) bm[b] = bmd[a.dist_of[b]];
-        __syncthreads();
-        if (a.distance == 0) {
-            for (uint32_t r = lane; r < m; r += 64)
-                xoff[r] = (uint16_t)(a.codes[I.x_start + r] * wl);
-        }
-        if (a.distance == 0 && !clean) {
-            for (size_t w = lane; w < (size_t)a.ndist * a.bm_words; w += 64) scr[w] = 0;
-            clean = true;
-        }
-        if (lane < a.ndist) lcount[lane] = 0;
-        bool listed = a.distance == 0;  // every shape's occurrences are in its list
-        // 1. box bitmaps
-        for (uint32_t d = 0; d < a.ndist; ++d) {
-            const uint32_t len = a.dlen[d], err = a.derr[d];
-            uint32_t *out = bmd[d];
-            if (err >= m) {
-                // passes everywhere inside the pair (functions.cc:597-598, 651, 669)
-                for (size_t w = lane; w < (size_t)wn * m; w += 64) {
-                    uint32_t word = (uint32_t)(w / m);
-                    uint32_t rem = n - 32 * word;
-                    out[w] = rem >= 32 ? ~0u : ((1u << rem) - 1);
-                }
-                listed = false;
-                continue;
-            }
+        // One ordered pair I (and, when sym, the transposed pair from the same box DPs).
+        // Returns true when the transposed pair still has to be run on its own (its
+        // occurrences did not fit the lists, or a box passes everywhere).
+        auto pair = [&](const StructItem &I, bool sym) -> bool {
+            const uint32_t m = I.m, n = I.n;
+            const uint32_t wn = (n + 31) / 32;
+            uint32_t *bmd[kMaxBoxes];
+            for (uint32_t d = 0; d < a.ndist; ++d) bmd[d] = scr + (size_t)d * wn * m;
+            // transposed bitmaps of a pair-symmetric item: bmT[(pos_x >> 5) * n + pos_t]
+            const uint32_t wm = (m + 31) / 32;
+            uint32_t *bmTd[kMaxBoxes], *bmT[kMaxBoxes];
+            for (uint32_t d = 0; d < a.ndist; ++d) bmTd[d] = scrT + (size_t)d * wm * n;
+            for (uint32_t b = 0; b <= a.nb_gaps; ++b) bmT[b] = bmTd[a.dist_of[b]];
+            uint32_t *bm[kMaxBoxes];
+            for (uint32_t b = 0; b <= a.nb_gaps; ++b) bm[b] = bmd[a.dist_of[b]];
+            __syncthreads();
             if (a.distance == 0) {
-                __syncthreads();
-                struct_hd_box<P>(a, I, len, err, out, ne_l, xoff, lists + (size_t)d * a.lcap, lcount + d);
-            } else {
-                EdOut eo{out, m};
-                if (len <= 14)
-                    ed_pair<4, false, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
-                                            buf0, buf1, nullptr, nullptr, eo);
-                else
-                    ed_pair<2, true, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
-                                           buf0, buf1, nullptr, nullptr, eo);
+                for (uint32_t r = lane; r < m; r += 64)
+                    xoff[r] = (uint16_t)(a.codes[I.x_start + r] * wl);
             }
-        }
-        __threadfence_block();
-        __syncthreads();
-        // 2. merge: gather box-0 hits word row by word row, then count per x combination
-        const uint32_t cwords = (a.nb_structs * m + 31) / 32;
-        for (uint32_t w = lane; w < cwords; w += 64) counted[w] = 0;
-        __threadfence_block();
-        if (lane == 0) *hcount = 0;
-        __syncthreads();
-        const uint32_t *b0 = bm[0];
-        for (uint32_t d = 0; d < a.ndist && listed; ++d)
-            if (__builtin_amdgcn_readfirstlane(lcount[d]) > a.lcap) listed = false;
-        if (listed) {
-            // walk box 0's listed occurrences (one per lane, all its x-side combinations in
-            // turn, so a lane's bitmap reads for different combinations overlap), then clear
-            // the bitmap words of every list.  The (combination, pos_x) counted bits live in
-            // LDS (in the hit-list area, unused here) when they fit.
-            const uint32_t d0 = a.dist_of[0];
-            const uint32_t *l0 = lists + (size_t)d0 * a.lcap;
-            const uint32_t c0 = __builtin_amdgcn_readfirstlane(lcount[d0]);
-            const bool lds_counted = (uint64_t)a.nb_structs * m <= (uint64_t)a.hcap * 32;
-            uint32_t *cnt_bits = lds_counted ? hits : counted;
-            if (lds_counted) {
-                for (uint32_t w = lane; w < cwords; w += 64) hits[w] = 0;
-                __syncthreads();
+            if (a.distance == 0 && !clean) {
+                for (size_t w = lane; w < (size_t)2 * a.ndist * a.bm_words; w += 64) scr[w] = 0;
+                clean = true;
             }
-            for (uint32_t h = lane; h < c0; h += 64) {
-                const uint32_t e = l0[h];
-                const uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
-#pragma unroll 4
-                for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
-                    const uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
-                    if (c) {
-                        const uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
-                        const uint32_t old = atomicOr(cnt_bits + (idx >> 5), bit);
-                        const size_t o = (size_t)I.uv_base + idx;
-                        if (!(old & bit)) atomicAdd(a.u + o, 1u);
-                        atomicAdd(a.v + o, c);
-                    }
-                }
-            }
-            __threadfence_block();
-            __syncthreads();
+            if (lane < a.ndist) lcount[lane] = 0;
+            bool listed = a.distance == 0;  // every shape's occurrences are in its list
+            // 1. box bitmaps
             for (uint32_t d = 0; d < a.ndist; ++d) {
-                const uint32_t *ld = lists + (size_t)d * a.lcap;
-                const uint32_t cd = __builtin_amdgcn_readfirstlane(lcount[d]);
-                for (uint32_t h = lane; h < cd; h += 64) {
-                    const uint32_t e = ld[h];
-                    bmd[d][(size_t)((e & 0xFFFFu) >> 5) * m + (e >> 16)] = 0;
+                const uint32_t len = a.dlen[d], err = a.derr[d];
+                uint32_t *out = bmd[d];
+                if (err >= m) {
+                    // passes everywhere inside the pair (functions.cc:597-598, 651, 669)
+                    for (size_t w = lane; w < (size_t)wn * m; w += 64) {
+                        uint32_t word = (uint32_t)(w / m);
+                        uint32_t rem = n - 32 * word;
+                        out[w] = rem >= 32 ? ~0u : ((1u << rem) - 1);
+                    }
+                    listed = false;
+                    continue;
+                }
+                if (a.distance == 0) {
+                    __syncthreads();
+                    struct_hd_box<P>(a, I, len, err, out, sym ? bmTd[d] : nullptr, ne_l, xoff,
+                                     lists + (size_t)d * a.lcap, lcount + d);
+                } else {
+                    EdOut eo{out, m};
+                    if (len <= 14)
+                        ed_pair<4, false, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
+                                                buf0, buf1, nullptr, nullptr, eo);
+                    else
+                        ed_pair<2, true, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
+                                               buf0, buf1, nullptr, nullptr, eo);
                 }
             }
             __threadfence_block();
-            continue;
-        }
-        // the bitmaps are scanned; Hamming ones are cleared in full after the merge
-        clean = a.distance != 0;
-        // box-0 occurrences of the pair: count them first (every lane streams its share of
-        // the bitmap with no barrier in between); when they fit the LDS list, gather them in
-        // one more pass and walk them once.  Otherwise fall back to gathering chunk by
-        // chunk, walking whenever the next chunk could overflow the list.
-        uint32_t mine = 0;
-        for (uint32_t w = 0; w < wn; ++w)
-            for (uint32_t px = lane; px < m; px += 64) mine += __builtin_popcount(b0[(size_t)w * m + px]);
-        if (lane == 0) *hcount = 0;
-        __syncthreads();
-        if (mine) atomicAdd(hcount, mine);
-        __syncthreads();
-        const uint32_t total = __builtin_amdgcn_readfirstlane(*hcount);
-        __syncthreads();
-        if (lane == 0) *hcount = 0;
-        __syncthreads();
-        auto walk_list = [&](uint32_t have) {
-            for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
-                for (uint32_t h = lane; h < have; h += 64) {
-                    uint32_t e = hits[h];
-                    uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
-                    uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
-                    if (c) {
-                        uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
-                        uint32_t old = atomicOr(counted + (idx >> 5), bit);
-                        size_t o = (size_t)I.uv_base + idx;
-                        if (!(old & bit)) atomicAdd(a.u + o, 1u);
-                        atomicAdd(a.v + o, c);
+            __syncthreads();
+            // 2. merge: gather box-0 hits word row by word row, then count per x combination
+            const uint32_t cwords = (a.nb_structs * m + 31) / 32;
+            for (uint32_t w = lane; w < cwords; w += 64) counted[w] = 0;
+            __threadfence_block();
+            if (lane == 0) *hcount = 0;
+            __syncthreads();
+            const uint32_t *b0 = bm[0];
+            for (uint32_t d = 0; d < a.ndist && listed; ++d)
+                if (__builtin_amdgcn_readfirstlane(lcount[d]) > a.lcap) listed = false;
+            if (listed) {
+                // walk box 0's listed occurrences (one per lane, all its x-side combinations in
+                // turn, so a lane's bitmap reads for different combinations overlap); for a
+                // pair-symmetric item walk them again with the roles of x and y swapped, over the
+                // transposed bitmaps; then clear the bitmap words of every list.  The
+                // (combination, pos_x) counted bits live in LDS (in the hit-list area, unused
+                // here) when they fit.
+                const uint32_t d0 = a.dist_of[0];
+                const uint32_t *l0 = lists + (size_t)d0 * a.lcap;
+                const uint32_t c0 = __builtin_amdgcn_readfirstlane(lcount[d0]);
+                // a box that passes everywhere in (y, x) (errors >= n) has no bitmap here
+                bool sym2 = sym;
+                for (uint32_t d = 0; d < a.ndist; ++d) sym2 = sym2 && a.derr[d] < n;
+                for (int orient = 0; orient < (sym2 ? 2 : 1); ++orient) {
+                    const uint32_t mo = orient ? n : m, no = orient ? m : n;
+                    uint32_t *const *bmo = orient ? bmT : bm;
+                    const uint32_t uvb = orient ? I.uv_base_t : I.uv_base;
+                    const uint32_t cw = (a.nb_structs * mo + 31) / 32;
+                    const bool lds_counted = (uint64_t)a.nb_structs * mo <= (uint64_t)a.hcap * 32;
+                    uint32_t *cnt_bits = lds_counted ? hits : counted;
+                    __syncthreads();
+                    for (uint32_t w = lane; w < cw; w += 64) cnt_bits[w] = 0;
+                    __threadfence_block();
+                    __syncthreads();
+                    for (uint32_t h = lane; h < c0; h += 64) {
+                        const uint32_t e = l0[h];
+                        const uint32_t px0 = orient ? (e & 0xFFFFu) : (e >> 16);
+                        const uint32_t pt0 = orient ? (e >> 16) : (e & 0xFFFFu);
+    #pragma unroll 4
+                        for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
+                            const uint32_t c = struct_walk(a, bmo, mo, no, kx, px0, pt0);
+                            if (c) {
+                                const uint32_t idx = kx * mo + px0, bit = 1u << (idx & 31);
+                                const uint32_t old = atomicOr(cnt_bits + (idx >> 5), bit);
+                                const size_t o = (size_t)uvb + idx;
+                                if (!(old & bit)) atomicAdd(a.u + o, 1u);
+                                atomicAdd(a.v + o, c);
+                            }
+                        }
                     }
                 }
-            }
-        };
-        auto append = [&](uint32_t w, uint32_t px) {
-            uint32_t word = px < m ? b0[(size_t)w * m + px] : 0u;
-            if (word) {
-                uint32_t off = atomicAdd(hcount, (uint32_t)__builtin_popcount(word));
-                while (word) {
-                    uint32_t t = __builtin_ctz(word);
-                    word &= word - 1;
-                    hits[off++] = (px << 16) | (32 * w + t);
+                __threadfence_block();
+                __syncthreads();
+                for (uint32_t d = 0; d < a.ndist; ++d) {
+                    const uint32_t *ld = lists + (size_t)d * a.lcap;
+                    const uint32_t cd = __builtin_amdgcn_readfirstlane(lcount[d]);
+                    for (uint32_t h = lane; h < cd; h += 64) {
+                        const uint32_t e = ld[h];
+                        const uint32_t px = e >> 16, pt = e & 0xFFFFu;
+                        bmd[d][(size_t)(pt >> 5) * m + px] = 0;
+                        if (sym) bmTd[d][(size_t)(px >> 5) * n + pt] = 0;
+                    }
                 }
+                __threadfence_block();
+                return sym && !sym2;
             }
-        };
-        if (total <= a.hcap) {
-            if (total) {
-                for (uint32_t w = 0; w < wn; ++w)
-                    for (uint32_t px0 = 0; px0 < m; px0 += 64) append(w, px0 + lane);
-                __syncthreads();
-                walk_list(total);
-            }
-            continue;
-        }
-        const uint32_t chunks = (m + 63) / 64;
-        for (uint32_t wc = 0; wc <= wn * chunks; ++wc) {
-            // walk before a chunk that could overflow the list, and at the end
-            uint32_t have = __builtin_amdgcn_readfirstlane(*hcount);
-            bool flush = (wc == wn * chunks) ? have > 0 : have + 64 * 32 > a.hcap;
-            if (flush) {
-                walk_list(have);
-                __syncthreads();
-                if (lane == 0) *hcount = 0;
-                __syncthreads();
-            }
-            if (wc == wn * chunks) break;
-            append(wc / chunks, (wc % chunks) * 64 + lane);
+            // the bitmaps are scanned; Hamming ones are cleared in full after the merge
+            clean = a.distance != 0;
+            // box-0 occurrences of the pair: count them first (every lane streams its share of
+            // the bitmap with no barrier in between); when they fit the LDS list, gather them in
+            // one more pass and walk them once.  Otherwise fall back to gathering chunk by
+            // chunk, walking whenever the next chunk could overflow the list.
+            uint32_t mine = 0;
+            for (uint32_t w = 0; w < wn; ++w)
+                for (uint32_t px = lane; px < m; px += 64) mine += __builtin_popcount(b0[(size_t)w * m + px]);
+            if (lane == 0) *hcount = 0;
             __syncthreads();
+            if (mine) atomicAdd(hcount, mine);
+            __syncthreads();
+            const uint32_t total = __builtin_amdgcn_readfirstlane(*hcount);
+            __syncthreads();
+            if (lane == 0) *hcount = 0;
+            __syncthreads();
+            auto walk_list = [&](uint32_t have) {
+                for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
+                    for (uint32_t h = lane; h < have; h += 64) {
+                        uint32_t e = hits[h];
+                        uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
+                        uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
+                        if (c) {
+                            uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
+                            uint32_t old = atomicOr(counted + (idx >> 5), bit);
+                            size_t o = (size_t)I.uv_base + idx;
+                            if (!(old & bit)) atomicAdd(a.u + o, 1u);
+                            atomicAdd(a.v + o, c);
+                        }
+                    }
+                }
+            };
+            auto append = [&](uint32_t w, uint32_t px) {
+                uint32_t word = px < m ? b0[(size_t)w * m + px] : 0u;
+                if (word) {
+                    uint32_t off = atomicAdd(hcount, (uint32_t)__builtin_popcount(word));
+                    while (word) {
+                        uint32_t t = __builtin_ctz(word);
+                        word &= word - 1;
+                        hits[off++] = (px << 16) | (32 * w + t);
+                    }
+                }
+            };
+            if (total <= a.hcap) {
+                if (total) {
+                    for (uint32_t w = 0; w < wn; ++w)
+                        for (uint32_t px0 = 0; px0 < m; px0 += 64) append(w, px0 + lane);
+                    __syncthreads();
+                    walk_list(total);
+                }
+                return sym;
+            }
+            const uint32_t chunks = (m + 63) / 64;
+            for (uint32_t wc = 0; wc <= wn * chunks; ++wc) {
+                // walk before a chunk that could overflow the list, and at the end
+                uint32_t have = __builtin_amdgcn_readfirstlane(*hcount);
+                bool flush = (wc == wn * chunks) ? have > 0 : have + 64 * 32 > a.hcap;
+                if (flush) {
+                    walk_list(have);
+                    __syncthreads();
+                    if (lane == 0) *hcount = 0;
+                    __syncthreads();
+                }
+                if (wc == wn * chunks) break;
+                append(wc / chunks, (wc % chunks) * 64 + lane);
+                __syncthreads();
+            }
+            return sym;
+        };
+        const StructItem I = a.items[it];
+        const bool sym = I.uv_base_t != kNoSplit;
+        if (pair(I, sym)) {
+            clean = false;  // the transposed bitmaps were set: start the swapped run clean
+            const StructItem I2{I.y_start, I.n, I.x_start, I.m, I.uv_base_t, kNoSplit};
+            pair(I2, false);
         }
     }
 }

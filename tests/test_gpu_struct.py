@@ -12,28 +12,41 @@ def rand_seqs(rng, n, lo, hi, alpha):
     return [a[rng.integers(0, len(a), rng.integers(lo, hi + 1))].tobytes() for _ in range(n)]
 
 
-def check(ctx, oracle, seqs, k, e, boxes, distance=0):
+@pytest.fixture(params=["1", "0"], ids=["symmetric", "ordered"])
+def sym(request, monkeypatch):
+    """Runs a test with the pair-symmetric structured plan (Hamming boxes: each unordered
+    pair's box DPs serve both orders) and with it switched off."""
+    monkeypatch.setenv("MOTEX_SYM", request.param)
+    return request.param == "1"
+
+
+def check(ctx, oracle, seqs, k, e, boxes, distance=0, sym=None):
     print(f"[struct parity] n={len(seqs)} total={sum(map(len, seqs))} k={k} e={e} d={distance} "
-          f"boxes={boxes}", flush=True)
+          f"boxes={boxes} sym={sym}", flush=True)
     p = MotexParams(k=k, e=e, distance=distance, boxes=boxes)
-    u, v, off = ctx.extract_all_pairs(seqs, p)
+    b = ctx.batch(seqs, p)
+    st = b.run()
+    u, v, off = b.download()
+    b.close()
+    if sym is not None:
+        assert st["symmetric"] == int(sym and distance == 0)
     ou, ov, ooff, _ = oracle.all_pairs(seqs, distance, k, e, boxes=boxes)
     bad = np.nonzero((u != ou) | (v != ov))[0]
     assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: gpu {u[bad[:5]]},{v[bad[:5]]} oracle {ou[bad[:5]]},{ov[bad[:5]]}"
     return u, v
 
 
-def test_config4a_subset(ctx, oracle):
+def test_config4a_subset(ctx, oracle, sym):
     w = synth.config4a(scale=0.02)  # 20 sequences of 600 nt, one planted structured motif
-    u, v = check(ctx, oracle, w.seqs, w.k, w.e, list(w.boxes))
+    u, v = check(ctx, oracle, w.seqs, w.k, w.e, list(w.boxes), sym=sym)
     assert v.sum() > 0
 
 
-def test_reference_boxes_example(ctx, oracle):
+def test_reference_boxes_example(ctx, oracle, sym):
     # the two-gap structure of data/boxes.txt: gaps [16,17] and [15,16], boxes 6/1 and 7/2
     rng = np.random.default_rng(3)
     seqs = rand_seqs(rng, 8, 40, 300, b"AC")
-    check(ctx, oracle, seqs, 6, 1, [(16, 17, 6, 1), (15, 16, 7, 2)])
+    check(ctx, oracle, seqs, 6, 1, [(16, 17, 6, 1), (15, 16, 7, 2)], sym=sym)
 
 
 @pytest.mark.parametrize("seed,k,e,boxes,alpha", [
@@ -41,10 +54,13 @@ def test_reference_boxes_example(ctx, oracle):
     (2, 5, 1, [(2, 4, 5, 1), (1, 3, 4, 1), (0, 2, 3, 0)], b"AC"),  # three gaps
     (3, 6, 2, [(5, 40, 6, 2)], b"ACGT"),                     # wide gap range
     (4, 3, 1, [(1, 2, 2, 500)], b"ACGT"),                    # box error >= m: passes everywhere
+    (8, 4, 1, [(1, 3, 4, 60)], b"ACGT"),                     # passes everywhere in some pairs only
+    (9, 3, 0, [(0, 1, 3, 0)], b"A"),                         # one symbol: lists overflow, bitmap scan
 ])
-def test_hamming_shapes(ctx, oracle, seed, k, e, boxes, alpha):
+def test_hamming_shapes(ctx, oracle, sym, seed, k, e, boxes, alpha):
     rng = np.random.default_rng(seed)
-    check(ctx, oracle, rand_seqs(rng, 7, 10, 200, alpha), k, e, boxes)
+    lo, hi = (40, 120) if seed == 8 else ((60, 400) if seed == 9 else (10, 200))
+    check(ctx, oracle, rand_seqs(rng, 7, lo, hi, alpha), k, e, boxes, sym=sym)
 
 
 @pytest.mark.parametrize("seed,k,e,boxes", [
