@@ -6,18 +6,6 @@
 
 namespace motex {
 
-// Index of the sequence containing text position pt among seq_start[j0 .. j0+ny-1].
-__device__ __forceinline__ uint32_t find_seq(const uint32_t *__restrict__ seq_start,
-                                             uint32_t j0, uint32_t ny, uint32_t pt)
-{
-    uint32_t lo = j0, hi = j0 + ny - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (seq_start[mid] <= pt) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
 // "count <= E" over P bit-sliced planes, least significant plane first.
 template <int P, int E>
 __device__ __forceinline__ uint32_t le_const(const uint32_t *c, uint32_t e)

@@ -87,16 +87,6 @@ struct HdPlan {
     uint32_t blocks = 4, rows = 512, rows_max = 0, wl = 0, W = 0, split_slots = 0;
 };
 
-struct EdItem {
-    uint32_t x;          // sequence index of x
-    uint32_t j0, ny;     // texts j0 .. j0+ny-1
-};
-
-struct EdPlan {
-    std::vector<EdItem> items;
-    uint32_t strips_max = 0;
-};
-
 struct Job;
 
 }  // namespace motex
@@ -130,7 +120,6 @@ struct motex_batch {
     motex::SeqSet ss;
     motex::TileShape tile;
     motex::HdPlan hd;
-    motex::EdPlan ed;
     uint32_t split_words = 0;
     uint32_t ed_nitems = 0;
     uint32_t grid_cap = 0;           // serving: bound on the persistent grid (0 = none)

@@ -168,8 +168,6 @@ int ed_prepare(motex_batch *b)
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t z) { return cost[a] > cost[z]; });
     std::vector<EdItemDev> sorted(items.size());
     for (size_t t = 0; t < items.size(); ++t) sorted[t] = items[order[t]];
-    b->ed.items.resize(sorted.size());
-    for (size_t t = 0; t < sorted.size(); ++t) b->ed.items[t] = {sorted[t].x_start, sorted[t].j0, sorted[t].ny};
     b->tile = choose_tile(b->ctx, b->params, ss);
     b->tile.variant = wide ? 1 : 0;
     hipError_t e;
