@@ -1,24 +1,91 @@
 // Device-side helpers of the bit-sliced Hamming sweep (see hd_kernels.hip).  Shared by the
 // Hamming and structured kernels.
+//
+// A window's mismatch count lives in P bit-sliced planes (plane p = bit p of the 32 counts of
+// a word), least significant plane first.  The counts are stored BIASED: a counter starts at
+//     bias = 2^PH - 1 - e,    PH = bits(e) (0 for e = 0),
+// so that "count <= e" is "biased count < 2^PH", i.e. planes PH..P-1 all zero: the threshold
+// test is an OR of the high planes instead of a bit-serial compare.  The counter must hold
+// bias + k without wrapping (P = bits(bias + k)).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace motex {
 
-// "count <= E" over P bit-sliced planes, least significant plane first.
-template <int P, int E>
-__device__ __forceinline__ uint32_t le_const(const uint32_t *c, uint32_t e)
+// Planes below the threshold plane for error bound e.
+__host__ __device__ inline uint32_t hd_low_planes(uint32_t e)
 {
-    uint32_t res = ~0u;
+    return e ? 32u - (uint32_t)__builtin_clz(e) : 0u;
+}
+
+__host__ __device__ inline uint32_t hd_bias(uint32_t e)
+{
+    return (1u << hd_low_planes(e)) - 1u - e;
+}
+
+// Planes a biased window counter of length k needs.
+__host__ __device__ inline uint32_t hd_counter_planes(uint32_t k, uint32_t e)
+{
+    return 32u - (uint32_t)__builtin_clz(hd_bias(e) + k);
+}
+
+// Counter planes holding `bias` in every lane.
+template <int P>
+__device__ __forceinline__ void planes_set(uint32_t *c, uint32_t bias)
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) c[p] = ((bias >> p) & 1u) ? ~0u : 0u;
+}
+
+// +1 in the lanes of t.
+template <int P>
+__device__ __forceinline__ void planes_inc(uint32_t *c, uint32_t t)
+{
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        uint32_t eb;
-        if constexpr (E >= 0) eb = ((E >> p) & 1) ? ~0u : 0u;
-        else eb = ((e >> p) & 1) ? ~0u : 0u;
-        res = (~c[p] & (res | eb)) | (res & eb);
+        const uint32_t cp = c[p];
+        c[p] = cp ^ t;
+        if (p + 1 < P) t &= cp;
     }
-    return res;
+}
+
+// Slides the window: +1 where a mismatch enters and none leaves, -1 where one leaves and none
+// enters.  The lanes that change are t = enter ^ leave, and in those lanes the direction is
+// the entering bit itself, so one carry chain serves both: an increment carries out of a
+// plane that was 1, a decrement borrows out of a plane that was 0, i.e. the chain continues
+// where the old plane equals the direction bit.
+template <int P>
+__device__ __forceinline__ void planes_slide(uint32_t *c, uint32_t enter, uint32_t leave)
+{
+    uint32_t t = enter ^ leave;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const uint32_t cp = c[p];
+        c[p] = cp ^ t;
+        if (p + 1 < P) t &= ~(cp ^ enter);
+    }
+}
+
+// OR of planes PH..P-1: bit clear <=> the window's count is within the error bound.
+template <int P, int PH>
+__device__ __forceinline__ uint32_t planes_high(const uint32_t *c)
+{
+    static_assert(PH < P, "the counter needs a plane above the threshold");
+    uint32_t h = c[PH];
+#pragma unroll
+    for (int p = PH + 1; p < P; ++p) h |= c[p];
+    return h;
+}
+
+// planes_high with the threshold plane ph chosen at run time (wave-uniform).
+template <int P>
+__device__ __forceinline__ uint32_t planes_high_rt(const uint32_t *c, uint32_t ph)
+{
+    uint32_t h = 0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) h |= (uint32_t)p >= ph ? c[p] : 0u;
+    return h;
 }
 
 }  // namespace motex

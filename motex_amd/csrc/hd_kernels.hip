@@ -9,11 +9,13 @@
 // Layout.  All sequences are concatenated into one text; for each symbol c a bit table
 // NE_c holds bit p = (text[p] != c).  A wave owns 64 lanes x 32 diagonals per block
 // (q = pos_t - pos_x) and sweeps the rows pos_x of x.  For its 32 diagonals a lane keeps
-// the mismatch count of the current k-window as P bit-sliced planes; one row step adds the
+// the mismatch count of the current k-window as P bit-sliced planes, biased so that the
+// count is within e exactly when planes PH..P-1 are zero (hd_device.h); one row step adds the
 // entering mismatch bits (NE_{x[px]} read at diagonal offset px) and subtracts the leaving
-// ones (NE_{x[px-k]} at offset px-k).  Both are funnel-shifted reads of two LDS words, so
-// no counter ever moves between lanes.  A bit-sliced "count <= e" gives 32 hit flags per
-// lane per row; hits are rare, so a wave-uniform slow path folds them into u and v
+// ones (NE_{x[px-k]} at offset px-k) with one up/down carry chain.  Both are funnel-shifted
+// reads of two LDS words, so no counter ever moves between lanes.  The clear bits of the
+// high planes' OR are the row's 32 hit flags per lane; hits are rare, so a wave-uniform slow
+// path folds them into u and v
 // (distinct texts per row for u, a global flag word per factor for texts split across
 // work items).
 #include <hip/hip_runtime.h>
@@ -216,8 +218,19 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
 // sequences of that range with an occurrence) is added to the row's u; a range that is
 // part of a long sequence marks u once.  The item's rows are flushed to global u/v at the
 // end.  The first k rows of an item have no leaving column, so they run a step without it.
-template <int B, int P, int E>
-__global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
+// Shapes whose registers fit 8 waves per SIMD (64 VGPRs) with at most a spill outside the
+// row loop; the others are left to the compiler's allocation.
+template <int B, int P, int PH>
+constexpr int hd_waves_per_eu()
+{
+    if constexpr (B == 4) return (P == 3 || (P == 4 && PH <= 2)) ? 8 : 1;
+    else if constexpr (B == 2) return P + PH <= 11 ? 8 : 1;
+    else return 8;
+}
+
+template <int B, int P, int PH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hd_waves_per_eu<B, P, PH>())))
+void hd_batched_kernel(HdArgs a)
 {
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x;
@@ -256,6 +269,7 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
         }
         const uint32_t rep0 = I.r0 > a.k - 1 ? I.r0 : a.k - 1;
         const uint32_t pxs5 = I.px_start >> 5;
+        const uint32_t bias = hd_bias(a.e);
         for (uint32_t ri = I.rb; ri < I.re; ++ri) {
             const HdRange R = a.ranges[ri];
             // first diagonal (pt - px) of this range, a multiple of 32
@@ -272,9 +286,7 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
             uint32_t nq = 0;  // queued (row, lane) entries, wave-uniform
             uint32_t cnt[B][P];
 #pragma unroll
-            for (int b = 0; b < B; ++b)
-#pragma unroll
-                for (int p = 0; p < P; ++p) cnt[b][p] = 0;
+            for (int b = 0; b < B; ++b) planes_set<P>(cnt[b], bias);
 
             // one row: window counters of row r (with the column leaving at r - k when
             // LEAVE), then the row's hit flags into the queue
@@ -287,35 +299,27 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
                     wo = xoff[r - a.k] + ((px - a.k) >> 5) - pxs5 + lane;
                     sho = (px - a.k) & 31;
                 }
-                uint32_t res[B];
-                uint32_t any = 0;
+                uint32_t high[B];
+                uint32_t all = ~0u;  // lanes' words with no hit in any block
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
                     const uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 64 * b + 1], ne_l[wn + 64 * b], shn);
-                    uint32_t inc, dec;
                     if constexpr (decltype(leave)::value) {
                         const uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 64 * b + 1], ne_l[wo + 64 * b], sho);
-                        inc = an & ~bo;
-                        dec = bo & ~an;
+                        planes_slide<P>(cnt[b], an, bo);
                     } else {
-                        inc = an;
-                        dec = 0;
+                        planes_inc<P>(cnt[b], an);
                     }
-#pragma unroll
-                    for (int p = 0; p < P; ++p) {
-                        const uint32_t cp = cnt[b][p];
-                        cnt[b][p] = cp ^ (inc | dec);
-                        const uint32_t ni = cp & inc, nd = ~cp & dec;
-                        inc = ni;
-                        dec = nd;
-                    }
-                    res[b] = le_const<P, E>(cnt[b], a.e);
-                    any |= res[b];
+                    high[b] = planes_high<P, PH>(cnt[b]);
+                    all &= high[b];
                 }
                 if (px >= rep0) {
-                    const uint64_t m = __ballot(any != 0);
+                    const uint64_t m = __ballot(all != ~0u);
                     if (m) {
-                        if (any) {
+                        if (all != ~0u) {
+                            uint32_t res[B];
+#pragma unroll
+                            for (int b = 0; b < B; ++b) res[b] = ~high[b];
                             const uint32_t slot =
                                 nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -385,34 +389,44 @@ __global__ __launch_bounds__(64) void hd_batched_kernel(HdArgs a)
 // Host-side launchers
 // ------------------------------------------------------------------------------------
 
-template <int B, int P, int E>
+template <int B, int P, int PH>
 static hipError_t launch_bp(const HdArgs &a, uint32_t grid, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((hd_batched_kernel<B, P, E>), dim3(grid), dim3(64), lds, s, a);
-    return hipGetLastError();
+    if constexpr (PH < P) {
+        hipLaunchKernelGGL((hd_batched_kernel<B, P, PH>), dim3(grid), dim3(64), lds, s, a);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
 }
 
 template <int B, int P>
-static hipError_t launch_e(const HdArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+static hipError_t launch_ph(const HdArgs &a, uint32_t grid, size_t lds, hipStream_t s)
 {
-    switch (a.e) {
+    switch (hd_low_planes(a.e)) {
     case 0: return launch_bp<B, P, 0>(a, grid, lds, s);
     case 1: return launch_bp<B, P, 1>(a, grid, lds, s);
     case 2: return launch_bp<B, P, 2>(a, grid, lds, s);
     case 3: return launch_bp<B, P, 3>(a, grid, lds, s);
-    default: return launch_bp<B, P, -1>(a, grid, lds, s);
+    case 4: return launch_bp<B, P, 4>(a, grid, lds, s);
+    case 5: return launch_bp<B, P, 5>(a, grid, lds, s);
+    case 6: return launch_bp<B, P, 6>(a, grid, lds, s);
+    default: return hipErrorInvalidValue;
     }
 }
 
 template <int B>
 static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream_t s)
 {
-    uint32_t planes = 32 - __builtin_clz(a.k);  // bits to hold 0..k
-    if (planes <= 3) return launch_e<B, 3>(a, grid, lds, s);
-    if (planes == 4) return launch_e<B, 4>(a, grid, lds, s);
-    if (planes == 5) return launch_e<B, 5>(a, grid, lds, s);
-    if (planes == 6) return launch_e<B, 6>(a, grid, lds, s);
-    return launch_e<B, 7>(a, grid, lds, s);
+    // the biased counter holds bias(e) + k: 2k at most, so 7 planes for k <= 64
+    switch (hd_counter_planes(a.k, a.e)) {
+    case 0: case 1: case 2: case 3: return launch_ph<B, 3>(a, grid, lds, s);
+    case 4: return launch_ph<B, 4>(a, grid, lds, s);
+    case 5: return launch_ph<B, 5>(a, grid, lds, s);
+    case 6: return launch_ph<B, 6>(a, grid, lds, s);
+    case 7: return launch_ph<B, 7>(a, grid, lds, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max)

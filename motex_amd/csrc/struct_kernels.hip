@@ -109,6 +109,7 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
     const uint32_t lane = threadIdx.x;
     const uint32_t m = I.m, n = I.n, wl = a.wl;
     const uint32_t zero_row = a.sigma * wl;
+    const uint32_t ph = hd_low_planes(err);  // window counts are biased (hd_device.h)
     int64_t q0 = (int64_t)I.y_start - (int64_t)(m - 1);
     q0 = q0 >= 0 ? (q0 & ~31LL) : -(((-q0) + 31) & ~31LL);
     const int64_t qend = (int64_t)I.y_start + n;  // diagonals below this can hold hits
@@ -121,8 +122,7 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
         }
         __syncthreads();
         uint32_t cnt[P];
-#pragma unroll
-        for (int p = 0; p < P; ++p) cnt[p] = 0;
+        planes_set<P>(cnt, hd_bias(err));
         for (uint32_t px = 0; px < m; ++px) {
             const uint32_t wn = xoff[px] + (px >> 5) + lane;
             uint32_t wo, sho;
@@ -135,16 +135,8 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
             }
             uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 1], ne_l[wn], px & 31);
             uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 1], ne_l[wo], sho);
-            uint32_t inc = an & ~bo, dec = bo & ~an;
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                uint32_t cp = cnt[p];
-                cnt[p] = cp ^ (inc | dec);
-                uint32_t ni = cp & inc, nd = ~cp & dec;
-                inc = ni;
-                dec = nd;
-            }
-            uint32_t res = le_const<P, -1>(cnt, err);
+            planes_slide<P>(cnt, an, bo);
+            const uint32_t res = ~planes_high_rt<P>(cnt, ph);
             if (px + 1 >= len && __ballot(res != 0)) {
                 uint32_t h = res;
                 while (h) {
@@ -401,9 +393,9 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
 
 hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStream_t s)
 {
-    uint32_t planes = 1;
+    uint32_t planes = 1;  // biased Hamming window counters of every shape (hd_device.h)
     for (uint32_t d = 0; d < a.ndist; ++d) {
-        uint32_t p = 32 - __builtin_clz(a.dlen[d]);
+        uint32_t p = hd_counter_planes(a.dlen[d], a.derr[d]);
         planes = p > planes ? p : planes;
     }
     if (planes <= 3) hipLaunchKernelGGL(struct_kernel<3>, dim3(grid), dim3(64), lds, s, a);
