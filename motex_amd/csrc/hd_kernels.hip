@@ -218,12 +218,14 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
 // sequences of that range with an occurrence) is added to the row's u; a range that is
 // part of a long sequence marks u once.  The item's rows are flushed to global u/v at the
 // end.  The first k rows of an item have no leaving column, so they run a step without it.
-// Shapes whose registers fit 8 waves per SIMD (64 VGPRs) with at most a spill outside the
-// row loop; the others are left to the compiler's allocation.
+// Register budget: the LDS window of a B = 4 wave (about 6 KB for DNA) already limits a CU
+// to 26 waves, so those shapes may use up to 80 VGPRs, which lets the compiler keep a row's
+// 16 window words in flight at once (measured faster than 64 or 72); the smaller windows of
+// B = 1, 2 fit 8 waves per SIMD (64 VGPRs) where the counters allow.
 template <int B, int P, int PH>
 constexpr int hd_waves_per_eu()
 {
-    if constexpr (B == 4) return (P == 3 || (P == 4 && PH <= 2)) ? 8 : 1;
+    if constexpr (B == 4) return P <= 4 ? 6 : 1;
     else if constexpr (B == 2) return P + PH <= 11 ? 8 : 1;
     else return 8;
 }
@@ -237,13 +239,14 @@ void hd_batched_kernel(HdArgs a)
     const uint32_t wl = a.wl;
     // LDS: NE window (sigma rows; at offset 0 so that the block offsets fit the read
     //      instructions' immediates) | per-row v, sequence mask | range sequence starts |
-    //      per-row u (16 bit) | x offsets (16 bit)
+    //      per-row NE byte offsets (2 spare entries for the prefetch) | per-row u (16 bit)
     uint32_t *ne_l = smem;
     uint32_t *acc_v = ne_l + a.sigma * wl;
     uint32_t *ymask = acc_v + a.rows_max;
     uint32_t *starts = ymask + a.rows_max;
-    uint16_t *acc_u = reinterpret_cast<uint16_t *>(starts + kMaxRangeSeqs + 1);
-    uint16_t *xoff = acc_u + a.rows_max;
+    uint32_t *xb = starts + kMaxRangeSeqs + 1;
+    uint16_t *acc_u = reinterpret_cast<uint16_t *>(xb + a.rows_max + 2);
+    const uint32_t lane4 = lane * 4;
     // this wave's hit queue: tags, then B words per entry (16-byte aligned)
     uint32_t *hq_meta = a.hitq + (size_t)blockIdx.x * a.hitq_cap * (B + 1);
     uint32_t *hq_res = hq_meta + a.hitq_cap;
@@ -260,15 +263,21 @@ void hd_batched_kernel(HdArgs a)
         if (it >= a.nitems) break;
         const HdItem I = a.items[it];
         const uint32_t nrows = I.r1 - I.px_start;
+        const uint32_t pxs5 = I.px_start >> 5;
         __syncthreads();  // previous item finished with LDS
-        for (uint32_t r = lane; r < nrows; r += 64) {
-            xoff[r] = (uint16_t)(a.codes[I.x_start + I.px_start + r] * wl);
-            acc_v[r] = 0;
-            acc_u[r] = 0;
-            ymask[r] = 0;
+        // row r reads its window of NE_{x[px]} at byte offset xb[r] (+ 4 lane + 256 block)
+        for (uint32_t r = lane; r < nrows + 2; r += 64) {
+            if (r < nrows) {
+                const uint32_t px = I.px_start + r;
+                xb[r] = (a.codes[I.x_start + px] * wl + (px >> 5) - pxs5) * 4;
+                acc_v[r] = 0;
+                acc_u[r] = 0;
+                ymask[r] = 0;
+            } else {
+                xb[r] = 0;
+            }
         }
         const uint32_t rep0 = I.r0 > a.k - 1 ? I.r0 : a.k - 1;
-        const uint32_t pxs5 = I.px_start >> 5;
         const uint32_t bias = hd_bias(a.e);
         for (uint32_t ri = I.rb; ri < I.re; ++ri) {
             const HdRange R = a.ranges[ri];
@@ -288,24 +297,39 @@ void hd_batched_kernel(HdArgs a)
 #pragma unroll
             for (int b = 0; b < B; ++b) planes_set<P>(cnt[b], bias);
 
-            // one row: window counters of row r (with the column leaving at r - k when
-            // LEAVE), then the row's hit flags into the queue
-            auto row = [&](uint32_t r, auto leave) {
-                const uint32_t px = I.px_start + r;
-                const uint32_t wn = xoff[r] + (px >> 5) - pxs5 + lane;
-                const uint32_t shn = px & 31;
-                uint32_t wo = 0, sho = 0;
-                if constexpr (decltype(leave)::value) {
-                    wo = xoff[r - a.k] + ((px - a.k) >> 5) - pxs5 + lane;
-                    sho = (px - a.k) & 31;
+            // The NE words a row reads: for each block, the two words under its window for the
+            // entering column (en) and for the column leaving at r - k (lv).
+            struct Win {
+                uint32_t en[B][2], lv[B][2];
+            };
+            // (xn, xo: xb of rows r and r - k)
+            auto load_win = [&](uint32_t xn, uint32_t xo, Win &w, auto leave) {
+                const char *pn = reinterpret_cast<const char *>(ne_l) + (xn + lane4);
+                const char *po = reinterpret_cast<const char *>(ne_l) + (xo + lane4);
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const uint32_t *wn = reinterpret_cast<const uint32_t *>(pn + 256 * b);
+                    w.en[b][0] = wn[0];
+                    w.en[b][1] = wn[1];
+                    if constexpr (decltype(leave)::value) {
+                        const uint32_t *wo = reinterpret_cast<const uint32_t *>(po + 256 * b);
+                        w.lv[b][0] = wo[0];
+                        w.lv[b][1] = wo[1];
+                    }
                 }
+            };
+            // one row: window counters of row r (with the column leaving at r - k when
+            // LEAVE) from its loaded words, then the row's hit flags into the queue
+            auto row = [&](uint32_t r, const Win &w, auto leave) {
+                const uint32_t px = I.px_start + r;
+                const uint32_t shn = px & 31, sho = (px - a.k) & 31;
                 uint32_t high[B];
                 uint32_t all = ~0u;  // lanes' words with no hit in any block
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
-                    const uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 64 * b + 1], ne_l[wn + 64 * b], shn);
+                    const uint32_t an = __builtin_amdgcn_alignbit(w.en[b][1], w.en[b][0], shn);
                     if constexpr (decltype(leave)::value) {
-                        const uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 64 * b + 1], ne_l[wo + 64 * b], sho);
+                        const uint32_t bo = __builtin_amdgcn_alignbit(w.lv[b][1], w.lv[b][0], sho);
                         planes_slide<P>(cnt[b], an, bo);
                     } else {
                         planes_inc<P>(cnt[b], an);
@@ -342,18 +366,38 @@ void hd_batched_kernel(HdArgs a)
             };
             const uint32_t warm = nrows < a.k ? nrows : a.k;
             // two rows per trip: the counter planes then stay in place across the back edge
-            // (one row per trip leaves register copies of the planes in the loop)
+            // (one row per trip leaves register copies of the planes in the loop).  The row
+            // offsets are loaded a trip ahead, so a row waits for one LDS round trip.
             uint32_t r = 0;
+            Win wa, wb;
             for (; r + 1 < warm; r += 2) {
-                row(r, std::false_type{});
-                row(r + 1, std::false_type{});
+                load_win(xb[r], 0u, wa, std::false_type{});
+                load_win(xb[r + 1], 0u, wb, std::false_type{});
+                row(r, wa, std::false_type{});
+                row(r + 1, wb, std::false_type{});
             }
-            if (r < warm) row(r++, std::false_type{});
-            for (; r + 1 < nrows; r += 2) {
-                row(r, std::true_type{});
-                row(r + 1, std::true_type{});
+            if (r < warm) {
+                load_win(xb[r], 0u, wa, std::false_type{});
+                row(r, wa, std::false_type{});
+                ++r;
             }
-            if (r < nrows) row(r, std::true_type{});
+            if (r < nrows) {
+                uint32_t n0 = xb[r], n1 = xb[r + 1], o0 = xb[r - a.k], o1 = xb[r + 1 - a.k];
+                for (; r + 1 < nrows; r += 2) {
+                    const uint32_t m0 = xb[r + 2], m1 = xb[r + 3];
+                    const uint32_t p0 = xb[r + 2 - a.k], p1 = xb[r + 3 - a.k];
+                    load_win(n0, o0, wa, std::true_type{});
+                    row(r, wa, std::true_type{});
+                    load_win(n1, o1, wb, std::true_type{});
+                    row(r + 1, wb, std::true_type{});
+                    n0 = m0;
+                    n1 = m1;
+                    o0 = p0;
+                    o1 = p1;
+                }
+                load_win(n0, o0, wa, std::true_type{});
+                if (r < nrows) row(r, wa, std::true_type{});
+            }
             if (nq) drain_hits<B>(a, R, q0, I.px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask);
             __syncthreads();
             // distinct sequences of this range per row
@@ -431,9 +475,9 @@ static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream
 
 size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max)
 {
-    // NE window | v, sequence mask | range starts | u, x offsets (16 bit)
+    // NE window | v, sequence mask | range starts | row offsets (+2) | u (16 bit)
     return (size_t)sigma * wl * 4 + (size_t)rows_max * 8 + (kMaxRangeSeqs + 1) * 4 +
-           (size_t)rows_max * 4;
+           ((size_t)rows_max + 2) * 4 + (size_t)rows_max * 2;
 }
 
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)
