@@ -115,8 +115,8 @@ __global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n, uint32_t v
 
 // Folds the queued hit words of the current text range into the per-row accumulators.
 // Entry q of the wave's queue holds the tag r | lane << 16 (row r of the item, lane of the
-// wave) and the lane's B hit words of that row; bit t of word b flags text position
-// pt = q0 + 2048 b + 32 lane + t + px.  Each lane takes whole entries.  A word is split
+// wave) and the lane's B high-plane words of that row; a clear bit t of word b flags text
+// position pt = q0 + 2048 b + 32 lane + t + px.  Each lane takes whole entries.  A word is split
 // at the sequence starts it spans (found once per word by binary search over the range's
 // starts in LDS); within one sequence y the flags that count are those inside the text
 // range whose window lies entirely in y (pt >= start(y) + k - 1).  Their number goes to
@@ -138,14 +138,15 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
     for (uint32_t q = threadIdx.x; q < nq; q += 64) {
         const uint32_t meta = hq_meta[q];
         uint32_t w[B];
+        // entries hold the high-plane words: a clear bit is a hit
         if constexpr (B == 4) {
             const uint4 v4 = *reinterpret_cast<const uint4 *>(hq_res + q * 4);
-            w[0] = v4.x; w[1] = v4.y; w[2] = v4.z; w[3] = v4.w;
+            w[0] = ~v4.x; w[1] = ~v4.y; w[2] = ~v4.z; w[3] = ~v4.w;
         } else if constexpr (B == 2) {
             const uint2 v2 = *reinterpret_cast<const uint2 *>(hq_res + q * 2);
-            w[0] = v2.x; w[1] = v2.y;
+            w[0] = ~v2.x; w[1] = ~v2.y;
         } else {
-            w[0] = hq_res[q];
+            w[0] = ~hq_res[q];
         }
         const uint32_t r = meta & 0xFFFFu;
         const int32_t base0 = q0 + (int32_t)(32 * (meta >> 16) + px_start + r);
@@ -341,20 +342,20 @@ void hd_batched_kernel(HdArgs a)
                     const uint64_t m = __ballot(all != ~0u);
                     if (m) {
                         if (all != ~0u) {
-                            uint32_t res[B];
-#pragma unroll
-                            for (int b = 0; b < B; ++b) res[b] = ~high[b];
+                            // the entry holds the high-plane words (clear bit = hit); 32-bit
+                            // byte offsets from the wave's (scalar) queue bases
                             const uint32_t slot =
                                 nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                            char *res_b = reinterpret_cast<char *>(hq_res) + slot * (4u * B);
                             if constexpr (B == 4) {
-                                *reinterpret_cast<uint4 *>(hq_res + slot * 4) = make_uint4(res[0], res[1], res[2], res[3]);
+                                *reinterpret_cast<uint4 *>(res_b) = make_uint4(high[0], high[1], high[2], high[3]);
                             } else if constexpr (B == 2) {
-                                *reinterpret_cast<uint2 *>(hq_res + slot * 2) = make_uint2(res[0], res[1]);
+                                *reinterpret_cast<uint2 *>(res_b) = make_uint2(high[0], high[1]);
                             } else {
-                                hq_res[slot] = res[0];
+                                *reinterpret_cast<uint32_t *>(res_b) = high[0];
                             }
-                            hq_meta[slot] = r | (lane << 16);
+                            *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hq_meta) + slot * 4u) = r | (lane << 16);
                         }
                         nq += (uint32_t)__builtin_popcountll(m);
                         if (nq > a.hitq_cap - 64) {
