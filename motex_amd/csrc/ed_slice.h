@@ -31,6 +31,14 @@ namespace motex {
 
 constexpr int kSlicePc = 5;  // popcount planes (counts up to 31)
 
+// Truth tables of the 3-input logic op (v_bitop3_b32: result bit = table bit number
+// 4 src0 + 2 src1 + src2) used below; the host check of this file evaluates them against the
+// plain expressions the host build uses.
+constexpr uint32_t kBop3Mux = 0xE4;  // src2 ? src0 : src1
+constexpr uint32_t kBop3And = 0xC0;  // src0 & src1
+constexpr uint32_t kBop3Gt = 0xB2;   // majority(src0, ~src1, src2)
+constexpr uint32_t kBop3Lt = 0x8E;   // majority(~src0, src1, src2)
+
 template <int K> struct SliceW { static constexpr int W = 2 * K; };
 
 template <int K>
@@ -45,7 +53,7 @@ struct Slice {
 MOTEX_HD uint32_t slice_mux(uint32_t sel, uint32_t t, uint32_t f)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_bitop3_b32(t, f, sel, 0xE4);
+    return __builtin_amdgcn_bitop3_b32(t, f, sel, kBop3Mux);
 #else
     return (t & sel) | (f & ~sel);
 #endif
@@ -55,7 +63,7 @@ MOTEX_HD uint32_t slice_mux(uint32_t sel, uint32_t t, uint32_t f)
 MOTEX_HD uint32_t slice_and(uint32_t x, uint32_t m)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_bitop3_b32(x, m, x, 0xC0);
+    return __builtin_amdgcn_bitop3_b32(x, m, x, kBop3And);
 #else
     return x & m;
 #endif
@@ -117,36 +125,40 @@ MOTEX_HD void slice_popcount(const uint32_t *x, uint32_t out[kSlicePc])
     for (int q = 2; q < kSlicePc; ++q) out[q] = hi[q - 2];
 }
 
-// Texts where a > b for numbers of N planes (plane N-1 most significant).  The planes are
-// combined pairwise in a tree (greater, equal) so the dependency chain is log2(N) deep.
-template <int N>
-MOTEX_HD void slice_cmp(const uint32_t *a, const uint32_t *b, uint32_t &gt, uint32_t &eq)
+// Running comparison bits.  Scanning planes from the least significant up, "a > b so far"
+// changes only where the planes differ: it becomes a's bit there.  That is
+// majority(a, ~b, r), one 3-input logic op per plane (gfx950 v_bitop3_b32, kBop3Gt); for
+// "a < b so far" it is majority(~a, b, r) (kBop3Lt).
+MOTEX_HD uint32_t slice_gt_step(uint32_t a, uint32_t b, uint32_t r)
 {
-    uint32_t g[N], e[N];
-    for (int p = 0; p < N; ++p) {
-        g[p] = a[p] & ~b[p];
-        e[p] = ~(a[p] ^ b[p]);
-    }
-    for (int step = 1; step < N; step *= 2)
-        for (int p = 0; p + step < N; p += 2 * step) {
-            g[p] = g[p + step] | (e[p + step] & g[p]);
-            e[p] = e[p + step] & e[p];
-        }
-    gt = g[0];
-    eq = e[0];
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, r, kBop3Gt);
+#else
+    return (a & ~b) | (a & r) | (~b & r);
+#endif
+}
+
+MOTEX_HD uint32_t slice_lt_step(uint32_t a, uint32_t b, uint32_t r)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, r, kBop3Lt);
+#else
+    return (~a & b) | (~a & r) | (b & r);
+#endif
 }
 
 // Texts where (pa, va) is strictly better than (pb, vb) under min*: smaller popcount, or
-// equal popcount and larger word.  va / vb have NV planes.
+// equal popcount and larger word.  va / vb have NV planes.  The key (popcount inverted, then
+// word) is compared in one ripple from the word's lowest plane up through the popcount
+// planes.
 template <int NV>
 MOTEX_HD uint32_t slice_better(const uint32_t pa[kSlicePc], const uint32_t *va,
                                const uint32_t pb[kSlicePc], const uint32_t *vb)
 {
-    uint32_t pgt, peq, vgt, veq;
-    slice_cmp<kSlicePc>(pb, pa, pgt, peq);  // pb > pa: a has the smaller popcount
-    slice_cmp<NV>(va, vb, vgt, veq);
-    (void)veq;
-    return pgt | (peq & vgt);
+    uint32_t r = 0;
+    for (int p = 0; p < NV; ++p) r = slice_gt_step(va[p], vb[p], r);
+    for (int q = 0; q < kSlicePc; ++q) r = slice_lt_step(pa[q], pb[q], r);
+    return r;
 }
 
 // "count <= e" over kSlicePc planes.
@@ -170,7 +182,6 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
     constexpr int W = SliceW<K>::W;
     // candidates, W+1 planes each (plane W only for the comparison)
     uint32_t a[W + 1], b[W + 1], c[W + 1];
-    uint32_t pa[kSlicePc], pb[kSlicePc], pcc[kSlicePc];
     a[0] = ~0u;
     b[0] = ~0u;
     c[0] = delta;
@@ -180,19 +191,20 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
         b[p] = p <= K - 1 ? L.d[p - 1] : slice_and(L.d[p - 1], full);
         c[p] = p <= K - 1 ? G.d[p - 1] : slice_and(G.d[p - 1], full);
     }
-    uint32_t lsrc[kSlicePc], gsrc[kSlicePc];
+    uint32_t lsrc[kSlicePc], gsrc[kSlicePc], pcc[kSlicePc];
     for (int q = 0; q < kSlicePc; ++q) {
         lsrc[q] = slice_mux(full, L.pc[q], L.pm[q]);
         gsrc[q] = slice_mux(full, G.pc[q], G.pm[q]);
     }
-    slice_inc(U.pc, ~0u, pa);
-    slice_inc(lsrc, ~0u, pb);
     slice_inc(gsrc, delta, pcc);
-    // min*: start from a, replace where b then c is strictly better
-    uint32_t sel = slice_better<W + 1>(pb, b, pa, a);
-    uint32_t d[W + 1], pd[kSlicePc];
-    for (int p = 0; p <= W; ++p) d[p] = slice_mux(sel, b[p], a[p]);
-    for (int q = 0; q < kSlicePc; ++q) pd[q] = slice_mux(sel, pb[q], pa[q]);
+    // min*: start from a, replace where b then c is strictly better.  a and b both set plane
+    // 0 and add one to their source's popcount, so they compare as their sources do.
+    uint32_t sel = slice_better<W>(lsrc, b + 1, U.pc, a + 1);
+    uint32_t d[W + 1], pd[kSlicePc], ps[kSlicePc];
+    d[0] = ~0u;
+    for (int p = 1; p <= W; ++p) d[p] = slice_mux(sel, b[p], a[p]);
+    for (int q = 0; q < kSlicePc; ++q) ps[q] = slice_mux(sel, lsrc[q], U.pc[q]);
+    slice_inc(ps, ~0u, pd);
     sel = slice_better<W + 1>(pcc, c, pd, d);
     for (int p = 0; p < W; ++p) D.d[p] = slice_mux(sel, c[p], d[p]);
     if (dropped) *dropped |= slice_mux(sel, c[W], d[W]);  // host check of the width bound
@@ -211,7 +223,7 @@ MOTEX_HD void slice_cell_masked(const uint32_t *ud, const uint32_t *upc, const u
 {
     constexpr int W = SliceW<K>::W;
     uint32_t a[W + 1], b[W + 1], c[W + 1];
-    uint32_t pa[kSlicePc], pb[kSlicePc], pcc[kSlicePc];
+    uint32_t pcc[kSlicePc];
     a[0] = ~0u;
     b[0] = ~0u;
     c[0] = delta;
@@ -220,13 +232,14 @@ MOTEX_HD void slice_cell_masked(const uint32_t *ud, const uint32_t *upc, const u
         b[p] = p <= K - 1 ? ld[p - 1] : 0u;
         c[p] = p <= K - 1 ? gd[p - 1] : 0u;
     }
-    slice_inc(upc, ~0u, pa);
-    slice_inc(lpm, ~0u, pb);
     slice_inc(gpm, delta, pcc);
-    uint32_t sel = slice_better<W + 1>(pb, b, pa, a);
-    uint32_t d[W + 1], pd[kSlicePc];
-    for (int p = 0; p <= W; ++p) d[p] = slice_mux(sel, b[p], a[p]);
-    for (int q = 0; q < kSlicePc; ++q) pd[q] = slice_mux(sel, pb[q], pa[q]);
+    // a and b both set plane 0 and add one to their source's popcount
+    uint32_t sel = slice_better<W>(lpm, b + 1, upc, a + 1);
+    uint32_t d[W + 1], pd[kSlicePc], ps[kSlicePc];
+    d[0] = ~0u;
+    for (int p = 1; p <= W; ++p) d[p] = slice_mux(sel, b[p], a[p]);
+    for (int q = 0; q < kSlicePc; ++q) ps[q] = slice_mux(sel, lpm[q], upc[q]);
+    slice_inc(ps, ~0u, pd);
     sel = slice_better<W + 1>(pcc, c, pd, d);
     for (int p = 0; p < W; ++p) dd[p] = slice_mux(sel, c[p], d[p]);
     if (dropped) *dropped |= slice_mux(sel, c[W], d[W]);

@@ -5,6 +5,7 @@
 // Usage: slice_check seed k e m nmin nmax sigma ntexts
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <string>
 #include <utility>
@@ -101,8 +102,46 @@ static int run(uint32_t e, uint32_t m, uint32_t nmin, uint32_t nmax, uint32_t si
     return bad || wide || masked_bad ? 1 : 0;
 }
 
+// The device build computes slice_mux, slice_and, slice_gt_step and slice_lt_step with
+// v_bitop3_b32 and the kBop3* tables; the host build with the plain expressions.  Evaluate
+// each table as the instruction does (bit 4 src0 + 2 src1 + src2 of the table) and compare.
+static uint32_t bitop3(uint32_t a, uint32_t b, uint32_t c, uint32_t table)
+{
+    uint32_t r = 0;
+    for (int t = 0; t < 32; ++t) {
+        const uint32_t idx = (((a >> t) & 1) << 2) | (((b >> t) & 1) << 1) | ((c >> t) & 1);
+        r |= ((table >> idx) & 1u) << t;
+    }
+    return r;
+}
+
+static int check_tables()
+{
+    // the 8 input combinations, one per bit position
+    const uint32_t a = 0xF0u, b = 0xCCu, c = 0xAAu;
+    int bad = 0;
+    bad += bitop3(a, b, c, motex::kBop3Mux) != motex::slice_mux(c, a, b);
+    bad += bitop3(a, b, a, motex::kBop3And) != motex::slice_and(a, b);
+    bad += bitop3(a, b, c, motex::kBop3Gt) != motex::slice_gt_step(a, b, c);
+    bad += bitop3(a, b, c, motex::kBop3Lt) != motex::slice_lt_step(a, b, c);
+    // and the comparisons themselves on small numbers, exhaustively
+    for (uint32_t x = 0; x < 64; ++x)
+        for (uint32_t y = 0; y < 64; ++y) {
+            uint32_t gt = 0, lt = 0;
+            for (int p = 0; p < 6; ++p) {
+                const uint32_t xp = (x >> p) & 1 ? ~0u : 0u, yp = (y >> p) & 1 ? ~0u : 0u;
+                gt = bitop3(xp, yp, gt, motex::kBop3Gt);
+                lt = bitop3(xp, yp, lt, motex::kBop3Lt);
+            }
+            bad += (gt != 0) != (x > y) || (lt != 0) != (x < y);
+        }
+    printf("tables: %d mismatches\n", bad);
+    return bad ? 1 : 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 2 && !strcmp(argv[1], "tables")) return check_tables();
     if (argc < 9) return 2;
     rng_state = strtoull(argv[1], 0, 10);
     const int k = atoi(argv[2]);

@@ -47,3 +47,13 @@ def test_sliced_step_matches_oracle(checker, case):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "mismatched_columns=0" in r.stdout and "wide_state=0" in r.stdout
     assert "masked_form_differs=0" in r.stdout
+
+
+def test_logic_tables_match_the_host_expressions(checker):
+    """The device build selects, masks and compares with v_bitop3_b32 and the kBop3* truth
+    tables of ed_slice.h; evaluated as the instruction does, each table must equal the plain
+    expression the host build (and so the check above) uses, and the comparison tables must
+    order 6-bit numbers exactly."""
+    r = subprocess.run([checker, "tables"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "tables: 0 mismatches" in r.stdout
