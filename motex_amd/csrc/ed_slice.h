@@ -161,15 +161,27 @@ MOTEX_HD uint32_t slice_better(const uint32_t pa[kSlicePc], const uint32_t *va,
     return r;
 }
 
+// The planes of the constant e (all ones where bit q of e is set), for slice_le_planes.
+MOTEX_HD void slice_e_planes(uint32_t e, uint32_t eb[kSlicePc])
+{
+    for (int q = 0; q < kSlicePc; ++q) eb[q] = ((e >> q) & 1) ? ~0u : 0u;
+}
+
+// "count <= e" over kSlicePc planes, e given as planes: deciding from the low bits up,
+// "c <= e so far" is majority(~c, e, r).
+MOTEX_HD uint32_t slice_le_planes(const uint32_t c[kSlicePc], const uint32_t eb[kSlicePc])
+{
+    uint32_t res = ~0u;
+    for (int q = 0; q < kSlicePc; ++q) res = slice_lt_step(c[q], eb[q], res);
+    return res;
+}
+
 // "count <= e" over kSlicePc planes.
 MOTEX_HD uint32_t slice_le(const uint32_t c[kSlicePc], uint32_t e)
 {
-    uint32_t res = ~0u;  // c <= e, deciding from the low bits up
-    for (int q = 0; q < kSlicePc; ++q) {
-        const uint32_t eb = ((e >> q) & 1) ? ~0u : 0u;
-        res = (~c[q] & (res | eb)) | (res & eb);
-    }
-    return res;
+    uint32_t eb[kSlicePc];
+    slice_e_planes(e, eb);
+    return slice_le_planes(c, eb);
 }
 
 // One cell of 32 texts.  U = D[i-1][j], L = D[i][j-1], G = D[i-1][j-1]; delta = texts whose

@@ -92,6 +92,9 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
 #pragma unroll
     for (int b = 0; b < 5; ++b) xb[b] = ((code >> b) & 1) ? ~0u : 0u;
     const bool report = j <= m && j >= (uint32_t)K;
+    uint32_t eb[kSlicePc];  // the planes of e, wave-uniform
+#pragma unroll
+    for (int q = 0; q < kSlicePc; ++q) eb[q] = __builtin_amdgcn_readfirstlane(((a.e >> q) & 1) ? ~0u : 0u);
     const uint32_t full = j <= (uint32_t)K ? ~0u : 0u;
     // state: D[i-1][j] (what lane L+1 receives next) and D[i-1][j-1]
     Slice<K> U, G;
@@ -125,7 +128,7 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
                 slice_cell_masked<K>(U.d, U.pc, L.d, L.pm, G.d, G.pm, ne, D.d, D.pc);
             }
             if (report) {
-                const uint32_t hit = slice_le(D.pc, a.e) & t[5];
+                const uint32_t hit = slice_le_planes(D.pc, eb) & t[5];
                 vcnt += __builtin_popcount(hit);
                 anyh |= hit;
             }
