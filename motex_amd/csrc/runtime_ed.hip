@@ -14,7 +14,7 @@ hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, hipStream_t s);
 uint32_t ed_slice_stream_row_words(uint32_t k);
 uint32_t ed_strip_cols(bool wide);
 
-static const uint64_t kEdTextPerItem = 8192;  // text symbols per work item
+static const uint64_t kEdTextPerItem = 8192;  // text symbols per work item, at most
 
 static bool ed_wide(const motex_params &p)
 {
@@ -23,9 +23,10 @@ static bool ed_wide(const motex_params &p)
     return longest > 14;
 }
 
-// Items: (x, run of consecutive texts of about kEdTextPerItem symbols).
+// Items: (x, run of consecutive texts of about text_per_item symbols).
 static void plan_ed(const SeqSet &ss, uint32_t k, const std::vector<uint32_t> &xs,
-                    const std::vector<uint32_t> *only_y, std::vector<EdItemDev> &out)
+                    const std::vector<uint32_t> *only_y, uint64_t text_per_item,
+                    std::vector<EdItemDev> &out)
 {
     out.clear();
     struct R { uint32_t j0, ny; };
@@ -37,7 +38,7 @@ static void plan_ed(const SeqSet &ss, uint32_t k, const std::vector<uint32_t> &x
         while (j < ss.num) {
             uint32_t j0 = j;
             uint64_t acc = 0;
-            while (j < ss.num && (acc == 0 || acc + ss.len[j] <= kEdTextPerItem)) { acc += ss.len[j]; ++j; }
+            while (j < ss.num && (acc == 0 || acc + ss.len[j] <= text_per_item)) { acc += ss.len[j]; ++j; }
             ranges.push_back({j0, j - j0});
         }
     }
@@ -81,13 +82,15 @@ static bool ed_slice_ok(const motex_batch *b)
     const uint32_t k = b->params.k;
     if (k < 4 || k > 14 || b->ss.sigma > 32 || ed_slice_stream_row_words(k) == 0) return false;
     if (env && env[0] == '1') return true;
-    // An item (x, group of 32 texts) is one wave's sequential sweep; with too few of them
-    // the device is not filled, and for k > 8 the one-text-per-lane kernel (many smaller
-    // items) is then faster (sweep over 200 vs 500 protein sequences of U[200,2000]).
+    // An item (x, group of 32 texts) is one wave's sequential sweep of 64-column strips, so
+    // its latency is long; with few items the one-text-per-lane kernel, whose items shrink
+    // to (x, one text) for small inputs, finishes first.  Measured on DNA and protein inputs
+    // of 40-400 sequences of U[100,2000], k 8-14: the text-parallel kernel wins 2-3x at 400
+    // sequences (5,200 items), ties or loses at 200 (1,400 items) and loses 4-10x below.
     uint64_t nx = 0;
     for (uint32_t i = 0; i < b->ss.num; ++i) nx += b->ss.len[i] >= k;
     const uint64_t items = nx * ((nx + 31) / 32);
-    return k <= 8 || items >= (uint64_t)b->ctx->num_cu * 6;
+    return items >= (uint64_t)b->ctx->num_cu * 8;
 }
 
 static int ed_prepare_slice(motex_batch *b)
@@ -153,14 +156,29 @@ static int ed_prepare_slice(motex_batch *b)
     return MOTEX_OK;
 }
 
+// Text symbols per item: an item is one wave's sequential wavefront, so a small job cut into
+// few long items is bound by the longest one.  Runs are shortened (down to one text) until
+// the items fill the persistent grid about twice; large inputs keep kEdTextPerItem.
+static uint64_t ed_text_per_item(const motex_batch *b, uint64_t nx)
+{
+    const uint64_t target = (uint64_t)b->ctx->num_cu * std::max<uint32_t>(b->tile.waves_per_cu, 1) * 2;
+    const uint64_t t = (uint64_t)b->ss.total * nx / target;
+    return std::min<uint64_t>(std::max<uint64_t>(t, 1), kEdTextPerItem);
+}
+
 int ed_prepare(motex_batch *b)
 {
     if (ed_slice_ok(b)) return ed_prepare_slice(b);
     const SeqSet &ss = b->ss;
     std::vector<uint32_t> xs(ss.num);
-    for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
+    uint64_t nx = 0;
+    for (uint32_t i = 0; i < ss.num; ++i) {
+        xs[i] = i;
+        nx += ss.len[i] >= b->params.k;
+    }
+    b->tile = choose_tile(b->ctx, b->params, ss);
     std::vector<EdItemDev> items;
-    plan_ed(ss, b->params.k, xs, nullptr, items);
+    plan_ed(ss, b->params.k, xs, nullptr, ed_text_per_item(b, nx), items);
     bool wide = ed_wide(b->params);
     std::vector<uint64_t> cost(items.size());
     std::vector<size_t> order(items.size());
@@ -168,7 +186,6 @@ int ed_prepare(motex_batch *b)
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t z) { return cost[a] > cost[z]; });
     std::vector<EdItemDev> sorted(items.size());
     for (size_t t = 0; t < items.size(); ++t) sorted[t] = items[order[t]];
-    b->tile = choose_tile(b->ctx, b->params, ss);
     b->tile.variant = wide ? 1 : 0;
     hipError_t e;
     uint64_t ni = std::max<size_t>(sorted.size(), 1);
