@@ -22,6 +22,20 @@ int set_error(int code, const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);
 int validate_params(const motex_params *p);
 
+// Indices 0..n-1 ordered by cost, largest first, equal costs in index order (an LSD radix
+// sort on 16-bit digits: linear in n, for work queues of 10^5-10^6 items).
+std::vector<uint32_t> order_by_cost_desc(const std::vector<uint64_t> &cost);
+
+// Host-side phase timer for batch set-up: with MOTEX_TRACE=1 in the environment, mark()
+// prints the milliseconds since the previous mark to stderr; otherwise it does nothing.
+struct PhaseTrace {
+    bool on;
+    double t;
+    const char *scope;
+    explicit PhaseTrace(const char *scope_);
+    void mark(const char *phase);
+};
+
 // Stream whose stream-ordered allocator DevBuf uses on this thread (serving jobs), or null
 // for plain hipMalloc/hipFree.
 extern thread_local hipStream_t g_alloc_stream;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
@@ -48,6 +49,60 @@ int hip_fail(hipError_t e, const char *what)
 // ---------------------------------------------------------------------------------------
 // parameter validation (the extraction-relevant checks of motex.cc:98-201)
 // ---------------------------------------------------------------------------------------
+std::vector<uint32_t> order_by_cost_desc(const std::vector<uint64_t> &cost)
+{
+    const size_t n = cost.size();
+    uint64_t hi = 0;
+    for (uint64_t c : cost) hi = std::max(hi, c);
+    // keys hi - cost ascending = cost descending; stable passes keep index order on ties
+    std::vector<uint64_t> k0(n), k1(n);
+    std::vector<uint32_t> a(n), b(n);
+    for (size_t i = 0; i < n; ++i) {
+        k0[i] = hi - cost[i];
+        a[i] = (uint32_t)i;
+    }
+    constexpr int kBits = 11;
+    uint32_t cnt[1u << kBits];
+    for (int shift = 0; shift < 64 && (hi >> shift); shift += kBits) {
+        std::fill(cnt, cnt + (1u << kBits), 0u);
+        for (size_t i = 0; i < n; ++i) ++cnt[(k0[i] >> shift) & ((1u << kBits) - 1)];
+        uint32_t sum = 0;
+        for (uint32_t &c : cnt) {
+            const uint32_t t = c;
+            c = sum;
+            sum += t;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t d = cnt[(k0[i] >> shift) & ((1u << kBits) - 1)]++;
+            k1[d] = k0[i];
+            b[d] = a[i];
+        }
+        k0.swap(k1);
+        a.swap(b);
+    }
+    return a;
+}
+
+static double wall_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+PhaseTrace::PhaseTrace(const char *scope_) : on(false), t(0), scope(scope_)
+{
+    const char *env = getenv("MOTEX_TRACE");
+    on = env && env[0] == '1';
+    if (on) t = wall_ms();
+}
+
+void PhaseTrace::mark(const char *phase)
+{
+    if (!on) return;
+    const double now = wall_ms();
+    fprintf(stderr, "[motex %s] %s %.3f ms\n", scope, phase, now - t);
+    t = now;
+}
+
 int validate_params(const motex_params *p)
 {
     if (!p) return set_error(MOTEX_EINVAL, "params is NULL");
@@ -371,13 +426,16 @@ int hd_prepare(motex_batch *b)
 {
     motex_ctx *c = b->ctx;
     const SeqSet &ss = b->ss;
+    PhaseTrace tr("hd_prepare");
     TileShape ts = choose_tile(c, b->params, ss);
     b->tile = ts;
     std::vector<uint32_t> xs(ss.num);
     for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
     // group size: aim at ~16 items per resident wave so the tail of the queue is short
     plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, 1u << 30);
+    tr.mark("ranges");
     const bool sym = hd_symmetric(b, b->hd);
+    tr.mark("symmetry estimate");
     uint64_t pairs_units = 0;  // tiles x ranges
     {
         size_t r_first = 0;
@@ -394,20 +452,29 @@ int hd_prepare(motex_batch *b)
     uint64_t target_items = (uint64_t)c->num_cu * ts.waves_per_cu * 16;
     uint32_t gmax = (uint32_t)std::max<uint64_t>(1, pairs_units / std::max<uint64_t>(target_items, 1));
     plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, gmax, sym);
+    tr.mark("items");
     // longest items first: they set the tail of the persistent schedule
     std::vector<uint64_t> span(b->hd.ranges.size() + 1, 0);
     for (size_t r = 0; r < b->hd.ranges.size(); ++r) span[r + 1] = span[r] + (b->hd.ranges[r].ye - b->hd.ranges[r].ys);
-    std::stable_sort(b->hd.items.begin(), b->hd.items.end(), [&](const HdItem &x, const HdItem &z) {
-        uint64_t cx = (uint64_t)(x.r1 - x.px_start) * (span[x.re] - span[x.rb] + (x.re - x.rb) * (x.r1 - x.r0));
-        uint64_t cz = (uint64_t)(z.r1 - z.px_start) * (span[z.re] - span[z.rb] + (z.re - z.rb) * (z.r1 - z.r0));
-        return cx > cz;
-    });
+    {
+        std::vector<uint64_t> cost(b->hd.items.size());
+        for (size_t t = 0; t < cost.size(); ++t) {
+            const HdItem &x = b->hd.items[t];
+            cost[t] = (uint64_t)(x.r1 - x.px_start) * (span[x.re] - span[x.rb] + (x.re - x.rb) * (x.r1 - x.r0));
+        }
+        const std::vector<uint32_t> order = order_by_cost_desc(cost);
+        std::vector<HdItem> sorted(order.size());
+        for (size_t t = 0; t < order.size(); ++t) sorted[t] = b->hd.items[order[t]];
+        b->hd.items.swap(sorted);
+    }
     b->hd.wl = hd_lds_words(ts.blocks, b->hd.rows_max);
     if (hd_lds_bytes(ss.sigma, b->hd.wl, b->hd.rows_max) > 65536 || (uint64_t)ss.sigma * b->hd.wl >= 65536)
         return set_error(MOTEX_EUNSUPPORTED, "alphabet of %u symbols does not fit the kernel's LDS window", ss.sigma);
+    tr.mark("sort");
     uint64_t W = 0;
     int rc = hd_upload_plan(b, b->hd, b->d_items, b->d_ranges, &W);
     if (rc) return rc;
+    tr.mark("upload plan");
     // the plain form's windows (one block, short row tiles) must fit as well
     W = std::max<uint64_t>(W, (ss.total + kPadBits) / 32 + 4 * 64 + 64);
     b->hd.W = (uint32_t)W;
@@ -418,7 +485,9 @@ int hd_prepare(motex_batch *b)
         return hip_fail(e, "NE build");
     // hit queues of the persistent grid (one per workgroup)
     const uint64_t grid = std::min<uint64_t>((uint64_t)c->num_cu * ts.waves_per_cu, b->hd.items.size());
+    tr.mark("NE tables");
     if ((e = hd_alloc_hitq(b->d_hitq, grid, ts.blocks)) != hipSuccess) return hip_fail(e, "alloc hit queues");
+    tr.mark("hit queues");
     b->stats.symmetric = sym ? 1u : 0u;
     b->stats.swept_cells = 0;
     for (const HdItem &it : b->hd.items)
@@ -430,7 +499,9 @@ int hd_prepare(motex_batch *b)
     b->split_words = (uint32_t)((ss.uv_off[ss.num] + 31) / 32 + 1);
     uint64_t sf = (uint64_t)std::max<uint32_t>(b->hd.split_slots, 1) * b->split_words;
     if ((e = b->d_split.alloc(sf * 4)) != hipSuccess) return hip_fail(e, "alloc split flags");
+    tr.mark("bits and flags");
     if ((e = hipStreamSynchronize(bstream(b))) != hipSuccess) return hip_fail(e, "NE build sync");
+    tr.mark("sync");
     return MOTEX_OK;
 }
 
@@ -744,15 +815,20 @@ int motex::batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *l
     b->params.gap_max = b->gmax.data();
     b->params.box_len = b->blen.data();
     b->params.box_err = b->berr.data();
+    PhaseTrace tr("batch_create");
     b->S = enumerate_structs(&b->params, &b->nb_structs);
     if ((rc = b->ss.build(seqs, lens, num, b->nb_structs))) return rc;
     b->stats.cells = pair_cells(b->ss, b->params, &b->stats.pairs, &b->stats.skipped_pairs);
+    tr.mark("pack");
     if ((rc = batch_upload_common(b.get()))) return rc;
+    tr.mark("upload");
     if (p->nb_gaps) rc = struct_prepare(b.get());
     else if (p->distance == 0) rc = hd_prepare(b.get());
     else rc = ed_prepare(b.get());
     if (rc) return rc;
+    tr.mark("prepare");
     if ((e = hipStreamSynchronize(bstream(b.get()))) != hipSuccess) return hip_fail(e, "batch upload");
+    tr.mark("sync");
     if (b->stats.swept_cells == 0) b->stats.swept_cells = b->stats.cells;  // edit, structured
     b->stats.tile_blocks = b->tile.blocks;
     b->stats.tile_rows = b->tile.rows;

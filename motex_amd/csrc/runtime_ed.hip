@@ -122,12 +122,15 @@ static int ed_prepare_slice(motex_batch *b)
         for (uint32_t g = 0; g < ngroups; ++g)
             items.push_back({ss.start[i], ss.len[i], (uint32_t)ss.uv_off[i], g});
     }
-    auto cost = [&](const EdSliceItem &it) {
-        return (uint64_t)((it.m + 63) / 64) * (goff[it.group + 1] - goff[it.group] + 63);
-    };
-    std::stable_sort(items.begin(), items.end(), [&](const EdSliceItem &x, const EdSliceItem &z) {
-        return cost(x) > cost(z);
-    });
+    {
+        std::vector<uint64_t> cost(items.size());
+        for (size_t t = 0; t < items.size(); ++t)
+            cost[t] = (uint64_t)((items[t].m + 63) / 64) * (goff[items[t].group + 1] - goff[items[t].group] + 63);
+        const std::vector<uint32_t> order = order_by_cost_desc(cost);
+        std::vector<EdSliceItem> sorted(order.size());
+        for (size_t t = 0; t < order.size(); ++t) sorted[t] = items[order[t]];
+        items.swap(sorted);
+    }
     b->tile = choose_tile(b->ctx, b->params, ss);
     b->tile.variant = 2;
     b->ed_slice = true;
@@ -181,9 +184,8 @@ int ed_prepare(motex_batch *b)
     plan_ed(ss, b->params.k, xs, nullptr, ed_text_per_item(b, nx), items);
     bool wide = ed_wide(b->params);
     std::vector<uint64_t> cost(items.size());
-    std::vector<size_t> order(items.size());
-    for (size_t t = 0; t < items.size(); ++t) { cost[t] = ed_item_cost(ss, items[t], wide); order[t] = t; }
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t z) { return cost[a] > cost[z]; });
+    for (size_t t = 0; t < items.size(); ++t) cost[t] = ed_item_cost(ss, items[t], wide);
+    const std::vector<uint32_t> order = order_by_cost_desc(cost);
     std::vector<EdItemDev> sorted(items.size());
     for (size_t t = 0; t < items.size(); ++t) sorted[t] = items[order[t]];
     b->tile.variant = wide ? 1 : 0;

@@ -109,9 +109,14 @@ int struct_prepare(motex_batch *b)
     std::vector<StructItem> items;
     b->stats.symmetric = struct_symmetric(b) ? 1u : 0u;
     plan_pairs(ss, total_motif_length(&b->params), nullptr, items, b->stats.symmetric != 0);
-    std::stable_sort(items.begin(), items.end(), [](const StructItem &x, const StructItem &z) {
-        return (uint64_t)x.m * x.n > (uint64_t)z.m * z.n;
-    });
+    {
+        std::vector<uint64_t> cost(items.size());
+        for (size_t t = 0; t < items.size(); ++t) cost[t] = (uint64_t)items[t].m * items[t].n;
+        const std::vector<uint32_t> order = order_by_cost_desc(cost);
+        std::vector<StructItem> sorted(order.size());
+        for (size_t t = 0; t < order.size(); ++t) sorted[t] = items[order[t]];
+        items.swap(sorted);
+    }
     if ((e = b->d_items.alloc(std::max<size_t>(items.size(), 1) * sizeof(StructItem))) != hipSuccess)
         return hip_fail(e, "alloc items");
     if (!items.empty() &&
