@@ -180,7 +180,7 @@ def main():
     achieved = cells * ops / (kms / 1e3) / 1e12
     kernel = ("struct_kernel" if w.boxes else
               "hd_batched_kernel" if w.distance == 0 else
-              "ed_slice_kernel" if st["kernel_variant"] == 2 else "ed_kernel")
+              {2: "ed_slice_kernel", 3: "ed_slice_pipe_kernel"}.get(st["kernel_variant"], "ed_kernel"))
     tr = traffic_from_profile(kernel, f"config{a.config}") if a.scale == 1.0 else None
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_INT32_PEAK_TOPS,
                 "unit": "TOP/s", "frac": round(achieved / VALU_INT32_PEAK_TOPS, 4),

@@ -193,6 +193,199 @@ __global__ __launch_bounds__(64) void ed_slice_kernel(EdSliceArgs a)
     }
 }
 
+
+// Fills an LDS chunk of 64 rows from r0 with the previous strip's boundary cells (when
+// from_stream; zeros for the first strip, whose left edge is column 0) and the group's text
+// planes.  Done by the 64 lanes of the wave that reads the chunk.
+template <int K>
+__device__ __forceinline__ void slice_chunk_fill(uint32_t *chunk, uint32_t r0, uint32_t nrows, bool from_stream,
+                                                 const uint32_t *sin, const uint32_t *gp, uint32_t lane)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr int W = Lay::W, SWS = Lay::SWS, LW = Lay::LW;
+    for (uint32_t w = lane; w < 64u * LW; w += 64) {
+        const uint32_t rr = w / LW, c = w % LW, row = r0 + rr;
+        uint32_t val = 0;
+        if (row < nrows) {
+            if (c < (uint32_t)W) val = from_stream ? sin[(size_t)row * SWS + c] : 0u;
+            else if (c < (uint32_t)W + 6) val = gp[(size_t)row * 6 + (c - W)];
+        }
+        chunk[w] = val;
+    }
+}
+
+// One lane's column j of a strip, carried from row to row: D[i-1][j] (what lane L+1 receives
+// next), D[i-1][j-1], the text planes lane L+1 receives next, and the column's hit counts.
+template <int K>
+struct SliceCol {
+    Slice<K> U, G;
+    uint32_t tpub[6];
+    uint32_t xb[5];          // planes of x[j]'s code
+    uint32_t eb[kSlicePc];   // planes of e, wave-uniform
+    uint32_t vcnt, anyh, j, full;
+    bool report;
+};
+
+template <int K>
+__device__ __forceinline__ void slice_col_begin(SliceCol<K> &c, const EdSliceArgs &a, const EdSliceItem &I,
+                                                uint32_t strip, uint32_t lane)
+{
+    c.j = strip * 64 + lane + 1;  // 1-based column
+    const uint32_t code = c.j <= I.m ? a.codes[I.x_start + c.j - 1] : 0u;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) c.xb[b] = ((code >> b) & 1) ? ~0u : 0u;
+    c.report = c.j <= I.m && c.j >= (uint32_t)K;
+#pragma unroll
+    for (int q = 0; q < kSlicePc; ++q) c.eb[q] = __builtin_amdgcn_readfirstlane(((a.e >> q) & 1) ? ~0u : 0u);
+    c.full = c.j <= (uint32_t)K ? ~0u : 0u;
+    slice_row0<K>(c.j, c.U);
+    slice_row0<K>(c.j - 1, c.G);
+#pragma unroll
+    for (int b = 0; b < 6; ++b) c.tpub[b] = 0;
+    c.vcnt = 0;
+    c.anyh = 0;
+}
+
+// Step s of the strip's skewed wavefront: lane L computes row s-1-L.  Only a cell's planes
+// travel between lanes; the receiver recounts its popcounts.  FIRST: the first strip, whose
+// left edge is column 0 and whose first K columns enter left and diag untruncated
+// (slice_cell, needing pc and pm of left).  Later strips have only columns j > K
+// (slice_cell_masked, needing pm).  CHECK: some lane has no row at this step.
+template <int K, bool FIRST, bool CHECK>
+__device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uint32_t *chunk, uint32_t nrows,
+                                           uint32_t lane, uint32_t *sout, bool emit)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr int W = Lay::W, LW = Lay::LW;
+    const uint32_t *cr = chunk + ((s - 1) & 63) * LW;
+    Slice<K> L;  // D[i][j-1]
+#pragma unroll
+    for (int p = 0; p < W; ++p) L.d[p] = shr1_from(FIRST ? 0u : cr[p], c.U.d[p]);
+    if constexpr (FIRST) slice_popcount<W>(L.d, L.pc);
+    slice_pm<K>(L.d, L.pm);
+    uint32_t t[6];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[W + b], c.tpub[b]);
+    bool active = true;
+    if constexpr (CHECK) active = s > lane && s - lane <= nrows;
+    if (active) {
+        uint32_t ne = 0;
+#pragma unroll
+        for (int b = 0; b < 5; ++b) ne |= t[b] ^ c.xb[b];
+        Slice<K> D;
+        if constexpr (FIRST) {
+            slice_cell<K>(c.U, L, c.G, ne, c.full, D);
+        } else {
+            slice_cell_masked<K>(c.U.d, c.U.pc, L.d, L.pm, c.G.d, c.G.pm, ne, D.d, D.pc);
+        }
+        if (c.report) {
+            const uint32_t hit = slice_le_planes(D.pc, c.eb) & t[5];
+            c.vcnt += __builtin_popcount(hit);
+            c.anyh |= hit;
+        }
+        c.G = L;
+        c.U = D;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) c.tpub[b] = t[b];
+        if (emit && lane == 63) slice_emit<K>(sout, s, D.d);
+    }
+}
+
+// The steps of chunk ch: 64 ch + 1 .. 64 ch + 64, up to nrows + 63.  Where every lane has a
+// row, two steps per trip, so the cell planes are not copied between registers on each
+// back edge.
+template <int K, bool FIRST>
+__device__ __forceinline__ void slice_run_chunk(SliceCol<K> &c, uint32_t ch, const uint32_t *chunk,
+                                                uint32_t nrows, uint32_t lane, uint32_t *sout, bool emit)
+{
+    const uint32_t s0 = 64 * ch + 1, s1 = min(64 * ch + 64, nrows + 63);
+    if (s0 >= 64 && s1 <= nrows) {
+        uint32_t s = s0;
+        for (; s + 1 <= s1; s += 2) {
+            slice_step<K, FIRST, false>(c, s, chunk, nrows, lane, sout, emit);
+            slice_step<K, FIRST, false>(c, s + 1, chunk, nrows, lane, sout, emit);
+        }
+        if (s <= s1) slice_step<K, FIRST, false>(c, s, chunk, nrows, lane, sout, emit);
+    } else {
+        for (uint32_t s = s0; s <= s1; ++s) slice_step<K, FIRST, true>(c, s, chunk, nrows, lane, sout, emit);
+    }
+}
+
+// Per column, the texts with an occurrence and the number of occurrences go to u and v.
+template <int K>
+__device__ __forceinline__ void slice_col_end(const SliceCol<K> &c, const EdSliceArgs &a, const EdSliceItem &I)
+{
+    if (c.report) {
+        if (c.anyh) atomicAdd(a.u + I.uv_base + c.j - 1, (uint32_t)__builtin_popcount(c.anyh));
+        if (c.vcnt) atomicAdd(a.v + I.uv_base + c.j - 1, c.vcnt);
+    }
+}
+
+// Chunks of 64 rows a strip runs through: its steps are 1 .. nrows + 63.
+__device__ __forceinline__ uint32_t slice_chunks(uint32_t nrows) { return (nrows + 63 + 63) / 64; }
+
+// Pipelined form for items that are few and long (kSlicePipe waves per item).  Wave w takes
+// strips w, w + P, w + 2P, ... and all waves advance in ticks of one 64-row chunk, with a
+// workgroup barrier between ticks.  Strip t starts two ticks after strip t - 1 (the rows
+// lane 63 of strip t - 1 emits during its chunks c and c + 1 are the input of strip t's
+// chunk c), and a wave starts its next strip when the previous one is done:
+//     start(t) = (t / P) * Cp + 2 (t % P),  Cp = max(chunks per strip, 2P).
+// Strip t writes boundary buffer t mod (P + 1) and strip t + 1 reads it; the next strip to
+// write that buffer, t + P + 1, starts Cp >= chunks ticks after strip t + 1, when its reads
+// are done.  An item of S strips then takes about (S / P) * Cp ticks instead of S * chunks.
+template <int K>
+__global__ __launch_bounds__(64 * kSlicePipe) void ed_slice_pipe_kernel(EdSliceArgs a)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr uint32_t P = kSlicePipe;
+    __shared__ uint32_t chunks[P][64 * Lay::LW];
+    __shared__ uint32_t item_slot;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t *chunk = chunks[wave];
+    uint32_t *bufs = a.scratch + (size_t)blockIdx.x * (P + 1) * a.stream_words;
+
+    for (;;) {
+        __syncthreads();  // every wave has read item_slot
+        if (threadIdx.x == 0) item_slot = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const uint32_t it = item_slot;
+        if (it >= a.nitems) break;
+        const EdSliceItem I = a.items[it];
+        const uint32_t g0 = a.goff[I.group];
+        const uint32_t nrows = a.goff[I.group + 1] - g0;
+        const uint32_t *gp = a.gplanes + (size_t)g0 * 6;
+        const uint32_t nstrips = (I.m + 63) / 64, nch = slice_chunks(nrows);
+        const uint32_t cp = nch > 2 * P ? nch : 2 * P;
+        const uint32_t ticks = ((nstrips - 1) / P) * cp + 2 * ((nstrips - 1) % P) + nch;
+        SliceCol<K> col;
+        for (uint32_t T = 0; T < ticks; ++T) {
+            // this wave's strip and chunk at tick T (wave-uniform)
+            uint32_t t = 0, ch = 0;
+            bool act = false;
+            if (T >= 2 * wave) {
+                const uint32_t i = (T - 2 * wave) / cp;
+                ch = (T - 2 * wave) - i * cp;
+                t = wave + P * i;
+                act = ch < nch && t < nstrips;
+            }
+            __syncthreads();  // boundary cells of the earlier ticks are written
+            if (act) {
+                const uint32_t *sin = t ? bufs + (size_t)((t - 1) % (P + 1)) * a.stream_words : nullptr;
+                slice_chunk_fill<K>(chunk, 64 * ch, nrows, t > 0, sin, gp, lane);
+            }
+            __syncthreads();  // the chunk is in LDS
+            if (act) {
+                uint32_t *sout = bufs + (size_t)(t % (P + 1)) * a.stream_words;
+                const bool emit = t + 1 < nstrips;
+                if (ch == 0) slice_col_begin<K>(col, a, I, t, lane);
+                if (t == 0) slice_run_chunk<K, true>(col, ch, chunk, nrows, lane, sout, emit);
+                else slice_run_chunk<K, false>(col, ch, chunk, nrows, lane, sout, emit);
+                if (ch + 1 == nch) slice_col_end<K>(col, a, I);
+            }
+        }
+    }
+}
+
 template <int K>
 static uint32_t stream_row_words() { return SliceLayout<K>::SWS; }
 
@@ -214,20 +407,28 @@ uint32_t ed_slice_stream_row_words(uint32_t k)
     }
 }
 
-hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, hipStream_t s)
+template <int K>
+static void launch_slice_k(const EdSliceArgs &a, uint32_t grid, bool pipe, hipStream_t s)
+{
+    if (pipe) hipLaunchKernelGGL(ed_slice_pipe_kernel<K>, dim3(grid), dim3(64 * kSlicePipe), 0, s, a);
+    else hipLaunchKernelGGL(ed_slice_kernel<K>, dim3(grid), dim3(64), 0, s, a);
+}
+
+// grid: workgroups (one wave each, or kSlicePipe waves each when pipe)
+hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, bool pipe, hipStream_t s)
 {
     switch (a.k) {
-    case 4: hipLaunchKernelGGL(ed_slice_kernel<4>, dim3(grid), dim3(64), 0, s, a); break;
-    case 5: hipLaunchKernelGGL(ed_slice_kernel<5>, dim3(grid), dim3(64), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(ed_slice_kernel<6>, dim3(grid), dim3(64), 0, s, a); break;
-    case 7: hipLaunchKernelGGL(ed_slice_kernel<7>, dim3(grid), dim3(64), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(ed_slice_kernel<8>, dim3(grid), dim3(64), 0, s, a); break;
-    case 9: hipLaunchKernelGGL(ed_slice_kernel<9>, dim3(grid), dim3(64), 0, s, a); break;
-    case 10: hipLaunchKernelGGL(ed_slice_kernel<10>, dim3(grid), dim3(64), 0, s, a); break;
-    case 11: hipLaunchKernelGGL(ed_slice_kernel<11>, dim3(grid), dim3(64), 0, s, a); break;
-    case 12: hipLaunchKernelGGL(ed_slice_kernel<12>, dim3(grid), dim3(64), 0, s, a); break;
-    case 13: hipLaunchKernelGGL(ed_slice_kernel<13>, dim3(grid), dim3(64), 0, s, a); break;
-    case 14: hipLaunchKernelGGL(ed_slice_kernel<14>, dim3(grid), dim3(64), 0, s, a); break;
+    case 4: launch_slice_k<4>(a, grid, pipe, s); break;
+    case 5: launch_slice_k<5>(a, grid, pipe, s); break;
+    case 6: launch_slice_k<6>(a, grid, pipe, s); break;
+    case 7: launch_slice_k<7>(a, grid, pipe, s); break;
+    case 8: launch_slice_k<8>(a, grid, pipe, s); break;
+    case 9: launch_slice_k<9>(a, grid, pipe, s); break;
+    case 10: launch_slice_k<10>(a, grid, pipe, s); break;
+    case 11: launch_slice_k<11>(a, grid, pipe, s); break;
+    case 12: launch_slice_k<12>(a, grid, pipe, s); break;
+    case 13: launch_slice_k<13>(a, grid, pipe, s); break;
+    case 14: launch_slice_k<14>(a, grid, pipe, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -153,6 +153,7 @@ struct motex_batch {
     motex::DevBuf d_fbits;           // pair-symmetric Hamming runs: x-by-text occurrence bits
     uint32_t f_words = 0;
     bool ed_slice = false;           // edit runs on the text-parallel kernel
+    bool ed_pipe = false;            // ... in its pipelined form (kSlicePipe waves per item)
     motex::DevBuf d_gplanes, d_goff; // its text groups
     uint32_t ed_slice_stream_words = 0;
     uint64_t work_items() const { return hd.items.size() + ed_nitems; }

@@ -10,7 +10,7 @@
 namespace motex {
 
 hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, hipStream_t s);
-hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, hipStream_t s);
+hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, bool pipe, hipStream_t s);
 uint32_t ed_slice_stream_row_words(uint32_t k);
 uint32_t ed_strip_cols(bool wide);
 
@@ -70,6 +70,21 @@ static uint32_t ed_grid(const motex_batch *b)
     return (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->num_cu * b->tile.waves_per_cu, b->ed_nitems);
 }
 
+// Workgroups of the text-parallel kernel (one wave each, or kSlicePipe when pipelined) and
+// the boundary-stream words they hold (two buffers per wave, or kSlicePipe + 1 per item).
+static uint32_t ed_slice_grid(const motex_batch *b)
+{
+    uint64_t waves = (uint64_t)b->ctx->num_cu * b->tile.waves_per_cu;
+    if (b->grid_cap) waves = std::min<uint64_t>(waves, b->grid_cap);
+    const uint64_t groups = b->ed_pipe ? waves / kSlicePipe : waves;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(groups, b->ed_nitems));
+}
+
+static uint64_t ed_slice_scratch_bytes(const motex_batch *b, uint32_t grid)
+{
+    return (uint64_t)grid * (b->ed_pipe ? kSlicePipe + 1 : 2) * b->ed_slice_stream_words * 4;
+}
+
 // Text-parallel plan (ed_slice_kernels.hip): the texts that can hold an occurrence
 // (length >= k), longest first, in groups of 32; per group row, the 5 bit planes of the 32
 // texts' symbol codes and the mask of texts still running.  Items are (x, group), most
@@ -82,15 +97,14 @@ static bool ed_slice_ok(const motex_batch *b)
     const uint32_t k = b->params.k;
     if (k < 4 || k > 14 || b->ss.sigma > 32 || ed_slice_stream_row_words(k) == 0) return false;
     if (env && env[0] == '1') return true;
-    // An item (x, group of 32 texts) is one wave's sequential sweep of 64-column strips, so
-    // its latency is long; with few items the one-text-per-lane kernel, whose items shrink
-    // to (x, one text) for small inputs, finishes first.  Measured on DNA and protein inputs
-    // of 40-400 sequences of U[100,2000], k 8-14: the text-parallel kernel wins 2-3x at 400
-    // sequences (5,200 items), ties or loses at 200 (1,400 items) and loses 4-10x below.
+    // An item (x, group of 32 texts) is a sequential sweep of 64-column strips, so its
+    // latency is long; with very few items the one-text-per-lane kernel, whose items shrink
+    // to (x, one text), finishes first (40-100 protein sequences: 4.5-9 ms against 10-16 ms
+    // pipelined and 31-51 ms with one wave per item).
     uint64_t nx = 0;
     for (uint32_t i = 0; i < b->ss.num; ++i) nx += b->ss.len[i] >= k;
     const uint64_t items = nx * ((nx + 31) / 32);
-    return items >= (uint64_t)b->ctx->num_cu * 8;
+    return items >= (uint64_t)b->ctx->num_cu * kSliceItemsPerCu;
 }
 
 static int ed_prepare_slice(motex_batch *b)
@@ -132,9 +146,15 @@ static int ed_prepare_slice(motex_batch *b)
         items.swap(sorted);
     }
     b->tile = choose_tile(b->ctx, b->params, ss);
-    b->tile.variant = 2;
     b->ed_slice = true;
     b->ed_nitems = (uint32_t)items.size();
+    // pipelined when the items are too few to hide one item's sequential sweep (200-300
+    // protein sequences: 11-33 ms against 35-58 ms with one wave per item); with many items
+    // one wave per item has the higher throughput (config 3: 44.9 against 69.8 ms)
+    const char *pipe_env = getenv("MOTEX_ED_PIPE");
+    if (pipe_env && (pipe_env[0] == '0' || pipe_env[0] == '1')) b->ed_pipe = pipe_env[0] == '1';
+    else b->ed_pipe = (uint64_t)items.size() < (uint64_t)b->ctx->num_cu * kSlicePipeItemsPerCu;
+    b->tile.variant = b->ed_pipe ? 3 : 2;
     uint32_t rows_max = 0;
     for (uint32_t g = 0; g < ngroups; ++g) rows_max = std::max(rows_max, goff[g + 1] - goff[g]);
     b->ed_slice_stream_words = std::max<uint32_t>(rows_max, 1) * ed_slice_stream_row_words(k);
@@ -153,7 +173,7 @@ static int ed_prepare_slice(motex_batch *b)
     if ((e = hipMemcpyAsync(b->d_goff.p, goff.data(), goff.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail(e, "copy group offsets");
     // boundary streams of the persistent grid, allocated here so that runs allocate nothing
-    const uint64_t need = (uint64_t)std::max<uint32_t>(ed_grid(b), 1) * 2 * b->ed_slice_stream_words * 4;
+    const uint64_t need = ed_slice_scratch_bytes(b, ed_slice_grid(b));
     if ((e = batch_realloc_scratch(b, need)) != hipSuccess) return hip_fail(e, "alloc edit streams");
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "upload items");
     return MOTEX_OK;
@@ -232,10 +252,13 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
 int ed_launch_all(motex_batch *b)
 {
     if (!b->ed_nitems) return MOTEX_OK;
-    uint32_t grid = ed_grid(b);
-    if (b->grid_cap) grid = std::min(grid, b->grid_cap);
-    if (!b->ed_slice) return ed_launch(b, b->d_items.p, b->ed_nitems, grid, 1, bstream(b));
-    if ((uint64_t)grid * 2 * b->ed_slice_stream_words * 4 > b->d_aux.bytes)
+    if (!b->ed_slice) {
+        uint32_t grid = ed_grid(b);
+        if (b->grid_cap) grid = std::min(grid, b->grid_cap);
+        return ed_launch(b, b->d_items.p, b->ed_nitems, grid, 1, bstream(b));
+    }
+    const uint32_t grid = ed_slice_grid(b);
+    if (ed_slice_scratch_bytes(b, grid) > b->d_aux.bytes)
         return set_error(MOTEX_EINVAL, "internal: edit grid of %u beyond its boundary streams", grid);
     EdSliceArgs a{};
     a.codes = (const uint8_t *)b->d_codes.p;
@@ -251,7 +274,7 @@ int ed_launch_all(motex_batch *b)
     a.k = b->params.k;
     a.e = b->params.e;
     a.persistent = 1;
-    hipError_t e = launch_ed_slice(a, grid, bstream(b));
+    hipError_t e = launch_ed_slice(a, grid, b->ed_pipe, bstream(b));
     return e == hipSuccess ? MOTEX_OK : hip_fail(e, "launch text-parallel edit kernel");
 }
 

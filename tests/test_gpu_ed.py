@@ -12,11 +12,13 @@ def rand_seqs(rng, n, lo, hi, alpha):
     return [a[rng.integers(0, len(a), rng.integers(lo, hi + 1))].tobytes() for _ in range(n)]
 
 
-@pytest.fixture(params=["1", "0"], ids=["text_parallel", "one_text_per_lane"])
+@pytest.fixture(params=["1", "pipe", "0"], ids=["text_parallel", "text_parallel_pipelined", "one_text_per_lane"])
 def kernel(request, monkeypatch):
     """Runs a test on the text-parallel edit kernel (32 texts per lane word; used for k in
-    4..14) and again with it switched off."""
-    monkeypatch.setenv("MOTEX_ED_SLICE", request.param)
+    4..14) in its one-wave-per-item and pipelined (several waves per item) forms, and again
+    with it switched off."""
+    monkeypatch.setenv("MOTEX_ED_SLICE", "0" if request.param == "0" else "1")
+    monkeypatch.setenv("MOTEX_ED_PIPE", "1" if request.param == "pipe" else "0")
     return request.param
 
 
@@ -28,8 +30,10 @@ def check(ctx, oracle, seqs, k, e, alphabet="DNA", kernel=None):
     u, v, off = b.download()
     b.close()
     if kernel is not None:
-        sliced = kernel == "1" and 4 <= k <= 14
-        assert (st["kernel_variant"] == 2) == sliced
+        sliced = kernel != "0" and 4 <= k <= 14
+        assert (st["kernel_variant"] in (2, 3)) == sliced
+        if sliced:
+            assert (st["kernel_variant"] == 3) == (kernel == "pipe")
     ou, ov, ooff, _ = oracle.all_pairs(seqs, 1, k, e)
     bad = np.nonzero((u != ou) | (v != ov))[0]
     assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: gpu {u[bad[:5]]},{v[bad[:5]]} oracle {ou[bad[:5]]},{ov[bad[:5]]}"
@@ -42,13 +46,25 @@ def test_config3_subset(ctx, oracle, kernel):
     assert v.sum() > 0
 
 
-def test_text_groups_of_mixed_lengths(ctx, oracle, monkeypatch):
+@pytest.mark.parametrize("pipe", ["0", "1"], ids=["one_wave", "pipelined"])
+def test_text_groups_of_mixed_lengths(ctx, oracle, monkeypatch, pipe):
     # 70 texts: three groups of 32 texts sorted by length, ragged ends, short texts skipped
     # (on the text-parallel kernel, which the default choice keeps for larger inputs)
     monkeypatch.setenv("MOTEX_ED_SLICE", "1")
+    monkeypatch.setenv("MOTEX_ED_PIPE", pipe)
     rng = np.random.default_rng(77)
     seqs = rand_seqs(rng, 70, 2, 900, b"ACGT")
-    check(ctx, oracle, seqs, 7, 2, kernel="1")
+    check(ctx, oracle, seqs, 7, 2, kernel="pipe" if pipe == "1" else "1")
+
+
+def test_pipelined_strips_of_long_x(ctx, oracle, monkeypatch):
+    # x up to 1,400 symbols: up to 22 strips, more than the 4 waves of an item, so waves run
+    # several strips and the boundary buffers are reused
+    monkeypatch.setenv("MOTEX_ED_SLICE", "1")
+    monkeypatch.setenv("MOTEX_ED_PIPE", "1")
+    rng = np.random.default_rng(78)
+    seqs = rand_seqs(rng, 9, 60, 1400, b"ACGT")
+    check(ctx, oracle, seqs, 9, 2, kernel="pipe")
 
 
 @pytest.mark.parametrize("seed,k,e,alpha,n,lo,hi", [
