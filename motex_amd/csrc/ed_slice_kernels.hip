@@ -291,15 +291,16 @@ __device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uin
     }
 }
 
-// The steps of chunk ch: 64 ch + 1 .. 64 ch + 64, up to nrows + 63.  Where every lane has a
-// row, two steps per trip, so the cell planes are not copied between registers on each
-// back edge.
-template <int K, bool FIRST>
+// The steps of chunk ch: 64 ch + 1 .. 64 ch + 64, up to nrows + 63.  With SPLIT, chunks
+// where every lane has a row run two steps per trip without the row check, so the cell
+// planes are not copied between registers on each back edge; without, every step checks
+// (fewer registers: the full-cell form of the first strip).
+template <int K, bool FIRST, bool SPLIT = true>
 __device__ __forceinline__ void slice_run_chunk(SliceCol<K> &c, uint32_t ch, const uint32_t *chunk,
                                                 uint32_t nrows, uint32_t lane, uint32_t *sout, bool emit)
 {
     const uint32_t s0 = 64 * ch + 1, s1 = min(64 * ch + 64, nrows + 63);
-    if (s0 >= 64 && s1 <= nrows) {
+    if (SPLIT && s0 >= 64 && s1 <= nrows) {
         uint32_t s = s0;
         for (; s + 1 <= s1; s += 2) {
             slice_step<K, FIRST, false>(c, s, chunk, nrows, lane, sout, emit);
@@ -358,28 +359,40 @@ __global__ __launch_bounds__(64 * kSlicePipe) void ed_slice_pipe_kernel(EdSliceA
         const uint32_t cp = nch > 2 * P ? nch : 2 * P;
         const uint32_t ticks = ((nstrips - 1) / P) * cp + 2 * ((nstrips - 1) % P) + nch;
         SliceCol<K> col;
-        for (uint32_t T = 0; T < ticks; ++T) {
-            // this wave's strip and chunk at tick T (wave-uniform)
+        uint32_t T = 0;
+        // Strip 0, whose first K columns take the full cell, is wave 0's during ticks
+        // [0, nch); its own loop keeps the full-cell registers out of the loop below.  Every
+        // wave passes two barriers per tick in either loop.
+        if (wave == 0) {
+            slice_col_begin<K>(col, a, I, 0, lane);
+            for (; T < nch; ++T) {
+                __syncthreads();
+                slice_chunk_fill<K>(chunk, 64 * T, nrows, false, nullptr, gp, lane);
+                __syncthreads();
+                slice_run_chunk<K, true, false>(col, T, chunk, nrows, lane, bufs, nstrips > 1);
+            }
+            slice_col_end<K>(col, a, I);
+        }
+        for (; T < ticks; ++T) {
+            // this wave's strip (never strip 0 here) and chunk at tick T, wave-uniform
             uint32_t t = 0, ch = 0;
             bool act = false;
             if (T >= 2 * wave) {
                 const uint32_t i = (T - 2 * wave) / cp;
                 ch = (T - 2 * wave) - i * cp;
                 t = wave + P * i;
-                act = ch < nch && t < nstrips;
+                act = t > 0 && ch < nch && t < nstrips;
             }
             __syncthreads();  // boundary cells of the earlier ticks are written
             if (act) {
-                const uint32_t *sin = t ? bufs + (size_t)((t - 1) % (P + 1)) * a.stream_words : nullptr;
-                slice_chunk_fill<K>(chunk, 64 * ch, nrows, t > 0, sin, gp, lane);
+                const uint32_t *sin = bufs + (size_t)((t - 1) % (P + 1)) * a.stream_words;
+                slice_chunk_fill<K>(chunk, 64 * ch, nrows, true, sin, gp, lane);
             }
             __syncthreads();  // the chunk is in LDS
             if (act) {
                 uint32_t *sout = bufs + (size_t)(t % (P + 1)) * a.stream_words;
-                const bool emit = t + 1 < nstrips;
                 if (ch == 0) slice_col_begin<K>(col, a, I, t, lane);
-                if (t == 0) slice_run_chunk<K, true>(col, ch, chunk, nrows, lane, sout, emit);
-                else slice_run_chunk<K, false>(col, ch, chunk, nrows, lane, sout, emit);
+                slice_run_chunk<K, false>(col, ch, chunk, nrows, lane, sout, t + 1 < nstrips);
                 if (ch + 1 == nch) slice_col_end<K>(col, a, I);
             }
         }
