@@ -19,7 +19,7 @@ static int struct_alloc_scratch(motex_batch *b);
 
 static size_t struct_lds_bytes(const StructArgs &a)
 {
-    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + 1 + kMaxBoxes) * 4;
+    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + 1 + kMaxBoxes + 8 * kMaxBoxes) * 4;
 }
 
 // Fills the shape fields of StructArgs from the parameters.

@@ -100,11 +100,15 @@ __device__ uint32_t struct_walk(const StructArgs &a, uint32_t *const bm[kMaxBoxe
     return total;
 }
 
-// Hamming hits of one box shape for the pair, by the diagonal sweep, into bm.
+// Hamming hits of one box shape for the pair, by the diagonal sweep, into its occurrence
+// list and (write_bm) into bm and bmT.  The sweep runs over blocks of 2048 diagonals and the
+// rows of a block in order, so the list is a sequence of runs sorted by pos_x, one per
+// block; when runs is not null, runs[0] receives their number and runs[1 + r] .. runs[2 + r]
+// bound run r.
 template <int P>
 __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t len, uint32_t err,
                               uint32_t *bm, uint32_t *bmT, uint32_t *ne_l, const uint16_t *xoff,
-                              uint32_t *list, uint32_t *lcount)
+                              uint32_t *list, uint32_t *lcount, bool write_bm, uint32_t *runs)
 {
     const uint32_t lane = threadIdx.x;
     const uint32_t m = I.m, n = I.n, wl = a.wl;
@@ -113,6 +117,8 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
     int64_t q0 = (int64_t)I.y_start - (int64_t)(m - 1);
     q0 = q0 >= 0 ? (q0 & ~31LL) : -(((-q0) + 31) & ~31LL);
     const int64_t qend = (int64_t)I.y_start + n;  // diagonals below this can hold hits
+    uint32_t nruns = 0;
+    if (runs && lane == 0) runs[1] = 0;
     for (int64_t qb = q0; qb < qend; qb += kDiagPerBlock) {
         const uint32_t wbase = (uint32_t)((qb + (int64_t)kPadBits) / 32);
         __syncthreads();
@@ -144,14 +150,19 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
                     h &= h - 1;
                     int64_t pt = qb + 32 * lane + t + px - (int64_t)I.y_start;
                     if (pt + 1 < (int64_t)len || pt >= (int64_t)n) continue;
-                    atomicOr(bm + (size_t)((uint32_t)pt >> 5) * m + px, 1u << ((uint32_t)pt & 31));
-                    if (bmT) atomicOr(bmT + (size_t)(px >> 5) * n + (uint32_t)pt, 1u << (px & 31));
+                    if (write_bm) {
+                        atomicOr(bm + (size_t)((uint32_t)pt >> 5) * m + px, 1u << ((uint32_t)pt & 31));
+                        if (bmT) atomicOr(bmT + (size_t)(px >> 5) * n + (uint32_t)pt, 1u << (px & 31));
+                    }
                     const uint32_t slot = atomicAdd(lcount, 1u);
                     if (slot < a.lcap) list[slot] = (px << 16) | (uint32_t)pt;
                 }
             }
         }
+        ++nruns;
+        if (runs && lane == 0) runs[1 + nruns] = min(*lcount, a.lcap);
     }
+    if (runs && lane == 0) runs[0] = nruns;
 }
 
 template <int P>
@@ -166,6 +177,7 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
     uint32_t *hits = smem + (a.sigma + 1) * wl + (a.m_max + 1) / 2;
     uint32_t *hcount = hits + a.hcap;
     uint32_t *lcount = hcount + 1;  // per box shape: occurrences listed this pair
+    uint32_t *runs = lcount + kMaxBoxes;  // per box shape: sorted runs of its list (8 words)
     for (uint32_t w = lane; w < wl; w += 64) ne_l[a.sigma * wl + w] = 0;
     // scratch: box bitmaps | (Hamming) their transposes | two edit boundary streams |
     //          counted (kx, pos_x) bits | (Hamming) occurrence lists
@@ -190,6 +202,58 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             it = round == 0 ? blockIdx.x : a.nitems;
         }
         if (it >= a.nitems) break;
+        // The merge of functions.cc:647-690 for two boxes from their occurrence lists.  With
+        // box 0 ending at (px0, pt0) and box 1 at (px1, pt1), the structured motif with x-side
+        // gap gx and y-side gap gy occurs when px1 = px0 + gx + len1 and pt1 = pt0 + gy + len1:
+        // every pair of occurrences with both differences in [gmin + len1, gmax + len1] is one
+        // (kx, ky) combination, counted for x at (kx, px0) and, for a pair-symmetric item,
+        // for y at (ky, pt0) (the transposed pair has the same occurrences, coordinates
+        // swapped).  Box-1 occurrences are found by binary search in the sorted runs of its
+        // list; u is deduplicated per (combination, position) with the counted bits.
+        auto pair_lists = [&](const StructItem &I, bool sym) {
+            const uint32_t m = I.m, n = I.n;
+            const uint32_t d0 = a.dist_of[0], d1 = a.dist_of[1];
+            const uint32_t *l0 = lists + (size_t)d0 * a.lcap, *l1 = lists + (size_t)d1 * a.lcap;
+            const uint32_t c0 = __builtin_amdgcn_readfirstlane(lcount[d0]);
+            const uint32_t *r1 = runs + 8 * d1;
+            const uint32_t nr1 = __builtin_amdgcn_readfirstlane(r1[0]);
+            const uint32_t lo = a.gmin[0] + a.blen[1], hi = a.gmax[0] + a.blen[1];
+            for (int orient = 0; orient < (sym ? 2 : 1); ++orient) {
+                const uint32_t mo = orient ? n : m;
+                const uint32_t uvb = orient ? I.uv_base_t : I.uv_base;
+                const uint32_t cw = (a.nb_structs * mo + 31) / 32;
+                uint32_t *cnt_bits = (uint64_t)a.nb_structs * mo <= (uint64_t)a.hcap * 32 ? hits : counted;
+                __syncthreads();
+                for (uint32_t w = lane; w < cw; w += 64) cnt_bits[w] = 0;
+                __threadfence_block();
+                __syncthreads();
+                for (uint32_t h = lane; h < c0; h += 64) {
+                    const uint32_t e0 = l0[h];
+                    const uint32_t px0 = e0 >> 16, pt0 = e0 & 0xFFFFu;
+                    const uint32_t xlo = px0 + lo, xhi = px0 + hi;
+                    for (uint32_t r = 0; r < nr1; ++r) {
+                        uint32_t b = r1[1 + r], en = r1[2 + r];
+                        while (b < en) {  // first entry of the run with pos_x >= xlo
+                            const uint32_t mid = (b + en) >> 1;
+                            if ((l1[mid] >> 16) < xlo) b = mid + 1; else en = mid;
+                        }
+                        for (const uint32_t end = r1[2 + r]; b < end; ++b) {
+                            const uint32_t e1 = l1[b];
+                            const uint32_t px1 = e1 >> 16, pt1 = e1 & 0xFFFFu;
+                            if (px1 > xhi) break;
+                            if (pt1 < pt0 + lo || pt1 > pt0 + hi) continue;
+                            const uint32_t kx = orient ? pt1 - pt0 - lo : px1 - px0 - lo;
+                            const uint32_t idx = kx * mo + (orient ? pt0 : px0), bit = 1u << (idx & 31);
+                            const uint32_t old = atomicOr(cnt_bits + (idx >> 5), bit);
+                            const size_t o = (size_t)uvb + idx;
+                            if (!(old & bit)) atomicAdd(a.u + o, 1u);
+                            atomicAdd(a.v + o, 1u);
+                        }
+                    }
+                }
+            }
+            __threadfence_block();
+        };
         // One ordered pair I (and, when sym, the transposed pair from the same box DPs).
         // Returns true when the transposed pair still has to be run on its own (its
         // occurrences did not fit the lists, or a box passes everywhere).
@@ -216,6 +280,10 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             }
             if (lane < a.ndist) lcount[lane] = 0;
             bool listed = a.distance == 0;  // every shape's occurrences are in its list
+            // Two Hamming boxes, neither passing everywhere: the merge pairs the two boxes'
+            // occurrence lists directly and needs no bitmaps (pair_lists below).
+            bool pairlist = a.distance == 0 && a.nb_gaps == 1;
+            for (uint32_t d = 0; d < a.ndist; ++d) pairlist = pairlist && a.derr[d] < m && a.derr[d] < n;
             // 1. box bitmaps
             for (uint32_t d = 0; d < a.ndist; ++d) {
                 const uint32_t len = a.dlen[d], err = a.derr[d];
@@ -233,7 +301,8 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
                 if (a.distance == 0) {
                     __syncthreads();
                     struct_hd_box<P>(a, I, len, err, out, sym ? bmTd[d] : nullptr, ne_l, xoff,
-                                     lists + (size_t)d * a.lcap, lcount + d);
+                                     lists + (size_t)d * a.lcap, lcount + d, !pairlist,
+                                     pairlist ? runs + 8 * d : nullptr);
                 } else {
                     EdOut eo{out, m};
                     if (len <= 14)
@@ -246,6 +315,25 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             }
             __threadfence_block();
             __syncthreads();
+            if (pairlist) {
+                bool fits = true;
+                for (uint32_t d = 0; d < a.ndist; ++d)
+                    if (__builtin_amdgcn_readfirstlane(lcount[d]) > a.lcap) fits = false;
+                if (fits) {
+                    pair_lists(I, sym);
+                    return false;
+                }
+                // a list overflowed: redo the box DPs with bitmaps for the scan below
+                __syncthreads();
+                if (lane < a.ndist) lcount[lane] = 0;
+                for (uint32_t d = 0; d < a.ndist; ++d) {
+                    __syncthreads();
+                    struct_hd_box<P>(a, I, a.dlen[d], a.derr[d], bmd[d], sym ? bmTd[d] : nullptr, ne_l, xoff,
+                                     lists + (size_t)d * a.lcap, lcount + d, true, nullptr);
+                }
+                __threadfence_block();
+                __syncthreads();
+            }
             // 2. merge: gather box-0 hits word row by word row, then count per x combination
             const uint32_t cwords = (a.nb_structs * m + 31) / 32;
             for (uint32_t w = lane; w < cwords; w += 64) counted[w] = 0;

@@ -56,10 +56,11 @@ def test_reference_boxes_example(ctx, oracle, sym):
     (4, 3, 1, [(1, 2, 2, 500)], b"ACGT"),                    # box error >= m: passes everywhere
     (8, 4, 1, [(1, 3, 4, 60)], b"ACGT"),                     # passes everywhere in some pairs only
     (9, 3, 0, [(0, 1, 3, 0)], b"A"),                         # one symbol: lists overflow, bitmap scan
+    (10, 5, 1, [(4, 15, 6, 1)], b"ACG"),                     # x + y over 2048: lists in several runs
 ])
 def test_hamming_shapes(ctx, oracle, sym, seed, k, e, boxes, alpha):
     rng = np.random.default_rng(seed)
-    lo, hi = (40, 120) if seed == 8 else ((60, 400) if seed == 9 else (10, 200))
+    lo, hi = {8: (40, 120), 9: (60, 400), 10: (900, 2600)}.get(seed, (10, 200))
     check(ctx, oracle, rand_seqs(rng, 7, lo, hi, alpha), k, e, boxes, sym=sym)
 
 
