@@ -146,8 +146,10 @@ struct StructArgs {
     uint32_t ndist;                             // distinct box shapes
     uint32_t dist_of[kMaxBoxes];                // box b -> distinct shape
     uint32_t dlen[kMaxBoxes], derr[kMaxBoxes];
-    uint32_t m_max, hcap;
+    uint32_t m_max, hcap;                       // hcap: LDS words of the hit list / counted bits
     uint32_t lcap;                              // Hamming boxes: occurrence list entries per shape
+    uint32_t gcap, gather_scratch;              // gathered box-0 hits: entries, and whether they
+                                                // live in scratch (after the lists) or in LDS
 };
 
 }  // namespace motex

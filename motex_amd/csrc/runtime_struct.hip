@@ -13,6 +13,10 @@ namespace motex {
 hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStream_t s);
 
 static const uint32_t kStructHitCap = 2048;  // >= one chunk: 64 lanes x 32 bits
+// Two Hamming boxes merge from their occurrence lists, which needs only the counted bits in
+// LDS; the gather list of the fallback path then lives in scratch, and the smaller LDS
+// footprint lets more waves run (config 4A: about 14 -> 27 waves per CU).
+static const uint32_t kStructPairListLds = 512;
 static const uint32_t kStructListCap = 8192; // Hamming occurrences listed per pair and box shape
 
 static int struct_alloc_scratch(motex_batch *b);
@@ -158,13 +162,17 @@ static int struct_fill_args(motex_batch *b, StructArgs &a)
     a.sigma = ss.sigma;
     a.wl = b->params.distance == 0 ? b->hd.wl : 0;
     a.m_max = std::max<uint32_t>(maxlen, 1);
-    a.hcap = kStructHitCap;
+    const bool pair_lists = b->params.distance == 0 && b->params.nb_gaps == 1;
+    a.hcap = pair_lists ? kStructPairListLds : kStructHitCap;
+    a.gcap = kStructHitCap;
+    a.gather_scratch = pair_lists ? 1u : 0u;
     a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
     a.stream_words = (maxlen + 128) * 2;
     a.lcap = b->params.distance == 0 ? kStructListCap : 0;
-    // bitmaps (Hamming: and their transposes), edit streams, counted bits, lists
+    // bitmaps (Hamming: and their transposes), edit streams, counted bits, lists, gather list
     a.scratch_words = (b->params.distance == 0 ? 2ull : 1ull) * a.ndist * a.bm_words + 2ull * a.stream_words +
-                      ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1 + (uint64_t)a.ndist * a.lcap;
+                      ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1 + (uint64_t)a.ndist * a.lcap +
+                      (a.gather_scratch ? a.gcap : 0u);
     return MOTEX_OK;
 }
 

@@ -187,6 +187,8 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
     uint32_t *buf1 = buf0 + a.stream_words;
     uint32_t *counted = buf1 + a.stream_words;
     uint32_t *lists = counted + ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1;
+    // gathered box-0 hits of the bitmap-scan path: in LDS, or after the lists in scratch
+    uint32_t *gather = a.gather_scratch ? lists + (size_t)a.ndist * a.lcap : hits;
     // Hamming boxes: the bitmaps are all zero between pairs (each pair clears the words it
     // set, through the occurrence lists, or all of them when a list overflowed); the first
     // pair of a launch clears them in full.
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             auto walk_list = [&](uint32_t have) {
                 for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
                     for (uint32_t h = lane; h < have; h += 64) {
-                        uint32_t e = hits[h];
+                        uint32_t e = gather[h];
                         uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
                         uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
                         if (c) {
@@ -439,11 +441,11 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
                     while (word) {
                         uint32_t t = __builtin_ctz(word);
                         word &= word - 1;
-                        hits[off++] = (px << 16) | (32 * w + t);
+                        gather[off++] = (px << 16) | (32 * w + t);
                     }
                 }
             };
-            if (total <= a.hcap) {
+            if (total <= a.gcap) {
                 if (total) {
                     for (uint32_t w = 0; w < wn; ++w)
                         for (uint32_t px0 = 0; px0 < m; px0 += 64) append(w, px0 + lane);
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             for (uint32_t wc = 0; wc <= wn * chunks; ++wc) {
                 // walk before a chunk that could overflow the list, and at the end
                 uint32_t have = __builtin_amdgcn_readfirstlane(*hcount);
-                bool flush = (wc == wn * chunks) ? have > 0 : have + 64 * 32 > a.hcap;
+                bool flush = (wc == wn * chunks) ? have > 0 : have + 64 * 32 > a.gcap;
                 if (flush) {
                     walk_list(have);
                     __syncthreads();
