@@ -37,7 +37,8 @@ extern "C" {
 #define MOTEX_EINVAL (-1)    /* invalid argument (the reference's validation, motex.cc:98-201) */
 #define MOTEX_ENOMEM (-2)    /* host or device allocation failed, or over the HBM budget */
 #define MOTEX_EDEVICE (-3)   /* HIP runtime error */
-#define MOTEX_EUNSUPPORTED (-4) /* parameters outside what the kernels implement (k > 64 ...) */
+#define MOTEX_EUNSUPPORTED (-4) /* parameters outside what the kernels implement (Hamming k > 1024,
+                                  structured boxes > 64, edit k > 32, ...) */
 #define MOTEX_EBUSY (-5)     /* serving: job not finished yet */
 #define MOTEX_ENOTFOUND (-6) /* serving: unknown job id */
 

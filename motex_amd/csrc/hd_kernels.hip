@@ -226,7 +226,8 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
 template <int B, int P, int PH>
 constexpr int hd_waves_per_eu()
 {
-    if constexpr (B == 4) return P <= 4 ? 6 : 1;
+    if constexpr (PH < 0) return 1;
+    else if constexpr (B == 4) return P <= 4 ? 6 : 1;
     else if constexpr (B == 2) return P + PH <= 11 ? 8 : 1;
     else return 8;
 }
@@ -280,6 +281,11 @@ void hd_batched_kernel(HdArgs a)
         }
         const uint32_t rep0 = I.r0 > a.k - 1 ? I.r0 : a.k - 1;
         const uint32_t bias = hd_bias(a.e);
+        // PH < 0 (factors longer than 64): the threshold plane is chosen at run time
+        uint32_t hmask[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+            hmask[p] = __builtin_amdgcn_readfirstlane((uint32_t)p >= hd_low_planes(a.e) ? ~0u : 0u);
         for (uint32_t ri = I.rb; ri < I.re; ++ri) {
             const HdRange R = a.ranges[ri];
             // first diagonal (pt - px) of this range, a multiple of 32
@@ -335,7 +341,14 @@ void hd_batched_kernel(HdArgs a)
                     } else {
                         planes_inc<P>(cnt[b], an);
                     }
-                    high[b] = planes_high<P, PH>(cnt[b]);
+                    if constexpr (PH >= 0) {
+                        high[b] = planes_high<P, PH>(cnt[b]);
+                    } else {
+                        uint32_t h = 0;
+#pragma unroll
+                        for (int p = 0; p < P; ++p) h |= cnt[b][p] & hmask[p];
+                        high[b] = h;
+                    }
                     all &= high[b];
                 }
                 if (px >= rep0) {
@@ -463,13 +476,17 @@ static hipError_t launch_ph(const HdArgs &a, uint32_t grid, size_t lds, hipStrea
 template <int B>
 static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream_t s)
 {
-    // the biased counter holds bias(e) + k: 2k at most, so 7 planes for k <= 64
+    // the biased counter holds bias(e) + k: 2k at most, so 7 planes for k <= 64 and 11 for
+    // k <= 1024 (one generic single-block kernel with the threshold plane set at run time)
     switch (hd_counter_planes(a.k, a.e)) {
     case 0: case 1: case 2: case 3: return launch_ph<B, 3>(a, grid, lds, s);
     case 4: return launch_ph<B, 4>(a, grid, lds, s);
     case 5: return launch_ph<B, 5>(a, grid, lds, s);
     case 6: return launch_ph<B, 6>(a, grid, lds, s);
     case 7: return launch_ph<B, 7>(a, grid, lds, s);
+    case 8: case 9: case 10: case 11:
+        if constexpr (B == 1) return launch_bp<1, 11, -1>(a, grid, lds, s);
+        else return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 }

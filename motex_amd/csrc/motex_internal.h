@@ -8,7 +8,8 @@ namespace motex {
 // Positions before 0 and after the end are padding whose bits are never reported.
 constexpr uint32_t kPadBits = 4096 + 2048;
 constexpr uint32_t kDiagPerBlock = 2048;     // 64 lanes x 32 diagonals
-constexpr uint32_t kMaxHdK = 64;             // Hamming kernels: counters of up to 7 planes
+constexpr uint32_t kMaxHdK = 64;             // structured Hamming boxes: counters of up to 7 planes
+constexpr uint32_t kMaxHdLongK = 1024;       // Hamming factors: 11 counter planes, single-block tiles
 constexpr uint32_t kMaxEdK = 32;             // edit kernels: bit-vectors stay below 64 bits
 constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
 constexpr uint32_t kMaxRangeSeqs = 32;       // sequences per Hamming text range (one mask word)

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "hd_device.h"
 #include "runtime.h"
 
 namespace motex {
@@ -73,6 +74,12 @@ TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss)
             t.rows = r;
             t.waves_per_cu = w;
         }
+    }
+    if (!p.nb_gaps && p.distance == 0 && hd_counter_planes(p.k, p.e) > 7) {
+        // factors longer than 64: the generic single-block kernel, with at least k rows per
+        // item so that the k - 1 warm-up rows are at most half of them
+        t.blocks = 1;
+        t.rows = std::max<uint32_t>(t.rows, (p.k + 63) & ~63u);
     }
     // keep the Hamming kernel's LDS window under 64 KB
     while (t.blocks > 1 && !hd_fits(ss, p.k, t.blocks, t.rows)) t.blocks /= 2;

@@ -94,10 +94,30 @@ typedef void (*hit_fn)(void *ctx, uint32_t pos_t, uint32_t pos_x, uint32_t err);
 
 /* MaxShift Hamming, functions.cc:289-302: D[i][j] = ((D[i-1][j-1] & y) << 1) | delta.
  * Reports are restricted to complete windows on both sides (libflasm semantics). */
+/* Factors longer than one 64-bit word (libflasm's multi-word "Limit", libflasm.h:44-54):
+ * the same full-window Hamming hits, from a sliding mismatch count along each diagonal
+ * d = pos_t - pos_x. */
+static int hd_hits_long(const unsigned char *x, uint32_t m, const unsigned char *t, uint32_t n,
+                        uint32_t ell, uint32_t err, hit_fn fn, void *ctx)
+{
+    for (int64_t d = -(int64_t)m + 1; d < (int64_t)n; ++d) {
+        uint32_t cnt = 0;
+        int64_t j = d < 0 ? -d : 0; /* pos_x of the diagonal's first cell */
+        for (int64_t i = j + d; i < (int64_t)n && j < (int64_t)m; ++i, ++j) {
+            cnt += t[i] != x[j];
+            if (i >= (int64_t)ell && j >= (int64_t)ell) cnt -= t[i - ell] != x[j - ell];
+            if (j + 1 >= (int64_t)ell && i + 1 >= (int64_t)ell && cnt <= err)
+                fn(ctx, (uint32_t)i, (uint32_t)j, cnt);
+        }
+    }
+    return 0;
+}
+
 static int hd_hits(const unsigned char *x, uint32_t m, const unsigned char *t, uint32_t n,
                    uint32_t ell, uint32_t err, hit_fn fn, void *ctx)
 {
-    if (ell == 0 || ell > 64) return -1;
+    if (ell == 0) return -1;
+    if (ell > 64) return hd_hits_long(x, m, t, n, ell, err, fn, ctx);
     word_t ymask = ones(ell - 1);
     word_t *D = (word_t *)malloc((size_t)(m + 1) * sizeof(word_t));
     if (!D) return -2;
@@ -170,7 +190,7 @@ static void fold_hit(void *c, uint32_t pos_t, uint32_t pos_x, uint32_t err)
 }
 
 /* distance: 0 Hamming, 1 edit.  Accumulates (+=) into u[0..m), v[0..m).  Returns 0 on
- * success, negative on failure (k outside 1..64 or allocation). */
+ * success, negative on failure (k = 0, edit distance with k > 64, or allocation). */
 int orc_pair(int distance, const char *x, uint32_t m, const char *y, uint32_t n,
              uint32_t k, uint32_t e, uint32_t *u, uint32_t *v)
 {

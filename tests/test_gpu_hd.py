@@ -69,11 +69,16 @@ def test_symmetric_is_the_default_for_rare_occurrences(ctx):
     (7, 64, 20, b"AC", 5, 60, 300),        # k = 64 (seven planes)
     (8, 10, 2, b"ARNDCQEGHILKMFPSTWYV", 10, 100, 900),  # protein alphabet
     (9, 6, 1, b"0123456789", 10, 50, 300),  # USR alphabet
+    (10, 65, 20, b"AC", 5, 60, 300),       # beyond 64: generic single-block kernel
+    (11, 200, 70, b"AC", 4, 150, 600),     # eleven-plane counters, long warm-up rows
+    (12, 1024, 400, b"AC", 3, 1000, 1600),  # longest factor
 ])
 def test_random_edges(ctx, oracle, sym, seed, k, e, alpha, n, lo, hi):
     rng = np.random.default_rng(seed)
     alphabet = "PROT" if len(alpha) == 20 else ("USR" if alpha[0:1] == b"0" else "DNA")
-    check(ctx, oracle, rand_seqs(rng, n, lo, hi, alpha), k, e, alphabet, sym=sym)
+    # k = 1024 items have 1024 rows, so texts over 2048 - 1024 - 32 symbols are split across
+    # items, which rules out the pair-symmetric plan even when it is asked for
+    check(ctx, oracle, rand_seqs(rng, n, lo, hi, alpha), k, e, alphabet, sym=None if k >= 1024 else sym)
 
 
 def test_long_sequence_split(ctx, oracle):

@@ -49,6 +49,7 @@ def rand_seqs(rng, n, lo, hi, alpha):
 @pytest.mark.parametrize("seed,k,e,alpha", [
     (1, 5, 1, b"ACGT"), (2, 8, 2, b"ACGTN"), (3, 3, 0, b"AC"), (4, 12, 3, b"ACGT"),
     (5, 1, 0, b"ACGT"), (6, 20, 6, b"ARNDCQEGHILKMFPSTWYV"), (7, 64, 20, b"AC"),
+    (8, 65, 20, b"AC"), (9, 130, 45, b"AC"),   # beyond one 64-bit word: sliding window count
 ])
 def test_hd_matches_brute_force(oracle, seed, k, e, alpha):
     rng = np.random.default_rng(seed)
