@@ -119,6 +119,9 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
     const int64_t qend = (int64_t)I.y_start + n;  // diagonals below this can hold hits
     uint32_t nruns = 0;
     if (runs && lane == 0) runs[1] = 0;
+    // list entries so far: in list-only mode a wave-uniform register (slots from ballots, no
+    // LDS atomic per occurrence); with bitmaps, the LDS counter
+    uint32_t nl = __builtin_amdgcn_readfirstlane(*lcount);
     for (int64_t qb = q0; qb < qend; qb += kDiagPerBlock) {
         const uint32_t wbase = (uint32_t)((qb + (int64_t)kPadBits) / 32);
         __syncthreads();
@@ -144,25 +147,46 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
             planes_slide<P>(cnt, an, bo);
             const uint32_t res = ~planes_high_rt<P>(cnt, ph);
             if (px + 1 >= len && __ballot(res != 0)) {
-                uint32_t h = res;
-                while (h) {
-                    uint32_t t = __builtin_ctz(h);
-                    h &= h - 1;
-                    int64_t pt = qb + 32 * lane + t + px - (int64_t)I.y_start;
-                    if (pt + 1 < (int64_t)len || pt >= (int64_t)n) continue;
-                    if (write_bm) {
-                        atomicOr(bm + (size_t)((uint32_t)pt >> 5) * m + px, 1u << ((uint32_t)pt & 31));
-                        if (bmT) atomicOr(bmT + (size_t)(px >> 5) * n + (uint32_t)pt, 1u << (px & 31));
+                // flag t of this lane is text position pt = base + t; only [len - 1, n) counts
+                const int64_t base = qb + 32 * lane + px - (int64_t)I.y_start;
+                const int64_t tlo = (int64_t)len - 1 - base, thi = (int64_t)n - base;
+                uint32_t hv = res;
+                if (tlo > 0) hv = tlo >= 32 ? 0u : hv & (~0u << (uint32_t)tlo);
+                if (thi < 32) hv = thi <= 0 ? 0u : hv & ((1u << (uint32_t)thi) - 1u);
+                if (write_bm) {
+                    while (hv) {
+                        const uint32_t t = __builtin_ctz(hv);
+                        hv &= hv - 1;
+                        const uint32_t pt = (uint32_t)(base + t);
+                        atomicOr(bm + (size_t)(pt >> 5) * m + px, 1u << (pt & 31));
+                        if (bmT) atomicOr(bmT + (size_t)(px >> 5) * n + pt, 1u << (px & 31));
+                        const uint32_t slot = atomicAdd(lcount, 1u);
+                        if (slot < a.lcap) list[slot] = (px << 16) | pt;
                     }
-                    const uint32_t slot = atomicAdd(lcount, 1u);
-                    if (slot < a.lcap) list[slot] = (px << 16) | (uint32_t)pt;
+                } else {
+                    // one occurrence per lane per round; slots from the round's ballot
+                    for (;;) {
+                        const uint64_t mb = __ballot(hv != 0);
+                        if (!mb) break;
+                        if (hv) {
+                            const uint32_t t = __builtin_ctz(hv);
+                            hv &= hv - 1;
+                            const uint32_t slot =
+                                nl + __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
+                            if (slot < a.lcap) list[slot] = (px << 16) | (uint32_t)(base + t);
+                        }
+                        nl += (uint32_t)__builtin_popcountll(mb);
+                    }
                 }
             }
         }
         ++nruns;
-        if (runs && lane == 0) runs[1 + nruns] = min(*lcount, a.lcap);
+        if (write_bm) nl = __builtin_amdgcn_readfirstlane(*lcount);
+        if (runs && lane == 0) runs[1 + nruns] = min(nl, a.lcap);
     }
     if (runs && lane == 0) runs[0] = nruns;
+    if (!write_bm && lane == 0) *lcount = nl;
 }
 
 template <int P>
