@@ -23,7 +23,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{os.path.join(ROOT, 'include')}", f"
 # Packed FP32 instructions are kept out of every device code object.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["hd_kernels.hip", "ed_kernels.hip", "ed_slice_kernels.hip", "struct_kernels.hip", "runtime.hip",
+LIB_SOURCES = ["hd_kernels.hip", "ed_kernels.hip", "ed_slice_kernels.hip", "ed_slice_kernels_k15.hip",
+               "ed_slice_kernels_k23.hip", "struct_kernels.hip", "runtime.hip",
                "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip"]
 CLI_SOURCES = ["cli/main.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/switches.cpp"]
 # Stand-alone micro-benchmarks that calibrate the roofline (motex_amd/bin/<name>).

@@ -11,6 +11,9 @@ constexpr uint32_t kDiagPerBlock = 2048;     // 64 lanes x 32 diagonals
 constexpr uint32_t kMaxHdK = 64;             // structured Hamming boxes: counters of up to 7 planes
 constexpr uint32_t kMaxHdLongK = 1024;       // Hamming factors: 11 counter planes, single-block tiles
 constexpr uint32_t kMaxEdK = 32;             // edit kernels: bit-vectors stay below 64 bits
+// Text-parallel edit kernel: k up to 30, so that a candidate's popcount (at most k + 1) fits
+// its 5 bit-sliced planes.
+constexpr uint32_t kMaxSliceK = 30;
 constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
 constexpr uint32_t kMaxRangeSeqs = 32;       // sequences per Hamming text range (one mask word)
 constexpr uint32_t kHitQueueCap = 512;       // Hamming hit queue entries per wave (global)

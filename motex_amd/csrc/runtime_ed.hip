@@ -85,17 +85,17 @@ static uint64_t ed_slice_scratch_bytes(const motex_batch *b, uint32_t grid)
     return (uint64_t)grid * (b->ed_pipe ? kSlicePipe + 1 : 2) * b->ed_slice_stream_words * 4;
 }
 
-// Text-parallel plan (ed_slice_kernels.hip): the texts that can hold an occurrence
+// Text-parallel plan (ed_slice_kernels.h): the texts that can hold an occurrence
 // (length >= k), longest first, in groups of 32; per group row, the 5 bit planes of the 32
 // texts' symbol codes and the mask of texts still running.  Items are (x, group), most
-// expensive first.  Used for k in 4..14 and alphabets of at most 32 symbols (5 planes);
+// expensive first.  Used for k in 4..kMaxSliceK and alphabets of at most 32 symbols (5 planes);
 // MOTEX_ED_SLICE=0 keeps the one-text-per-lane kernel.
 static bool ed_slice_ok(const motex_batch *b)
 {
     const char *env = getenv("MOTEX_ED_SLICE");
     if (env && env[0] == '0') return false;
     const uint32_t k = b->params.k;
-    if (k < 4 || k > 14 || b->ss.sigma > 32 || ed_slice_stream_row_words(k) == 0) return false;
+    if (k < 4 || k > kMaxSliceK || b->ss.sigma > 32 || ed_slice_stream_row_words(k) == 0) return false;
     if (env && env[0] == '1') return true;
     // An item (x, group of 32 texts) is a sequential sweep of 64-column strips, so its
     // latency is long; with very few items the one-text-per-lane kernel, whose items shrink

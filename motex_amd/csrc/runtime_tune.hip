@@ -40,9 +40,10 @@ static TileShape default_tile(const motex_params &p, const SeqSet &ss)
         t.blocks = ss.sigma <= 8 ? 4 : (ss.sigma <= 24 ? 2 : 1);
         t.rows = 128;
         t.waves_per_cu = 32;
-    } else if (!p.nb_gaps && p.k >= 4 && p.k <= 14 && ss.sigma <= 32) {
-        // text-parallel edit kernel: 130-230 VGPRs, so 2-3 waves per SIMD are resident
-        t.waves_per_cu = 12;
+    } else if (!p.nb_gaps && p.k >= 4 && p.k <= kMaxSliceK && ss.sigma <= 32) {
+        // text-parallel edit kernel: 100-200 VGPRs up to k = 14 (2-4 waves per SIMD
+        // resident), 2 waves per SIMD beyond
+        t.waves_per_cu = p.k <= 14 ? 12 : 8;
     } else {
         t.blocks = 4;
         t.rows = 256;

@@ -54,7 +54,7 @@ static int run(uint32_t e, uint32_t m, uint32_t nmin, uint32_t nmax, uint32_t si
         else slice_row0<K>(j, prev[j]);
     }
     std::vector<uint32_t> anyhit(m, 0), su(m, 0), sv(m, 0);
-    int wide = 0, masked_bad = 0;
+    int wide = 0, masked_bad = 0, pc_bad = 0;
     for (uint32_t i = 1; i <= nlong; ++i) {
         uint32_t valid = 0;
         for (uint32_t t = 0; t < nt; ++t)
@@ -69,6 +69,14 @@ static int run(uint32_t e, uint32_t m, uint32_t nmin, uint32_t nmax, uint32_t si
             uint32_t dropped = 0;
             slice_cell<K>(prev[j], cur[j - 1], prev[j - 1], delta, full, cur[j], &dropped);
             wide |= (dropped & valid) != 0;
+            // the carried popcount planes must equal the word's popcount, which stays <= K
+            for (uint32_t t = 0; t < nt; ++t) {
+                if (!((valid >> t) & 1)) continue;
+                uint32_t cnt = 0, held = 0;
+                for (int p = 0; p < SliceW<K>::W; ++p) cnt += (cur[j].d[p] >> t) & 1;
+                for (int q = 0; q < kSlicePc; ++q) held |= ((cur[j].pc[q] >> t) & 1) << q;
+                pc_bad |= cnt != held || cnt > (uint32_t)K;
+            }
             if (j > (uint32_t)K) {
                 // the truncated-column form must give the same word and popcounts
                 Slice<K> alt;
@@ -97,9 +105,10 @@ static int run(uint32_t e, uint32_t m, uint32_t nmin, uint32_t nmax, uint32_t si
         }
     uint64_t tv = 0;
     for (uint32_t j = 0; j < m; ++j) tv += rv[j];
-    printf("k=%d e=%u m=%u texts=%u mismatched_columns=%d occurrences=%llu wide_state=%d masked_form_differs=%d\n",
-           K, e, m, nt, bad, (unsigned long long)tv, wide, masked_bad);
-    return bad || wide || masked_bad ? 1 : 0;
+    printf("k=%d e=%u m=%u texts=%u mismatched_columns=%d occurrences=%llu wide_state=%d masked_form_differs=%d "
+           "popcount_differs=%d\n",
+           K, e, m, nt, bad, (unsigned long long)tv, wide, masked_bad, pc_bad);
+    return bad || wide || masked_bad || pc_bad ? 1 : 0;
 }
 
 // The device build computes slice_mux, slice_and, slice_gt_step and slice_lt_step with
@@ -158,6 +167,9 @@ int main(int argc, char **argv)
     case 10: return run<10>(e, m, nmin, nmax, sigma, nt);
     case 12: return run<12>(e, m, nmin, nmax, sigma, nt);
     case 14: return run<14>(e, m, nmin, nmax, sigma, nt);
+    case 15: return run<15>(e, m, nmin, nmax, sigma, nt);
+    case 20: return run<20>(e, m, nmin, nmax, sigma, nt);
+    case 30: return run<30>(e, m, nmin, nmax, sigma, nt);
     default: return 2;
     }
 }

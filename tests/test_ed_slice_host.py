@@ -28,7 +28,7 @@ CASES = [
     (2, 5, 2, 40, 5, 60, 4, 32),       # texts shorter than k mixed in
     (3, 3, 0, 30, 1, 40, 2, 17),       # e = 0, partial group
     (4, 12, 3, 50, 10, 90, 4, 32),
-    (5, 14, 5, 40, 14, 60, 2, 32),     # largest k of the 32-bit path
+    (5, 14, 5, 40, 14, 60, 2, 32),     # largest k of the 32-bit word path
     (6, 1, 0, 20, 1, 30, 3, 8),        # k = 1
     (7, 10, 2, 100, 50, 200, 20, 32),
     (8, 14, 13, 30, 14, 80, 1, 32),    # one symbol: widest words, most ties
@@ -38,6 +38,12 @@ CASES = [
     (12, 14, 0, 40, 14, 70, 1, 32),
     (13, 4, 3, 40, 4, 90, 1, 32),
     (14, 6, 2, 40, 6, 90, 2, 32),
+    (15, 15, 4, 60, 15, 120, 4, 32),
+    (16, 20, 6, 80, 20, 160, 4, 32),
+    (17, 20, 19, 40, 20, 90, 1, 32),   # one symbol, e = k - 1
+    (18, 30, 8, 90, 30, 200, 4, 32),   # largest k of the text-parallel path
+    (19, 30, 29, 60, 30, 120, 1, 32),  # one symbol: widest words, popcount up to k + 1
+    (20, 30, 12, 70, 30, 140, 20, 24),
 ]
 
 

@@ -15,7 +15,7 @@ def rand_seqs(rng, n, lo, hi, alpha):
 @pytest.fixture(params=["1", "pipe", "0"], ids=["text_parallel", "text_parallel_pipelined", "one_text_per_lane"])
 def kernel(request, monkeypatch):
     """Runs a test on the text-parallel edit kernel (32 texts per lane word; used for k in
-    4..14) in its one-wave-per-item and pipelined (several waves per item) forms, and again
+    4..30) in its one-wave-per-item and pipelined (several waves per item) forms, and again
     with it switched off."""
     monkeypatch.setenv("MOTEX_ED_SLICE", "0" if request.param == "0" else "1")
     monkeypatch.setenv("MOTEX_ED_PIPE", "1" if request.param == "pipe" else "0")
@@ -30,7 +30,7 @@ def check(ctx, oracle, seqs, k, e, alphabet="DNA", kernel=None):
     u, v, off = b.download()
     b.close()
     if kernel is not None:
-        sliced = kernel != "0" and 4 <= k <= 14
+        sliced = kernel != "0" and 4 <= k <= 30
         assert (st["kernel_variant"] in (2, 3)) == sliced
         if sliced:
             assert (st["kernel_variant"] == 3) == (kernel == "pipe")
@@ -74,7 +74,10 @@ def test_pipelined_strips_of_long_x(ctx, oracle, monkeypatch):
     (4, 14, 3, b"ACGT", 6, 100, 600),      # widest 32-bit word case
     (5, 12, 0, b"AC", 6, 100, 300),        # e = 0
     (6, 20, 4, b"ACGT", 5, 100, 400),      # 64-bit words
-    (7, 32, 6, b"AC", 4, 100, 300),        # k = 32
+    (7, 32, 6, b"AC", 4, 100, 300),        # k = 32 (one-text-per-lane kernel only)
+    (11, 15, 3, b"ACGT", 40, 10, 500),     # text-parallel kernel past 32-bit words, two groups
+    (12, 23, 22, b"A", 5, 20, 200),        # one symbol, e = k - 1: first k with AGPR state
+    (13, 30, 7, b"AC", 36, 30, 300),       # largest k of the text-parallel kernel
     (8, 3, 2, b"ACGT", 8, 2, 70),          # e = k-1, texts shorter than one wave
     (9, 10, 9, b"A", 5, 10, 200),          # one symbol: widest words, every tie
     (10, 4, 1, b"ARNDCQEGHILKMFPSTWYV", 40, 4, 300),  # protein, more than one text group
