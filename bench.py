@@ -38,7 +38,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="2", choices=["1", "2", "3", "4a", "4b"])
+    ap.add_argument("--workload", default="dna1200", choices=["dna50", "dna1200", "prot500", "struct1000", "long4m"],
+                    help="dna50: 50 x 500 nt Hamming k=10 e=1; dna1200: 1200 x 1000 nt Hamming k=12 "
+                         "e=2 (the metric's workload); prot500: 500 proteins of U[200,2000], edit k=8 "
+                         "e=1; struct1000: 1000 x 600 nt, structured 8+[10,30]+8; long4m: one 4 Mnt "
+                         "sequence, Hamming k=10 e=2")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--plain-x", type=int, default=4,
                     help="x sequences timed in the plain one-launch-per-pair form")
@@ -136,7 +140,7 @@ def main():
     import motex_amd
     from motex_amd import MotexParams, synth
 
-    w = synth.CONFIGS[a.config](a.scale)
+    w = synth.WORKLOADS[a.workload](a.scale)
     params = MotexParams(k=w.k, e=w.e, distance=w.distance, alphabet=w.alphabet,
                          boxes=list(w.boxes), long_mode=w.long_mode)
     ctx = motex_amd.Context(0)
@@ -181,7 +185,7 @@ def main():
     kernel = ("struct_kernel" if w.boxes else
               "hd_batched_kernel" if w.distance == 0 else
               {2: "ed_slice_kernel", 3: "ed_slice_pipe_kernel"}.get(st["kernel_variant"], "ed_kernel"))
-    tr = traffic_from_profile(kernel, f"config{a.config}") if a.scale == 1.0 else None
+    tr = traffic_from_profile(kernel, a.workload) if a.scale == 1.0 else None
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_INT32_PEAK_TOPS,
                 "unit": "TOP/s", "frac": round(achieved / VALU_INT32_PEAK_TOPS, 4),
                 "traffic": tr["bytes"] if tr else None,

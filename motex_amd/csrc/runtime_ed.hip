@@ -150,7 +150,7 @@ static int ed_prepare_slice(motex_batch *b)
     b->ed_nitems = (uint32_t)items.size();
     // pipelined when the items are too few to hide one item's sequential sweep (200-300
     // protein sequences: 11-33 ms against 35-58 ms with one wave per item); with many items
-    // one wave per item has the higher throughput (config 3: 44.9 against 69.8 ms)
+    // one wave per item has the higher throughput (500 proteins of U[200,2000], k=8: 44.9 against 69.8 ms)
     const char *pipe_env = getenv("MOTEX_ED_PIPE");
     if (pipe_env && (pipe_env[0] == '0' || pipe_env[0] == '1')) b->ed_pipe = pipe_env[0] == '1';
     else b->ed_pipe = (uint64_t)items.size() < (uint64_t)b->ctx->num_cu * kSlicePipeItemsPerCu;

@@ -15,7 +15,7 @@ hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStre
 static const uint32_t kStructHitCap = 2048;  // >= one chunk: 64 lanes x 32 bits
 // Two Hamming boxes merge from their occurrence lists, which needs only the counted bits in
 // LDS; the gather list of the fallback path then lives in scratch, and the smaller LDS
-// footprint lets more waves run (config 4A: about 14 -> 27 waves per CU).
+// footprint lets more waves run (1000 x 600 nt, boxes 8+[10,30]+8: about 14 -> 27 waves per CU).
 static const uint32_t kStructPairListLds = 512;
 static const uint32_t kStructListCap = 8192; // Hamming occurrences listed per pair and box shape
 

@@ -1,16 +1,16 @@
-"""Seeded synthetic inputs for the five workload configurations.
+"""Seeded synthetic inputs for the five benchmark workloads.
 
 Every byte is produced from SplitMix64 with integer-threshold sampling only (no floating
-point in the sampled data), so a given (config, seed) yields identical sequences on every
-machine.  The base seed is ``0x4D6F54655800 + config_id`` ("MoTeX\\0" + id); serving jobs
-mix their job id into the seed.
+point in the sampled data), so a given (workload, seed) yields identical sequences on every
+machine.  The base seed is ``0x4D6F54655800 + workload id`` ("MoTeX\\0" + id: 1, 2, 3,
+0x4A, 0x4B in the order of WORKLOADS); serving jobs mix their job id into the seed.
 
 The planted-motif scheme follows the intent of the reference's simulators
 (data/motif_sim.c:165-236, data/str_motif_sim.c:215-346) but overwrites at a uniform
 offset instead of appending, so the stated sequence lengths hold exactly.
 
-Command line: ``python -m motex_amd.synth CONFIG OUT.fa [--scale F]`` writes a FASTA file
-(and, for config 4A, ``OUT.fa.boxes``).
+Command line: ``python -m motex_amd.synth WORKLOAD OUT.fa [--scale F]`` writes a FASTA file
+(and, for the structured workload, ``OUT.fa.boxes``).
 """
 from __future__ import annotations
 
@@ -28,7 +28,7 @@ BASE_SEED = 0x4D6F54655800
 DNA4 = b"ACGT"
 PROT20 = b"ARNDCQEGHILKMFPSTWYV"
 
-# Config 3 letter frequencies: one Dirichlet(alpha=2) draw over PROT20, fixed once and
+# Letter frequencies of the protein workload: one Dirichlet(alpha=2) draw over PROT20, fixed once and
 # kept as integer weights out of 2**32 so that sampling stays integer-only.  (The draw was
 # Gamma(2) = -ln(U1*U2) per letter from SplitMix64 seeded with BASE_SEED + 3 + 0x5EED,
 # normalised and floored to 32-bit fixed point; the remainder went to the largest weight.)
@@ -191,7 +191,7 @@ def _iid(rng, count, length, alphabet, weights=None):
     return [rng.symbols(length, alphabet, weights) for _ in range(count)]
 
 
-def config1(scale: float = 1.0) -> Workload:
+def dna50_hamming(scale: float = 1.0) -> Workload:
     rng = SplitMix64(BASE_SEED + 1)
     nseq = max(2, int(round(50 * scale)))
     seqs = _iid(rng, nseq, 500, DNA4)
@@ -199,12 +199,12 @@ def config1(scale: float = 1.0) -> Workload:
     for mo in motifs:
         for i in rng.choose(nseq, max(1, (nseq * 60) // 100)):
             plant(rng, seqs[i], substitute(rng, mo, rng.uniform_int(0, 1), DNA4))
-    return Workload("config1: 50 DNA x 500 nt, Hamming k=10 e=1 q=50", "DNA",
+    return Workload("50 DNA x 500 nt, Hamming k=10 e=1 q=50", "DNA",
                     [s.tobytes() for s in seqs], 0, 10, 1, 50,
                     planted=[m.tobytes() for m in motifs])
 
 
-def config2(scale: float = 1.0) -> Workload:
+def dna1200_hamming(scale: float = 1.0) -> Workload:
     rng = SplitMix64(BASE_SEED + 2)
     nseq = max(2, int(round(1200 * scale)))
     w = _weights([0.295, 0.205, 0.205, 0.295])
@@ -214,12 +214,12 @@ def config2(scale: float = 1.0) -> Workload:
         frac = rng.uniform_int(30, 70)
         for i in rng.choose(nseq, max(1, (nseq * frac) // 100)):
             plant(rng, seqs[i], substitute(rng, mo, rng.uniform_int(0, 2), DNA4))
-    return Workload("config2: 1200 DNA x 1000 nt GC41%, Hamming k=12 e=2 q=25", "DNA",
+    return Workload("1200 DNA x 1000 nt GC41%, Hamming k=12 e=2 q=25", "DNA",
                     [s.tobytes() for s in seqs], 0, 12, 2, 25,
                     planted=[m.tobytes() for m in motifs])
 
 
-def config3(scale: float = 1.0) -> Workload:
+def prot500_edit(scale: float = 1.0) -> Workload:
     rng = SplitMix64(BASE_SEED + 3)
     nseq = max(2, int(round(500 * scale)))
     w = _prot_weights()
@@ -230,12 +230,12 @@ def config3(scale: float = 1.0) -> Workload:
         for i in rng.choose(nseq, max(1, (nseq * 40) // 100)):
             c = edit_once(rng, mo, PROT20) if rng.uniform_int(0, 1) else mo
             plant(rng, seqs[i], c)
-    return Workload("config3: 500 PROT x U[200,2000], edit k=8 e=1 q=30", "PROT",
+    return Workload("500 PROT x U[200,2000], edit k=8 e=1 q=30", "PROT",
                     [s.tobytes() for s in seqs], 1, 8, 1, 30,
                     planted=[m.tobytes() for m in motifs])
 
 
-def config4a(scale: float = 1.0) -> Workload:
+def dna1000_structured(scale: float = 1.0) -> Workload:
     rng = SplitMix64(BASE_SEED + 0x4A)
     nseq = max(2, int(round(1000 * scale)))
     seqs = _iid(rng, nseq, 600, DNA4)
@@ -246,12 +246,12 @@ def config4a(scale: float = 1.0) -> Workload:
         c2 = substitute(rng, b2, rng.uniform_int(0, 1), DNA4)
         copy = np.concatenate([c1, rng.symbols(gap, DNA4), c2])
         plant(rng, seqs[i], copy)
-    return Workload("config4A: 1000 DNA x 600 nt, structured 8+[10,30]+8 e=1 q=40", "DNA",
+    return Workload("1000 DNA x 600 nt, structured 8+[10,30]+8 e=1 q=40", "DNA",
                     [s.tobytes() for s in seqs], 0, 8, 1, 40, boxes=[(10, 30, 8, 1)],
                     planted=[b1.tobytes(), b2.tobytes()])
 
 
-def config4b(scale: float = 1.0) -> Workload:
+def dna4m_long(scale: float = 1.0) -> Workload:
     rng = SplitMix64(BASE_SEED + 0x4B)
     n = max(1000, int(round(4_000_000 * scale)))
     w = _weights([0.285, 0.215, 0.215, 0.285])
@@ -263,7 +263,7 @@ def config4b(scale: float = 1.0) -> Workload:
         cp = substitute(rng, mo, rng.uniform_int(0, 2), DNA4)
         off = c * seg + rng.uniform_int(0, seg - len(cp))
         seq[off:off + len(cp)] = cp
-    return Workload("config4B: 1 DNA x 4 Mnt GC43%, Hamming k=10 e=2, long-sequence mode",
+    return Workload("1 DNA x 4 Mnt GC43%, Hamming k=10 e=2, long-sequence mode",
                     "DNA", [seq.tobytes()], 0, 10, 2, 0, long_mode=True,
                     planted=[mo.tobytes()])
 
@@ -282,17 +282,19 @@ def serving_job(job_id: int, scale: float = 1.0) -> Workload:
     return Workload(f"job{job_id}", "DNA" if dna else "PROT", seqs, d, k, e, q)
 
 
-CONFIGS = {"1": config1, "2": config2, "3": config3, "4a": config4a, "4b": config4b}
+# Workloads by name: sequences and sizes, distance, alphabet.
+WORKLOADS = {"dna50": dna50_hamming, "dna1200": dna1200_hamming, "prot500": prot500_edit,
+             "struct1000": dna1000_structured, "long4m": dna4m_long}
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("config", choices=sorted(CONFIGS) + ["job"])
+    ap.add_argument("workload", choices=sorted(WORKLOADS) + ["job"])
     ap.add_argument("out")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--job", type=int, default=0)
     a = ap.parse_args(argv)
-    w = serving_job(a.job, a.scale) if a.config == "job" else CONFIGS[a.config](a.scale)
+    w = serving_job(a.job, a.scale) if a.workload == "job" else WORKLOADS[a.workload](a.scale)
     with open(a.out, "wb") as f:
         f.write(w.fasta())
     if w.boxes:

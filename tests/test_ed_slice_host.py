@@ -24,7 +24,7 @@ def checker(tmp_path_factory, oracle):
 
 # seed k e m nmin nmax sigma ntexts
 CASES = [
-    (1, 8, 1, 60, 20, 80, 20, 32),     # protein-like alphabet, config 3's k and e
+    (1, 8, 1, 60, 20, 80, 20, 32),     # protein-like alphabet, the protein workload's k and e
     (2, 5, 2, 40, 5, 60, 4, 32),       # texts shorter than k mixed in
     (3, 3, 0, 30, 1, 40, 2, 17),       # e = 0, partial group
     (4, 12, 3, 50, 10, 90, 4, 32),

@@ -48,8 +48,8 @@ def run_both(tmp_path, oracle, w, smile=True):
     return masked(gdir / "out.motex")
 
 
-def test_config1(tmp_path, oracle):
-    w = synth.config1()
+def test_dna50(tmp_path, oracle):
+    w = synth.dna50_hamming()
     lines = run_both(tmp_path, oracle, w)
     motifs = [l.split()[0] for l in lines if l and not l.startswith("#")]
     # the planted 10-mers (in 60% of sequences, q=50) are reported
@@ -57,19 +57,19 @@ def test_config1(tmp_path, oracle):
         assert p.decode() in motifs
 
 
-def test_structured_config4a_subset(tmp_path, oracle):
-    w = synth.config4a(scale=0.03)
+def test_structured_dna1000_subset(tmp_path, oracle):
+    w = synth.dna1000_structured(scale=0.03)
     w.q = 10
     run_both(tmp_path, oracle, w)
 
 
 def test_protein_edit_subset(tmp_path, oracle):
-    w = synth.config3(scale=0.02)
+    w = synth.prot500_edit(scale=0.02)
     w.q = 5
     run_both(tmp_path, oracle, w)
 
 
 def test_quorum_zero_prints_every_factor(tmp_path, oracle):
-    w = synth.config1(scale=0.1)
+    w = synth.dna50_hamming(scale=0.1)
     w.q, w.N = 0, 2000000000
     run_both(tmp_path, oracle, w, smile=False)

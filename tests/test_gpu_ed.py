@@ -40,8 +40,8 @@ def check(ctx, oracle, seqs, k, e, alphabet="DNA", kernel=None):
     return u, v
 
 
-def test_config3_subset(ctx, oracle, kernel):
-    w = synth.config3(scale=0.03)  # 15 protein sequences, lengths 200..2000
+def test_prot500_subset(ctx, oracle, kernel):
+    w = synth.prot500_edit(scale=0.03)  # 15 protein sequences, lengths 200..2000
     u, v = check(ctx, oracle, w.seqs, w.k, w.e, "PROT", kernel)
     assert v.sum() > 0
 

@@ -37,19 +37,19 @@ def check(ctx, oracle, seqs, k, e, alphabet="DNA", sym=None):
     return u, v
 
 
-def test_config1_full(ctx, oracle, sym):
-    w = synth.config1()
+def test_dna50_full(ctx, oracle, sym):
+    w = synth.dna50_hamming()
     u, v = check(ctx, oracle, w.seqs, w.k, w.e, sym=sym)
     assert v.sum() > 0
 
 
-def test_config2_subset(ctx, oracle, sym):
-    w = synth.config2(scale=0.05)  # 60 sequences of 1000 nt
+def test_dna1200_subset(ctx, oracle, sym):
+    w = synth.dna1200_hamming(scale=0.05)  # 60 sequences of 1000 nt
     check(ctx, oracle, w.seqs, w.k, w.e, sym=sym)
 
 
 def test_symmetric_is_the_default_for_rare_occurrences(ctx):
-    w = synth.config2(scale=0.05)
+    w = synth.dna1200_hamming(scale=0.05)
     b = ctx.batch(w.seqs, MotexParams(k=w.k, e=w.e))
     assert b.stats()["symmetric"] == 1
     b.close()
@@ -94,7 +94,7 @@ def test_empty_and_single(ctx, oracle):
 
 
 def test_plain_form_equals_batched(ctx, oracle):
-    w = synth.config1(scale=0.2)
+    w = synth.dna50_hamming(scale=0.2)
     p = MotexParams(k=w.k, e=w.e)
     b = ctx.batch(w.seqs, p)
     b.run()

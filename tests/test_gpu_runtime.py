@@ -12,11 +12,11 @@ pytestmark = pytest.mark.gpu
 
 def workload(kind):
     if kind == "hd":
-        w = synth.config1(scale=0.4)
+        w = synth.dna50_hamming(scale=0.4)
     elif kind == "ed":
-        w = synth.config3(scale=0.02)
+        w = synth.prot500_edit(scale=0.02)
     else:
-        w = synth.config4a(scale=0.02)
+        w = synth.dna1000_structured(scale=0.02)
     return w, MotexParams(w.k, w.e, w.distance, w.alphabet, boxes=w.boxes)
 
 

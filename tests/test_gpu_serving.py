@@ -10,9 +10,9 @@ pytestmark = pytest.mark.gpu
 
 def mixed_jobs():
     jobs = [synth.serving_job(i, scale=0.08) for i in range(10)]
-    w4 = synth.config4a(scale=0.01)
+    w4 = synth.dna1000_structured(scale=0.01)
     jobs.append(synth.Workload("structured", "DNA", w4.seqs, 0, w4.k, w4.e, 40, boxes=list(w4.boxes)))
-    w3 = synth.config3(scale=0.01)
+    w3 = synth.prot500_edit(scale=0.01)
     jobs.append(synth.Workload("edit", "PROT", w3.seqs, 1, w3.k, w3.e, 30))
     return jobs
 
@@ -52,7 +52,7 @@ def test_invalid_job_is_refused(ctx):
 def test_admission_refuses_jobs_over_the_budget():
     small = motex_amd.Context(0, hbm_budget=1 << 16)
     try:
-        w = synth.config2(scale=0.05)
+        w = synth.dna1200_hamming(scale=0.05)
         with pytest.raises(motex_amd.MotexError) as ei:
             serving.JobHandle(small, w.seqs, serving.job_params(w))
         assert ei.value.code == -2

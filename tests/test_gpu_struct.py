@@ -36,8 +36,8 @@ def check(ctx, oracle, seqs, k, e, boxes, distance=0, sym=None):
     return u, v
 
 
-def test_config4a_subset(ctx, oracle, sym):
-    w = synth.config4a(scale=0.02)  # 20 sequences of 600 nt, one planted structured motif
+def test_dna1000_structured_subset(ctx, oracle, sym):
+    w = synth.dna1000_structured(scale=0.02)  # 20 sequences of 600 nt, one planted structured motif
     u, v = check(ctx, oracle, w.seqs, w.k, w.e, list(w.boxes), sym=sym)
     assert v.sum() > 0
 
@@ -75,7 +75,7 @@ def test_edit_shapes(ctx, oracle, seed, k, e, boxes):
 
 
 def test_plain_form_equals_batched(ctx):
-    w = synth.config4a(scale=0.01)
+    w = synth.dna1000_structured(scale=0.01)
     p = MotexParams(k=w.k, e=w.e, boxes=list(w.boxes))
     b = ctx.batch(w.seqs, p)
     b.run()

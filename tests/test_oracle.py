@@ -226,12 +226,12 @@ def test_writer_double_filter_and_dedup(oracle, tmp_path):
     assert sl[4] == "CGTA 1230 30\t40"
 
 
-def test_config1_generator_is_deterministic():
-    a = synth.config1()
-    b = synth.config1()
+def test_dna50_generator_is_deterministic():
+    a = synth.dna50_hamming()
+    b = synth.dna50_hamming()
     assert a.seqs == b.seqs
     assert len(a.seqs) == 50 and all(len(s) == 500 for s in a.seqs)
     import hashlib
     h = hashlib.sha256(b"".join(a.seqs)).hexdigest()
-    golden = os.path.join(os.path.dirname(__file__), "golden", "config1_sha256.txt")
+    golden = os.path.join(os.path.dirname(__file__), "golden", "dna50_sha256.txt")
     assert open(golden).read().strip() == h
