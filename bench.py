@@ -134,8 +134,9 @@ def main():
         copy_probe(a.copy_probe)
         return
     if a.gpus != 1:
-        print(f"bench.py measures one GPU's share only; --gpus {a.gpus} is not supported "
-              "(sharding is described in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU's share only; --gpus {a.gpus} is not supported: "
+              "this build runs on one GPU (the x sequences would be split across GPUs, one "
+              "process each, with no collective).", file=sys.stderr)
         sys.exit(2)
     import motex_amd
     from motex_amd import MotexParams, synth
