@@ -32,12 +32,50 @@ __device__ __forceinline__ uint32_t shr1_from(uint32_t in0, uint32_t x)
     return __builtin_amdgcn_update_dpp(in0, x, 0x138, 0xF, 0xF, false);
 }
 
+// Past the first strip every column has j > K, where the left and diagonal candidates keep
+// only bits 0 .. K-2 of their source: those SP = K-1 planes are all a cell's right neighbour
+// reads from it, so only they cross lanes and strips.
 template <int K>
 struct SliceLayout {
     static constexpr int W = SliceW<K>::W;
-    static constexpr int SWS = (W + 3) & ~3;      // boundary-stream row: the word's planes
-    static constexpr int LW = (W + 6 + 3) & ~3;   // LDS chunk row: stream row + 6 text words
+    static constexpr int SP = K - 1;               // planes a later strip's left neighbour sends
+    static constexpr int SWS = (SP + 3) & ~3;      // boundary-stream row
+    static constexpr int LW = SWS + 8;             // LDS chunk row: stream row, 6 text words, 2 pad
 };
+
+// Lane rr fills row rr of the LDS chunk of 64 rows from r0: the previous strip's boundary row
+// (when from_stream; zeros for the first strip, whose left edge is column 0) with 16-byte
+// loads and stores, then the group's 6 text words with 8-byte ones.  The stream rows
+// (SWS words, a multiple of 4, from a 16-byte aligned base), the text rows (6 words) and the
+// chunk rows (LW words, a multiple of 4) keep those accesses aligned.
+template <int K>
+__device__ __forceinline__ void slice_fill_row(uint32_t *chunk, uint32_t r0, uint32_t nrows, bool from_stream,
+                                               const uint32_t *sin, const uint32_t *gp, uint32_t rr)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr int SWS = Lay::SWS, LW = Lay::LW;
+    const uint32_t row = r0 + rr;
+    uint4 *dst = reinterpret_cast<uint4 *>(chunk + rr * LW);
+    uint2 *dtext = reinterpret_cast<uint2 *>(chunk + rr * LW + SWS);
+    if (row < nrows) {
+        if (from_stream) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(sin + (size_t)row * SWS);
+#pragma unroll
+            for (int q = 0; q < SWS / 4; ++q) dst[q] = src[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < SWS / 4; ++q) dst[q] = make_uint4(0, 0, 0, 0);
+        }
+        const uint2 *tsrc = reinterpret_cast<const uint2 *>(gp + (size_t)row * 6);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) dtext[q] = tsrc[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < SWS / 4; ++q) dst[q] = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) dtext[q] = make_uint2(0, 0);
+    }
+}
 
 // Loads the LDS chunk of 64 rows from r0: the previous strip's boundary cells (strip > 0)
 // and the group's text planes.
@@ -45,18 +83,8 @@ template <int K>
 __device__ __forceinline__ void slice_chunk(uint32_t *chunk, uint32_t r0, uint32_t nrows, bool from_stream,
                                             const uint32_t *sin, const uint32_t *gp)
 {
-    typedef SliceLayout<K> Lay;
-    constexpr int W = Lay::W, SWS = Lay::SWS, LW = Lay::LW;
     __syncthreads();  // the previous chunk is no longer read
-    for (uint32_t w = threadIdx.x; w < 64u * LW; w += 64) {
-        const uint32_t rr = w / LW, c = w % LW, row = r0 + rr;
-        uint32_t val = 0;
-        if (row < nrows) {
-            if (c < (uint32_t)W) val = from_stream ? sin[(size_t)row * SWS + c] : 0u;
-            else if (c < (uint32_t)W + 6) val = gp[(size_t)row * 6 + (c - W)];
-        }
-        chunk[w] = val;
-    }
+    slice_fill_row<K>(chunk, r0, nrows, from_stream, sin, gp, threadIdx.x);
     __syncthreads();
 }
 
@@ -65,13 +93,35 @@ template <int K>
 __device__ __forceinline__ void slice_emit(uint32_t *sout, uint32_t s, const uint32_t *d)
 {
     typedef SliceLayout<K> Lay;
-    constexpr int W = Lay::W, SWS = Lay::SWS;
+    constexpr int SP = Lay::SP, SWS = Lay::SWS;
     uint32_t w[SWS];
 #pragma unroll
-    for (int p = 0; p < SWS; ++p) w[p] = p < W ? d[p] : 0u;
+    for (int p = 0; p < SWS; ++p) w[p] = p < SP ? d[p] : 0u;
     uint4 *dst = reinterpret_cast<uint4 *>(sout + (size_t)(s - 64) * SWS);
 #pragma unroll
     for (int p = 0; p < SWS / 4; ++p) dst[p] = make_uint4(w[4 * p], w[4 * p + 1], w[4 * p + 2], w[4 * p + 3]);
+}
+
+// The left neighbour's cell D[i][j-1]: lane L-1's, by one DPP wave shift per plane; lane 0
+// takes column 0 (zero) in the first strip and the previous strip's boundary row cr later.
+// The first strip moves all W planes and recounts pc and pm; later strips move the SP
+// planes their cells read (the rest stay zero) and recount pm.
+template <int K, bool FIRST>
+__device__ __forceinline__ void slice_left(Slice<K> &L, const uint32_t *ud, const uint32_t *cr)
+{
+    typedef SliceLayout<K> Lay;
+    constexpr int W = Lay::W, SP = Lay::SP;
+    if constexpr (FIRST) {
+#pragma unroll
+        for (int p = 0; p < W; ++p) L.d[p] = shr1_from(0u, ud[p]);
+        slice_popcount<W>(L.d, L.pc);
+    } else {
+#pragma unroll
+        for (int p = 0; p < SP; ++p) L.d[p] = shr1_from(cr[p], ud[p]);
+#pragma unroll
+        for (int p = SP; p < W; ++p) L.d[p] = 0u;
+    }
+    slice_pm<K>(L.d, L.pm);
 }
 
 // One strip of 64 columns over all rows of the group.  Only a cell's planes travel between
@@ -84,7 +134,7 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
                                             uint32_t *sout, bool last, uint32_t *chunk)
 {
     typedef SliceLayout<K> Lay;
-    constexpr int W = Lay::W, LW = Lay::LW;
+    constexpr int SP = Lay::SP, LW = Lay::LW;
     const uint32_t lane = threadIdx.x;
     const uint32_t m = I.m;
     const uint32_t j = strip * 64 + lane + 1;  // 1-based column
@@ -109,13 +159,10 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
         if ((r0 & 63) == 0) slice_chunk<K>(chunk, r0, nrows, !FIRST, sin, gp);
         const uint32_t *cr = chunk + (r0 & 63) * LW;
         Slice<K> L;  // D[i][j-1]; only its planes travel, the receiver recounts pc and pm
-#pragma unroll
-        for (int p = 0; p < W; ++p) L.d[p] = shr1_from(FIRST ? 0u : cr[p], U.d[p]);
-        if constexpr (FIRST) slice_popcount<W>(L.d, L.pc);
-        slice_pm<K>(L.d, L.pm);
+        slice_left<K, FIRST>(L, U.d, cr);
         uint32_t t[6];
 #pragma unroll
-        for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[W + b], tpub[b]);
+        for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[Lay::SWS + b], tpub[b]);
         bool active = true;
         if constexpr (decltype(check)::value) active = s > lane && s - lane <= nrows;
         if (active) {
@@ -173,7 +220,7 @@ template <int K>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(slice_waves_per_eu<K>())))
 void ed_slice_kernel(EdSliceArgs a)
 {
-    __shared__ uint32_t chunk[64 * SliceLayout<K>::LW];
+    __shared__ __align__(16) uint32_t chunk[64 * SliceLayout<K>::LW];
     const uint32_t lane = threadIdx.x;
     uint32_t *st0 = a.scratch + (size_t)blockIdx.x * 2 * a.stream_words;
     uint32_t *st1 = st0 + a.stream_words;
@@ -210,17 +257,7 @@ template <int K>
 __device__ __forceinline__ void slice_chunk_fill(uint32_t *chunk, uint32_t r0, uint32_t nrows, bool from_stream,
                                                  const uint32_t *sin, const uint32_t *gp, uint32_t lane)
 {
-    typedef SliceLayout<K> Lay;
-    constexpr int W = Lay::W, SWS = Lay::SWS, LW = Lay::LW;
-    for (uint32_t w = lane; w < 64u * LW; w += 64) {
-        const uint32_t rr = w / LW, c = w % LW, row = r0 + rr;
-        uint32_t val = 0;
-        if (row < nrows) {
-            if (c < (uint32_t)W) val = from_stream ? sin[(size_t)row * SWS + c] : 0u;
-            else if (c < (uint32_t)W + 6) val = gp[(size_t)row * 6 + (c - W)];
-        }
-        chunk[w] = val;
-    }
+    slice_fill_row<K>(chunk, r0, nrows, from_stream, sin, gp, lane);
 }
 
 // One lane's column j of a strip, carried from row to row: D[i-1][j] (what lane L+1 receives
@@ -265,16 +302,13 @@ __device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uin
                                            uint32_t lane, uint32_t *sout, bool emit)
 {
     typedef SliceLayout<K> Lay;
-    constexpr int W = Lay::W, LW = Lay::LW;
+    constexpr int SP = Lay::SP, LW = Lay::LW;
     const uint32_t *cr = chunk + ((s - 1) & 63) * LW;
     Slice<K> L;  // D[i][j-1]
-#pragma unroll
-    for (int p = 0; p < W; ++p) L.d[p] = shr1_from(FIRST ? 0u : cr[p], c.U.d[p]);
-    if constexpr (FIRST) slice_popcount<W>(L.d, L.pc);
-    slice_pm<K>(L.d, L.pm);
+    slice_left<K, FIRST>(L, c.U.d, cr);
     uint32_t t[6];
 #pragma unroll
-    for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[W + b], c.tpub[b]);
+    for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[Lay::SWS + b], c.tpub[b]);
     bool active = true;
     if constexpr (CHECK) active = s > lane && s - lane <= nrows;
     if (active) {
@@ -349,7 +383,7 @@ void ed_slice_pipe_kernel(EdSliceArgs a)
 {
     typedef SliceLayout<K> Lay;
     constexpr uint32_t P = kSlicePipe;
-    __shared__ uint32_t chunks[P][64 * Lay::LW];
+    __shared__ __align__(16) uint32_t chunks[P][64 * Lay::LW];
     __shared__ uint32_t item_slot;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t *chunk = chunks[wave];
