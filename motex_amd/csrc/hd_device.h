@@ -78,13 +78,24 @@ __device__ __forceinline__ uint32_t planes_high(const uint32_t *c)
     return h;
 }
 
-// planes_high with the threshold plane ph chosen at run time (wave-uniform).
+// Masks selecting planes ph..P-1 (wave-uniform), for planes_high_masked: the threshold
+// plane chosen at run time.
 template <int P>
-__device__ __forceinline__ uint32_t planes_high_rt(const uint32_t *c, uint32_t ph)
+__device__ __forceinline__ void planes_high_masks(uint32_t *hm, uint32_t ph)
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) hm[p] = (uint32_t)p >= ph ? ~0u : 0u;
+}
+
+// planes_high with the threshold plane chosen at run time, from planes_high_masks: one
+// 3-input logic op per plane, (c & m) | h (v_bitop3_b32 table 0xEA), in place of a select and
+// an OR.
+template <int P>
+__device__ __forceinline__ uint32_t planes_high_masked(const uint32_t *c, const uint32_t *hm)
 {
     uint32_t h = 0;
 #pragma unroll
-    for (int p = 0; p < P; ++p) h |= (uint32_t)p >= ph ? c[p] : 0u;
+    for (int p = 0; p < P; ++p) h = __builtin_amdgcn_bitop3_b32(c[p], hm[p], h, 0xEA);
     return h;
 }
 

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "ed_device.h"
 #include "hd_device.h"
 #include "motex_internal.h"
@@ -113,7 +115,8 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
     const uint32_t lane = threadIdx.x;
     const uint32_t m = I.m, n = I.n, wl = a.wl;
     const uint32_t zero_row = a.sigma * wl;
-    const uint32_t ph = hd_low_planes(err);  // window counts are biased (hd_device.h)
+    uint32_t hm[P];  // window counts are biased (hd_device.h): hits have planes ph.. clear
+    planes_high_masks<P>(hm, hd_low_planes(err));
     int64_t q0 = (int64_t)I.y_start - (int64_t)(m - 1);
     q0 = q0 >= 0 ? (q0 & ~31LL) : -(((-q0) + 31) & ~31LL);
     const int64_t qend = (int64_t)I.y_start + n;  // diagonals below this can hold hits
@@ -132,21 +135,16 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
         __syncthreads();
         uint32_t cnt[P];
         planes_set<P>(cnt, hd_bias(err));
-        for (uint32_t px = 0; px < m; ++px) {
+        // Row px: the window gains column px and, from px = len on, loses column px - len
+        // (before that, the all-zero row: nothing leaves).
+        // WARM: rows before len, of which only the last can hold hits.
+        auto row = [&](uint32_t px, uint32_t wo, uint32_t sho, auto warm) {
             const uint32_t wn = xoff[px] + (px >> 5) + lane;
-            uint32_t wo, sho;
-            if (px >= len) {
-                wo = xoff[px - len] + ((px - len) >> 5) + lane;
-                sho = (px - len) & 31;
-            } else {
-                wo = zero_row + lane;
-                sho = 0;
-            }
             uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 1], ne_l[wn], px & 31);
             uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 1], ne_l[wo], sho);
             planes_slide<P>(cnt, an, bo);
-            const uint32_t res = ~planes_high_rt<P>(cnt, ph);
-            if (px + 1 >= len && __ballot(res != 0)) {
+            const uint32_t res = ~planes_high_masked<P>(cnt, hm);
+            if ((!decltype(warm)::value || px + 1 >= len) && __ballot(res != 0)) {
                 // flag t of this lane is text position pt = base + t; only [len - 1, n) counts
                 const int64_t base = qb + 32 * lane + px - (int64_t)I.y_start;
                 const int64_t tlo = (int64_t)len - 1 - base, thi = (int64_t)n - base;
@@ -180,7 +178,11 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
                     }
                 }
             }
-        }
+        };
+        const uint32_t warm = len < m ? len : m;
+        uint32_t px = 0;
+        for (; px < warm; ++px) row(px, zero_row + lane, 0u, std::true_type{});
+        for (; px < m; ++px) row(px, xoff[px - len] + ((px - len) >> 5) + lane, (px - len) & 31, std::false_type{});
         ++nruns;
         if (write_bm) nl = __builtin_amdgcn_readfirstlane(*lcount);
         if (runs && lane == 0) runs[1 + nruns] = min(nl, a.lcap);
