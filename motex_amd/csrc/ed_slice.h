@@ -38,6 +38,10 @@ constexpr uint32_t kBop3Mux = 0xE4;  // src2 ? src0 : src1
 constexpr uint32_t kBop3And = 0xC0;  // src0 & src1
 constexpr uint32_t kBop3Gt = 0xB2;   // majority(src0, ~src1, src2)
 constexpr uint32_t kBop3Lt = 0x8E;   // majority(~src0, src1, src2)
+constexpr uint32_t kBop3Or3 = 0xFE;  // src0 | src1 | src2
+constexpr uint32_t kBop3AndNot = 0x30;   // src0 & ~src1
+constexpr uint32_t kBop3AndNot2 = 0x10;  // src0 & ~src1 & ~src2
+constexpr uint32_t kBop3AndNotUnless = 0xB0;  // src0 & ~(src1 & ~src2)
 
 template <int K> struct SliceW { static constexpr int W = 2 * K; };
 
@@ -66,6 +70,43 @@ MOTEX_HD uint32_t slice_and(uint32_t x, uint32_t m)
     return __builtin_amdgcn_bitop3_b32(x, m, x, kBop3And);
 #else
     return x & m;
+#endif
+}
+
+MOTEX_HD uint32_t slice_or3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, kBop3Or3);
+#else
+    return a | b | c;
+#endif
+}
+
+MOTEX_HD uint32_t slice_andnot(uint32_t a, uint32_t s)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, s, a, kBop3AndNot);
+#else
+    return a & ~s;
+#endif
+}
+
+MOTEX_HD uint32_t slice_andnot2(uint32_t a, uint32_t s1, uint32_t s2)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, s1, s2, kBop3AndNot2);
+#else
+    return a & ~s1 & ~s2;
+#endif
+}
+
+// a & ~(h & ~s): a, with the lanes of h cleared except where s is set
+MOTEX_HD uint32_t slice_andnot_unless(uint32_t a, uint32_t h, uint32_t s)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, h, s, kBop3AndNotUnless);
+#else
+    return a & ~(h & ~s);
 #endif
 }
 
@@ -227,6 +268,12 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
 // The cell for columns j > K, where left and diag enter truncated: their candidates have no
 // plane at or above K, so only pm of left and diag and pc of up are needed, and the result's
 // pm is left to the reader (slice_pm).  up: d, pc; left, diag: d, pm.  Returns d, pc.
+//
+// Planes K..W of the candidates are up's alone (left and diag are zero there), so both
+// comparisons decide those planes from one OR of them: in the ripple, a plane where the
+// other side is zero turns "better so far" off wherever up's plane is set, and the planes of
+// the first choice d are up's masked by "not left".  The selected word's planes K..W-1 are
+// up's where neither left nor diag was chosen.
 template <int K>
 MOTEX_HD void slice_cell_masked(const uint32_t *ud, const uint32_t *upc, const uint32_t *ld,
                                 const uint32_t *lpm, const uint32_t *gd, const uint32_t *gpm,
@@ -234,28 +281,43 @@ MOTEX_HD void slice_cell_masked(const uint32_t *ud, const uint32_t *upc, const u
                                 uint32_t *dropped = nullptr)
 {
     constexpr int W = SliceW<K>::W;
-    uint32_t a[W + 1], b[W + 1], c[W + 1];
-    uint32_t pcc[kSlicePc];
+    // candidate planes 0..K-1: a = shift(up) | 1, b = shiftc(left) | 1, c = shiftc(diag) | delta
+    uint32_t a[K], b[K], c[K];
     a[0] = ~0u;
     b[0] = ~0u;
     c[0] = delta;
-    for (int p = 1; p <= W; ++p) {
+    for (int p = 1; p < K; ++p) {
         a[p] = ud[p - 1];
-        b[p] = p <= K - 1 ? ld[p - 1] : 0u;
-        c[p] = p <= K - 1 ? gd[p - 1] : 0u;
+        b[p] = ld[p - 1];
+        c[p] = gd[p - 1];
     }
+    // any of a's planes K..W (= up's planes K-1..W-1)
+    uint32_t hi = 0;
+    for (int p = K - 1; p < W; p += 2) hi = slice_or3(hi, ud[p], p + 1 < W ? ud[p + 1] : 0u);
+    uint32_t pcc[kSlicePc];
     slice_inc(gpm, delta, pcc);
-    // a and b both set plane 0 and add one to their source's popcount
-    uint32_t sel = slice_better<W>(lpm, b + 1, upc, a + 1);
-    uint32_t d[W + 1], pd[kSlicePc], ps[kSlicePc];
+    // min*: start from a, replace where b then c is strictly better.  a and b both set plane
+    // 0 and add one to their source's popcount, so they compare as their sources do.
+    uint32_t r = 0;
+    for (int p = 1; p < K; ++p) r = slice_gt_step(b[p], a[p], r);
+    r = slice_andnot(r, hi);
+    for (int q = 0; q < kSlicePc; ++q) r = slice_lt_step(lpm[q], upc[q], r);
+    const uint32_t sel = r;
+    uint32_t d[K], pd[kSlicePc], ps[kSlicePc];
     d[0] = ~0u;
-    for (int p = 1; p <= W; ++p) d[p] = slice_mux(sel, b[p], a[p]);
+    for (int p = 1; p < K; ++p) d[p] = slice_mux(sel, b[p], a[p]);
     for (int q = 0; q < kSlicePc; ++q) ps[q] = slice_mux(sel, lpm[q], upc[q]);
     slice_inc(ps, ~0u, pd);
-    sel = slice_better<W + 1>(pcc, c, pd, d);
-    for (int p = 0; p < W; ++p) dd[p] = slice_mux(sel, c[p], d[p]);
-    if (dropped) *dropped |= slice_mux(sel, c[W], d[W]);
-    for (int q = 0; q < kSlicePc; ++q) dpc[q] = slice_mux(sel, pcc[q], pd[q]);
+    // c against d: d's planes K..W are a's where b was not chosen
+    r = 0;
+    for (int p = 0; p < K; ++p) r = slice_gt_step(c[p], d[p], r);
+    r = slice_andnot_unless(r, hi, sel);
+    for (int q = 0; q < kSlicePc; ++q) r = slice_lt_step(pcc[q], pd[q], r);
+    const uint32_t sel2 = r;
+    for (int p = 0; p < K; ++p) dd[p] = slice_mux(sel2, c[p], d[p]);
+    for (int p = K; p < W; ++p) dd[p] = slice_andnot2(ud[p - 1], sel, sel2);
+    if (dropped) *dropped |= slice_andnot2(ud[W - 1], sel, sel2);
+    for (int q = 0; q < kSlicePc; ++q) dpc[q] = slice_mux(sel2, pcc[q], pd[q]);
 }
 
 // pm of a word: the popcount of its low K-1 planes.

@@ -133,6 +133,10 @@ static int check_tables()
     bad += bitop3(a, b, a, motex::kBop3And) != motex::slice_and(a, b);
     bad += bitop3(a, b, c, motex::kBop3Gt) != motex::slice_gt_step(a, b, c);
     bad += bitop3(a, b, c, motex::kBop3Lt) != motex::slice_lt_step(a, b, c);
+    bad += bitop3(a, b, c, motex::kBop3Or3) != motex::slice_or3(a, b, c);
+    bad += bitop3(a, b, a, motex::kBop3AndNot) != motex::slice_andnot(a, b);
+    bad += bitop3(a, b, c, motex::kBop3AndNot2) != motex::slice_andnot2(a, b, c);
+    bad += bitop3(a, b, c, motex::kBop3AndNotUnless) != motex::slice_andnot_unless(a, b, c);
     // and the comparisons themselves on small numbers, exhaustively
     for (uint32_t x = 0; x < 64; ++x)
         for (uint32_t y = 0; y < 64; ++y) {
