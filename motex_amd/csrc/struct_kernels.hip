@@ -135,13 +135,17 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
         __syncthreads();
         uint32_t cnt[P];
         planes_set<P>(cnt, hd_bias(err));
-        // Row px: the window gains column px and, from px = len on, loses column px - len
-        // (before that, the all-zero row: nothing leaves).
-        // WARM: rows before len, of which only the last can hold hits.
-        auto row = [&](uint32_t px, uint32_t wo, uint32_t sho, auto warm) {
-            const uint32_t wn = xoff[px] + (px >> 5) + lane;
-            uint32_t an = __builtin_amdgcn_alignbit(ne_l[wn + 1], ne_l[wn], px & 31);
-            uint32_t bo = __builtin_amdgcn_alignbit(ne_l[wo + 1], ne_l[wo], sho);
+        // xoff[px] is the byte offset of column px's first window word (its symbol's row, word
+        // px / 32); lane L reads word L after it.
+        const char *ne_b = reinterpret_cast<const char *>(ne_l) + 4 * lane;
+        // Row px: the window gains column px and, from px = len on, loses column px - len (ob:
+        // its byte offset as above; before len, the all-zero row: nothing leaves).  WARM: rows
+        // before len, of which only the last can hold hits.
+        auto row = [&](uint32_t px, uint32_t ob, uint32_t sho, auto warm) {
+            const uint32_t *wn = reinterpret_cast<const uint32_t *>(ne_b + xoff[px]);
+            const uint32_t *wo = reinterpret_cast<const uint32_t *>(ne_b + ob);
+            uint32_t an = __builtin_amdgcn_alignbit(wn[1], wn[0], px & 31);
+            uint32_t bo = __builtin_amdgcn_alignbit(wo[1], wo[0], sho);
             planes_slide<P>(cnt, an, bo);
             const uint32_t res = ~planes_high_masked<P>(cnt, hm);
             if ((!decltype(warm)::value || px + 1 >= len) && __ballot(res != 0)) {
@@ -181,8 +185,8 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
         };
         const uint32_t warm = len < m ? len : m;
         uint32_t px = 0;
-        for (; px < warm; ++px) row(px, zero_row + lane, 0u, std::true_type{});
-        for (; px < m; ++px) row(px, xoff[px - len] + ((px - len) >> 5) + lane, (px - len) & 31, std::false_type{});
+        for (; px < warm; ++px) row(px, 4 * zero_row, 0u, std::true_type{});
+        for (; px < m; ++px) row(px, xoff[px - len], (px - len) & 31, std::false_type{});
         ++nruns;
         if (write_bm) nl = __builtin_amdgcn_readfirstlane(*lcount);
         if (runs && lane == 0) runs[1 + nruns] = min(nl, a.lcap);
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x;
     const uint32_t wl = a.wl;
-    // LDS: NE window (sigma+1 rows) | x offsets (m_max u16) | hit list | count
+    // LDS: NE window (sigma+1 rows) | x window offsets (m_max u16, bytes) | hit list | count
     uint32_t *ne_l = smem;
     uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + (a.sigma + 1) * wl);
     uint32_t *hits = smem + (a.sigma + 1) * wl + (a.m_max + 1) / 2;
@@ -299,8 +303,9 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
             for (uint32_t b = 0; b <= a.nb_gaps; ++b) bm[b] = bmd[a.dist_of[b]];
             __syncthreads();
             if (a.distance == 0) {
+                // below (sigma + 1) * wl * 4 <= the LDS size, checked on the host: fits 16 bits
                 for (uint32_t r = lane; r < m; r += 64)
-                    xoff[r] = (uint16_t)(a.codes[I.x_start + r] * wl);
+                    xoff[r] = (uint16_t)(4 * (a.codes[I.x_start + r] * wl + (r >> 5)));
             }
             if (a.distance == 0 && !clean) {
                 for (size_t w = lane; w < (size_t)2 * a.ndist * a.bm_words; w += 64) scr[w] = 0;
