@@ -127,6 +127,8 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
     uint32_t nl = __builtin_amdgcn_readfirstlane(*lcount);
     for (int64_t qb = q0; qb < qend; qb += kDiagPerBlock) {
         const uint32_t wbase = (uint32_t)((qb + (int64_t)kPadBits) / 32);
+        // text position of lane 0's flag 0 at row 0: within (-m - 32, n + 2048), so 32 bits
+        const int32_t rb = (int32_t)(qb - (int64_t)I.y_start);
         __syncthreads();
         for (uint32_t c = 0; c < a.sigma; ++c) {
             const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
@@ -150,8 +152,8 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
             const uint32_t res = ~planes_high_masked<P>(cnt, hm);
             if ((!decltype(warm)::value || px + 1 >= len) && __ballot(res != 0)) {
                 // flag t of this lane is text position pt = base + t; only [len - 1, n) counts
-                const int64_t base = qb + 32 * lane + px - (int64_t)I.y_start;
-                const int64_t tlo = (int64_t)len - 1 - base, thi = (int64_t)n - base;
+                const int32_t base = rb + (int32_t)(32 * lane + px);
+                const int32_t tlo = (int32_t)len - 1 - base, thi = (int32_t)n - base;
                 uint32_t hv = res;
                 if (tlo > 0) hv = tlo >= 32 ? 0u : hv & (~0u << (uint32_t)tlo);
                 if (thi < 32) hv = thi <= 0 ? 0u : hv & ((1u << (uint32_t)thi) - 1u);
