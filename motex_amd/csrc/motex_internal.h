@@ -19,10 +19,11 @@ constexpr uint32_t kMaxRangeSeqs = 32;       // sequences per Hamming text range
 constexpr uint32_t kHitQueueCap = 512;       // Hamming hit queue entries per wave (global)
 constexpr uint32_t kSlicePipe = 4;           // waves per item of the pipelined text-parallel edit kernel
 // Edit kernel choice by the number of (x, group of 32 texts) items per CU, from sweeps of
-// 40-400 protein sequences of U[100,2000], k 8-14: below kSliceItemsPerCu the
+// 40-400 protein and DNA sequences of U[100,2000], k 8-14: below kSliceItemsPerCu the
 // one-text-per-lane kernel, below kSlicePipeItemsPerCu the pipelined text-parallel kernel,
-// otherwise one wave per item.
-constexpr uint32_t kSliceItemsPerCu = 2;
+// otherwise one wave per item.  (The one-text-per-lane and pipelined kernels cross at about
+// 0.9 items per CU: 7.4 against 6.3 ms at 0.94, 9.0 against 6.4 ms at 1.56.)
+constexpr uint32_t kSliceItemsPerCu = 1;
 constexpr uint32_t kSlicePipeItemsPerCu = 12;
 
 // A text range of the Hamming work queue: whole sequences j0 .. j0+ny-1, or one part of a
