@@ -212,9 +212,11 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
 // Resident waves per SIMD the register allocation is held to.  From k = 21 the state needs
 // more than 256 registers; holding two waves per SIMD anyway spills the excess to scratch
 // (12 B per lane at k = 21, about 360 B at k = 30), which on MI355X runs about 1.6x faster
-// (k = 22 to 30) than one wave per SIMD with the excess in AGPRs.
+// (k = 22 to 30) than one wave per SIMD with the excess in AGPRs.  For k = 9 to 14 (175-225
+// registers) three waves per SIMD with 20-250 B of spills run 6-16% faster than two; for
+// k <= 8 (three waves without spills) four waves, and for k = 16-20 three, ran slower.
 template <int K>
-constexpr int slice_waves_per_eu() { return 2; }
+constexpr int slice_waves_per_eu() { return K >= 9 && K <= 14 ? 3 : 2; }
 
 template <int K>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(slice_waves_per_eu<K>())))
