@@ -10,7 +10,7 @@
 
 namespace motex {
 
-hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, bool dense, hipStream_t s);
 
 static const uint32_t kStructHitCap = 2048;  // >= one chunk: 64 lanes x 32 bits
 // Two Hamming boxes merge from their occurrence lists, which needs only the counted bits in
@@ -191,7 +191,9 @@ static int struct_launch(motex_batch *b, const void *items, uint32_t nitems, uin
     a.scratch = (uint32_t *)b->d_aux.p;
     size_t lds = struct_lds_bytes(a);
     if (lds > 65536) return set_error(MOTEX_EUNSUPPORTED, "structured kernel LDS window too large");
-    if ((e = launch_struct(a, grid, lds, s)) != hipSuccess) return hip_fail(e, "launch structured kernel");
+    // eight waves per SIMD for the list merge when the LDS lets at least 24 waves per CU in
+    const bool dense = a.distance == 0 && a.nb_gaps == 1 && lds * 24 <= 160 * 1024;
+    if ((e = launch_struct(a, grid, lds, dense, s)) != hipSuccess) return hip_fail(e, "launch structured kernel");
     return MOTEX_OK;
 }
 

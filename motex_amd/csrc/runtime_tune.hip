@@ -44,6 +44,11 @@ static TileShape default_tile(const motex_params &p, const SeqSet &ss)
         // text-parallel edit kernel: 100-200 VGPRs up to k = 14 (2-4 waves per SIMD
         // resident), 2 waves per SIMD beyond
         t.waves_per_cu = p.k <= 14 ? 12 : 8;
+    } else if (p.nb_gaps == 1 && p.distance == 0 && ss.sigma <= 8) {
+        // structured list merge on a small alphabet: the kernel is held to 8 waves per SIMD
+        t.blocks = 4;
+        t.rows = 256;
+        t.waves_per_cu = 32;
     } else {
         t.blocks = 4;
         t.rows = 256;

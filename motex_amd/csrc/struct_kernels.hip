@@ -197,8 +197,19 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
     if (!write_bm && lane == 0) *lcount = nl;
 }
 
-template <int P>
-__global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
+// DENSE: the register allocation is held to eight resident waves per SIMD (64 registers,
+// about 180 B of spills per lane), for launches whose LDS footprint lets that many waves in
+// and whose pairs take the list merge (one gap, Hamming boxes): there the pair sweeps are
+// short and their set-up latency-bound, so more waves beat the spills (1000 x 600 nt, boxes
+// 8+[10,30]+8: 18.2 ms at five waves, 16.6 at eight with the grid at 32 waves per CU).  Two
+// gaps (the bitmap merge) ran 1.6% slower and a protein alphabet (a larger LDS window, so
+// fewer waves fit anyway) 13% slower that way, so they keep the free allocation.
+template <bool DENSE>
+constexpr int struct_waves_per_eu() { return DENSE ? 8 : 1; }
+
+template <int P, bool DENSE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(struct_waves_per_eu<DENSE>())))
+void struct_kernel(StructArgs a)
 {
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x;
@@ -514,18 +525,25 @@ __global__ __launch_bounds__(64) void struct_kernel(StructArgs a)
     }
 }
 
-hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+template <bool DENSE>
+static void launch_struct_p(const StructArgs &a, uint32_t planes, uint32_t grid, size_t lds, hipStream_t s)
+{
+    if (planes <= 3) hipLaunchKernelGGL((struct_kernel<3, DENSE>), dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 4) hipLaunchKernelGGL((struct_kernel<4, DENSE>), dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 5) hipLaunchKernelGGL((struct_kernel<5, DENSE>), dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 6) hipLaunchKernelGGL((struct_kernel<6, DENSE>), dim3(grid), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((struct_kernel<7, DENSE>), dim3(grid), dim3(64), lds, s, a);
+}
+
+hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, bool dense, hipStream_t s)
 {
     uint32_t planes = 1;  // biased Hamming window counters of every shape (hd_device.h)
     for (uint32_t d = 0; d < a.ndist; ++d) {
         uint32_t p = hd_counter_planes(a.dlen[d], a.derr[d]);
         planes = p > planes ? p : planes;
     }
-    if (planes <= 3) hipLaunchKernelGGL(struct_kernel<3>, dim3(grid), dim3(64), lds, s, a);
-    else if (planes == 4) hipLaunchKernelGGL(struct_kernel<4>, dim3(grid), dim3(64), lds, s, a);
-    else if (planes == 5) hipLaunchKernelGGL(struct_kernel<5>, dim3(grid), dim3(64), lds, s, a);
-    else if (planes == 6) hipLaunchKernelGGL(struct_kernel<6>, dim3(grid), dim3(64), lds, s, a);
-    else hipLaunchKernelGGL(struct_kernel<7>, dim3(grid), dim3(64), lds, s, a);
+    if (dense) launch_struct_p<true>(a, planes, grid, lds, s);
+    else launch_struct_p<false>(a, planes, grid, lds, s);
     return hipGetLastError();
 }
 
