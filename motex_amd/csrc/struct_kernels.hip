@@ -152,11 +152,12 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
             const uint32_t res = ~planes_high_masked<P>(cnt, hm);
             if ((!decltype(warm)::value || px + 1 >= len) && __ballot(res != 0)) {
                 // flag t of this lane is text position pt = base + t; only [len - 1, n) counts
+                // valid flags t in [tlo, thi), both clamped to [0, 32]; the 64-bit shifts give
+                // the empty and full masks at the ends without selects
                 const int32_t base = rb + (int32_t)(32 * lane + px);
-                const int32_t tlo = (int32_t)len - 1 - base, thi = (int32_t)n - base;
-                uint32_t hv = res;
-                if (tlo > 0) hv = tlo >= 32 ? 0u : hv & (~0u << (uint32_t)tlo);
-                if (thi < 32) hv = thi <= 0 ? 0u : hv & ((1u << (uint32_t)thi) - 1u);
+                const uint32_t tlo = (uint32_t)min(max((int32_t)len - 1 - base, 0), 32);
+                const uint32_t thi = (uint32_t)min(max((int32_t)n - base, 0), 32);
+                uint32_t hv = res & (uint32_t)(~0ull << tlo) & ~(uint32_t)(~0ull << thi);
                 if (write_bm) {
                     while (hv) {
                         const uint32_t t = __builtin_ctz(hv);
