@@ -103,14 +103,14 @@ __device__ uint32_t struct_walk(const StructArgs &a, uint32_t *const bm[kMaxBoxe
 }
 
 // Hamming hits of one box shape for the pair, by the diagonal sweep, into its occurrence
-// list and (write_bm) into bm and bmT.  The sweep runs over blocks of 2048 diagonals and the
+// list and (WRITE_BM) into bm and bmT.  The sweep runs over blocks of 2048 diagonals and the
 // rows of a block in order, so the list is a sequence of runs sorted by pos_x, one per
 // block; when runs is not null, runs[0] receives their number and runs[1 + r] .. runs[2 + r]
 // bound run r.
-template <int P>
+template <int P, bool WRITE_BM>
 __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t len, uint32_t err,
                               uint32_t *bm, uint32_t *bmT, uint32_t *ne_l, const uint16_t *xoff,
-                              uint32_t *list, uint32_t *lcount, bool write_bm, uint32_t *runs)
+                              uint32_t *list, uint32_t *lcount, uint32_t *runs)
 {
     const uint32_t lane = threadIdx.x;
     const uint32_t m = I.m, n = I.n, wl = a.wl;
@@ -158,7 +158,7 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
                 const uint32_t tlo = (uint32_t)min(max((int32_t)len - 1 - base, 0), 32);
                 const uint32_t thi = (uint32_t)min(max((int32_t)n - base, 0), 32);
                 uint32_t hv = res & (uint32_t)(~0ull << tlo) & ~(uint32_t)(~0ull << thi);
-                if (write_bm) {
+                if constexpr (WRITE_BM) {
                     while (hv) {
                         const uint32_t t = __builtin_ctz(hv);
                         hv &= hv - 1;
@@ -191,11 +191,11 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
         for (; px < warm; ++px) row(px, 4 * zero_row, 0u, std::true_type{});
         for (; px < m; ++px) row(px, xoff[px - len], (px - len) & 31, std::false_type{});
         ++nruns;
-        if (write_bm) nl = __builtin_amdgcn_readfirstlane(*lcount);
+        if constexpr (WRITE_BM) nl = __builtin_amdgcn_readfirstlane(*lcount);
         if (runs && lane == 0) runs[1 + nruns] = min(nl, a.lcap);
     }
     if (runs && lane == 0) runs[0] = nruns;
-    if (!write_bm && lane == 0) *lcount = nl;
+    if (!WRITE_BM && lane == 0) *lcount = nl;
 }
 
 // DENSE: the register allocation is held to eight resident waves per SIMD (64 registers,
@@ -347,9 +347,12 @@ void struct_kernel(StructArgs a)
                 }
                 if (a.distance == 0) {
                     __syncthreads();
-                    struct_hd_box<P>(a, I, len, err, out, sym ? bmTd[d] : nullptr, ne_l, xoff,
-                                     lists + (size_t)d * a.lcap, lcount + d, !pairlist,
-                                     pairlist ? runs + 8 * d : nullptr);
+                    if (pairlist)
+                        struct_hd_box<P, false>(a, I, len, err, out, sym ? bmTd[d] : nullptr, ne_l, xoff,
+                                                lists + (size_t)d * a.lcap, lcount + d, runs + 8 * d);
+                    else
+                        struct_hd_box<P, true>(a, I, len, err, out, sym ? bmTd[d] : nullptr, ne_l, xoff,
+                                               lists + (size_t)d * a.lcap, lcount + d, nullptr);
                 } else {
                     EdOut eo{out, m};
                     if (len <= 14)
@@ -375,8 +378,8 @@ void struct_kernel(StructArgs a)
                 if (lane < a.ndist) lcount[lane] = 0;
                 for (uint32_t d = 0; d < a.ndist; ++d) {
                     __syncthreads();
-                    struct_hd_box<P>(a, I, a.dlen[d], a.derr[d], bmd[d], sym ? bmTd[d] : nullptr, ne_l, xoff,
-                                     lists + (size_t)d * a.lcap, lcount + d, true, nullptr);
+                    struct_hd_box<P, true>(a, I, a.dlen[d], a.derr[d], bmd[d], sym ? bmTd[d] : nullptr, ne_l,
+                                           xoff, lists + (size_t)d * a.lcap, lcount + d, nullptr);
                 }
                 __threadfence_block();
                 __syncthreads();
