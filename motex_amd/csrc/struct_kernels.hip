@@ -264,6 +264,12 @@ void struct_kernel(StructArgs a)
             const uint32_t *r1 = runs + 8 * d1;
             const uint32_t nr1 = __builtin_amdgcn_readfirstlane(r1[0]);
             const uint32_t lo = a.gmin[0] + a.blen[1], hi = a.gmax[0] + a.blen[1];
+            // The binary searches probe box 1's pos_x keys from LDS when they fit where the x
+            // table was (it is rebuilt for the next pair); the barrier below orders the copy.
+            const uint32_t c1 = __builtin_amdgcn_readfirstlane(lcount[d1]);
+            const bool lds_keys = c1 <= a.m_max;
+            if (lds_keys)
+                for (uint32_t i = lane; i < c1; i += 64) xoff[i] = (uint16_t)(l1[i] >> 16);
             for (int orient = 0; orient < (sym ? 2 : 1); ++orient) {
                 const uint32_t mo = orient ? n : m;
                 const uint32_t uvb = orient ? I.uv_base_t : I.uv_base;
@@ -281,7 +287,8 @@ void struct_kernel(StructArgs a)
                         uint32_t b = r1[1 + r], en = r1[2 + r];
                         while (b < en) {  // first entry of the run with pos_x >= xlo
                             const uint32_t mid = (b + en) >> 1;
-                            if ((l1[mid] >> 16) < xlo) b = mid + 1; else en = mid;
+                            const uint32_t key = lds_keys ? xoff[mid] : l1[mid] >> 16;
+                            if (key < xlo) b = mid + 1; else en = mid;
                         }
                         for (const uint32_t end = r1[2 + r]; b < end; ++b) {
                             const uint32_t e1 = l1[b];
