@@ -18,13 +18,15 @@ constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
 constexpr uint32_t kMaxRangeSeqs = 32;       // sequences per Hamming text range (one mask word)
 constexpr uint32_t kHitQueueCap = 512;       // Hamming hit queue entries per wave (global)
 constexpr uint32_t kSlicePipe = 4;           // waves per item of the pipelined text-parallel edit kernel
-// Edit kernel choice by the number of (x, group of 32 texts) items per CU, from sweeps of
-// 40-400 protein and DNA sequences of U[100,2000], k 8-14: below kSliceItemsPerCu the
-// one-text-per-lane kernel, below kSlicePipeItemsPerCu the pipelined text-parallel kernel,
-// otherwise one wave per item.  (The one-text-per-lane and pipelined kernels cross at about
-// 0.9 items per CU: 7.4 against 6.3 ms at 0.94, 9.0 against 6.4 ms at 1.56.)
+// Edit kernel choice, from sweeps of 40-500 protein and DNA sequences of U[100,2000], k 8-14:
+// below kSliceItemsPerCu (x, group of 32 texts) items per CU the one-text-per-lane kernel
+// (it and the pipelined kernel cross at about 0.9 items per CU: 7.4 against 6.3 ms at 0.94,
+// 9.0 against 6.4 ms at 1.56); above, the text-parallel kernel, pipelined or one wave per
+// item by kSlicePipeShare.
 constexpr uint32_t kSliceItemsPerCu = 1;
-constexpr uint32_t kSlicePipeItemsPerCu = 12;
+// Pipelined text-parallel kernel when a wave's share of all sweep steps is below this
+// fraction of the longest item's steps (runtime_ed.hip).
+constexpr double kSlicePipeShare = 0.5;
 
 // A text range of the Hamming work queue: whole sequences j0 .. j0+ny-1, or one part of a
 // long sequence j0 (ny = 1), covering text positions [ys, ye).

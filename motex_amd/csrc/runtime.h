@@ -137,6 +137,7 @@ struct motex_batch {
     uint32_t split_words = 0;
     uint32_t ed_nitems = 0;
     uint32_t grid_cap = 0;           // serving: bound on the persistent grid (0 = none)
+    uint32_t grid_share = 1;         // serving: the grid the plan can count on is 1/grid_share
     motex_stats stats{};
     hipStream_t stream = nullptr;   // overrides ctx->stream when set (serving)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // run start, end, main kernel start
@@ -163,7 +164,7 @@ namespace motex {
 
 inline hipStream_t bstream(const motex_batch *b) { return b->stream ? b->stream : b->ctx->stream; }
 int batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num,
-                 const motex_params *p, hipStream_t stream, motex_batch **out);
+                 const motex_params *p, hipStream_t stream, motex_batch **out, uint32_t grid_share = 1);
 int batch_launch(motex_batch *b);   // enqueue clear + batched kernel, no wait
 int batch_launch_graph(motex_batch *b);  // same, replayed from a HIP graph after the first run
 hipError_t batch_realloc_scratch(motex_batch *b, uint64_t bytes);  // grows d_aux, drops the graph

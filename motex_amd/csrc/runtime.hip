@@ -790,7 +790,7 @@ extern "C" int motex_batch_create(motex_ctx *c, const char *const *seqs, const u
 }
 
 int motex::batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num,
-                        const motex_params *p, hipStream_t stream, motex_batch **out)
+                        const motex_params *p, hipStream_t stream, motex_batch **out, uint32_t grid_share)
 {
     if (!c || !out) return set_error(MOTEX_EINVAL, "ctx/out is NULL");
     *out = nullptr;
@@ -802,6 +802,7 @@ int motex::batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *l
     std::unique_ptr<motex_batch> b(new motex_batch());
     b->ctx = c;
     b->stream = stream;
+    b->grid_share = std::max<uint32_t>(grid_share, 1);
     AllocStreamGuard guard(stream);
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess ||
         (e = hipEventCreate(&b->ev2)) != hipSuccess)

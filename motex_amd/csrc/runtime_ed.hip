@@ -136,10 +136,15 @@ static int ed_prepare_slice(motex_batch *b)
         for (uint32_t g = 0; g < ngroups; ++g)
             items.push_back({ss.start[i], ss.len[i], (uint32_t)ss.uv_off[i], g});
     }
+    // an item's cost: its sequential steps, strips x (rows + the 63-step skew)
+    uint64_t total_cost = 0, max_cost = 0;
     {
         std::vector<uint64_t> cost(items.size());
-        for (size_t t = 0; t < items.size(); ++t)
+        for (size_t t = 0; t < items.size(); ++t) {
             cost[t] = (uint64_t)((items[t].m + 63) / 64) * (goff[items[t].group + 1] - goff[items[t].group] + 63);
+            total_cost += cost[t];
+            max_cost = std::max(max_cost, cost[t]);
+        }
         const std::vector<uint32_t> order = order_by_cost_desc(cost);
         std::vector<EdSliceItem> sorted(order.size());
         for (size_t t = 0; t < order.size(); ++t) sorted[t] = items[order[t]];
@@ -148,12 +153,19 @@ static int ed_prepare_slice(motex_batch *b)
     b->tile = choose_tile(b->ctx, b->params, ss);
     b->ed_slice = true;
     b->ed_nitems = (uint32_t)items.size();
-    // pipelined when the items are too few to hide one item's sequential sweep (200-300
-    // protein sequences: 11-33 ms against 35-58 ms with one wave per item); with many items
-    // one wave per item has the higher throughput (500 proteins of U[200,2000], k=8: 44.9 against 69.8 ms)
+    // One wave per item takes about max(0.41 us x the longest item's steps, 0.61 us x a
+    // wave's share of all steps); the pipelined form (four waves per item, barriers between
+    // chunks) about max(0.41 us x longest / 3.5, 0.82 us x the share) (k=10 protein sweeps of
+    // 200-500 sequences, MI355X).  So the pipelined form wins when the share is below about
+    // half the longest item: there one item's sequential sweep is the critical path.
     const char *pipe_env = getenv("MOTEX_ED_PIPE");
-    if (pipe_env && (pipe_env[0] == '0' || pipe_env[0] == '1')) b->ed_pipe = pipe_env[0] == '1';
-    else b->ed_pipe = (uint64_t)items.size() < (uint64_t)b->ctx->num_cu * kSlicePipeItemsPerCu;
+    if (pipe_env && (pipe_env[0] == '0' || pipe_env[0] == '1')) {
+        b->ed_pipe = pipe_env[0] == '1';
+    } else {
+        const uint64_t waves = std::max<uint64_t>(
+            (uint64_t)b->ctx->num_cu * std::max<uint32_t>(b->tile.waves_per_cu, 1) / b->grid_share, 1);
+        b->ed_pipe = (double)total_cost < kSlicePipeShare * (double)max_cost * (double)waves;
+    }
     b->tile.variant = b->ed_pipe ? 3 : 2;
     uint32_t rows_max = 0;
     for (uint32_t g = 0; g < ngroups; ++g) rows_max = std::max(rows_max, goff[g + 1] - goff[g]);

@@ -126,7 +126,7 @@ static void dispatcher(motex_ctx *c)
             lk.unlock();
             motex_batch *b = nullptr;
             int rc = batch_create(c, jp->seqs.data(), jp->lens.data(), (uint32_t)jp->seqs.size(), &jp->params,
-                                  sv->streams[slot], &b);
+                                  sv->streams[slot], &b, 2);
             if (!rc) {
                 jp->batch = b;
                 b->grid_cap = std::max<uint32_t>(c->num_cu, (uint32_t)c->num_cu * b->tile.waves_per_cu / 2);
