@@ -27,6 +27,8 @@ constexpr uint32_t kSliceItemsPerCu = 1;
 // Pipelined text-parallel kernel when a wave's share of all sweep steps is below this
 // fraction of the longest item's steps (runtime_ed.hip).
 constexpr double kSlicePipeShare = 0.5;
+// Serving jobs (sharing the device): pipelined below this many items per CU.
+constexpr uint32_t kSlicePipeItemsPerCu = 12;
 
 // A text range of the Hamming work queue: whole sequences j0 .. j0+ny-1, or one part of a
 // long sequence j0 (ny = 1), covering text positions [ys, ye).

@@ -161,9 +161,13 @@ static int ed_prepare_slice(motex_batch *b)
     const char *pipe_env = getenv("MOTEX_ED_PIPE");
     if (pipe_env && (pipe_env[0] == '0' || pipe_env[0] == '1')) {
         b->ed_pipe = pipe_env[0] == '1';
+    } else if (b->grid_share > 1) {
+        // a serving job shares the device, where other jobs fill what its critical path
+        // leaves idle: it takes the higher-throughput one-wave form unless its items are too
+        // few to fill its share
+        b->ed_pipe = (uint64_t)items.size() < (uint64_t)b->ctx->num_cu * kSlicePipeItemsPerCu;
     } else {
-        const uint64_t waves = std::max<uint64_t>(
-            (uint64_t)b->ctx->num_cu * std::max<uint32_t>(b->tile.waves_per_cu, 1) / b->grid_share, 1);
+        const uint64_t waves = std::max<uint64_t>((uint64_t)b->ctx->num_cu * std::max<uint32_t>(b->tile.waves_per_cu, 1), 1);
         b->ed_pipe = (double)total_cost < kSlicePipeShare * (double)max_cost * (double)waves;
     }
     b->tile.variant = b->ed_pipe ? 3 : 2;
