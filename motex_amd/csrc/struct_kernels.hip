@@ -215,7 +215,8 @@ void struct_kernel(StructArgs a)
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x;
     const uint32_t wl = a.wl;
-    // LDS: NE window (sigma+1 rows) | x window offsets (m_max u16, bytes) | hit list | count
+    // LDS: NE window (sigma+1 rows) | x window offsets (m_max u16, bytes; during the list
+    // merge, box 1's pos_x keys) | hit list | count
     uint32_t *ne_l = smem;
     uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + (a.sigma + 1) * wl);
     uint32_t *hits = smem + (a.sigma + 1) * wl + (a.m_max + 1) / 2;
