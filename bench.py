@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--plain-x", type=int, default=4,
                     help="x sequences timed in the plain one-launch-per-pair form")
     ap.add_argument("--cpu-x", type=int, default=0,
-                    help="x sequences in the CPU baseline sample (0: sized for ~15 s)")
+                    help="x sequences in the CPU baseline sample (0: sized for ~12 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving stream (p50/p99 job latency)")
@@ -234,8 +234,10 @@ def main():
         if nx <= 0:
             cores = len(os.sched_getaffinity(0))
             per_x = float(np.mean(L) - w.k + 1) * float(L.sum())
-            rate = 0.4e9 * cores if w.distance == 0 else 0.06e9 * cores
-            nx = int(max(1, min(len(w.seqs), 15.0 * rate / per_x)))
+            # the oracle's cells per second per core on the box's host (measured: Hamming 1.5e9,
+            # structured Hamming 0.5e9, edit 0.13e9), so the sample takes about 12 s
+            per_core = 0.13e9 if w.distance else (0.5e9 if w.boxes else 1.4e9)
+            nx = int(max(1, min(len(w.seqs), 12.0 * per_core * cores / per_x)))
         out["cpu_baseline"] = cpu_baseline(w, gpu_u, gpu_v, off, nx)
     if not a.no_serving:
         from motex_amd import serving
