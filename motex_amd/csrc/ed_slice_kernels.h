@@ -11,8 +11,8 @@
 // word; lane 0 takes them from an LDS chunk of 64 rows holding the previous strip's last
 // column (the boundary stream, in the wave's scratch) and the group's text planes.  Lane 63
 // writes its cells to the next strip's boundary stream.  Past the first strip every column
-// has j > K, so only the word's planes travel and the receiver recounts the popcount of its
-// low K-1 planes (slice_cell_masked).
+// has j > K, so only the low K-1 planes of a word travel (all a truncated left or diagonal
+// candidate keeps) and the receiver recounts their popcount (slice_cell_masked).
 // Per column, the texts with an occurrence (OR of the hit masks over rows) and the number
 // of occurrences (popcount of the hit masks) go to u and v with one atomic each per strip.
 #pragma once
