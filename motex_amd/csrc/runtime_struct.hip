@@ -163,7 +163,10 @@ static int struct_fill_args(motex_batch *b, StructArgs &a)
     a.wl = b->params.distance == 0 ? b->hd.wl : 0;
     a.m_max = std::max<uint32_t>(maxlen, 1);
     const bool pair_lists = b->params.distance == 0 && b->params.nb_gaps == 1;
-    a.hcap = pair_lists ? kStructPairListLds : kStructHitCap;
+    // list merge: LDS keeps the counted bits (nb_structs x m_max), up to kStructPairListLds
+    // words; larger ones go to scratch
+    a.hcap = pair_lists ? (uint32_t)std::min<uint64_t>(kStructPairListLds, ((uint64_t)b->nb_structs * a.m_max + 31) / 32)
+                        : kStructHitCap;
     a.gcap = kStructHitCap;
     a.gather_scratch = pair_lists ? 1u : 0u;
     a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
