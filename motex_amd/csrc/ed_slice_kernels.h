@@ -134,7 +134,7 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
                                             uint32_t *sout, bool last, uint32_t *chunk)
 {
     typedef SliceLayout<K> Lay;
-    constexpr int SP = Lay::SP, LW = Lay::LW;
+    constexpr int LW = Lay::LW;
     const uint32_t lane = threadIdx.x;
     const uint32_t m = I.m;
     const uint32_t j = strip * 64 + lane + 1;  // 1-based column
@@ -304,7 +304,7 @@ __device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uin
                                            uint32_t lane, uint32_t *sout, bool emit)
 {
     typedef SliceLayout<K> Lay;
-    constexpr int SP = Lay::SP, LW = Lay::LW;
+    constexpr int LW = Lay::LW;
     const uint32_t *cr = chunk + ((s - 1) & 63) * LW;
     Slice<K> L;  // D[i][j-1]
     slice_left<K, FIRST>(L, c.U.d, cr);

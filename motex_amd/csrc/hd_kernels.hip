@@ -67,8 +67,7 @@ __global__ __launch_bounds__(256) void fold_fbits_kernel(const uint32_t *__restr
         const uint32_t x0 = (uint32_t)((uint64_t)nx * wv / 4), x1 = (uint32_t)((uint64_t)nx * (wv + 1) / 4);
         for (uint32_t x = x0; x < x1; ++x) {
             uint32_t carry = fbits[(size_t)x * f_words + w];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
+            for (int q = 0; q < 16; ++q) {  // stops where the carry dies out
                 if (!carry) break;
                 const uint32_t t = plane[q] & carry;
                 plane[q] ^= carry;

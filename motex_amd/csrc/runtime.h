@@ -67,8 +67,8 @@ struct DevBuf {
     void release()
     {
         if (p) {
-            if (s) hipFreeAsync(p, s);
-            else hipFree(p);
+            if (s) (void)hipFreeAsync(p, s);
+            else (void)hipFree(p);
         }
         p = nullptr;
         bytes = 0;
@@ -146,9 +146,9 @@ struct motex_batch {
     ~motex_batch()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
-        if (ev0) hipEventDestroy(ev0);
-        if (ev1) hipEventDestroy(ev1);
-        if (ev2) hipEventDestroy(ev2);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (ev2) (void)hipEventDestroy(ev2);
     }
     motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2, d_ranges, d_hitq;
     motex::DevBuf d_fbits;           // pair-symmetric Hamming runs: x-by-text occurrence bits

@@ -199,7 +199,6 @@ uint32_t total_motif_length(const motex_params *p)
 // ---------------------------------------------------------------------------------------
 // Hamming work-queue planning
 // ---------------------------------------------------------------------------------------
-static int64_t floor32(int64_t v) { return v >= 0 ? (v & ~31LL) : -(((-v) + 31) & ~31LL); }
 
 void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
                    const std::vector<uint32_t> &xs, const std::vector<int32_t> *only_y,
@@ -364,11 +363,12 @@ extern "C" void motex_ctx_destroy(motex_ctx *c)
 {
     if (!c) return;
     serving_shutdown(c);
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamDestroy(c->stream);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->ev2) hipEventDestroy(c->ev2);
+    // destroy: nothing to report to
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
     delete c;
 }
 
@@ -656,8 +656,8 @@ int batch_finish(motex_batch *b)
     hipError_t e;
     if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return hip_fail(e, "extraction kernel");
     float ms = 0, kms = 0;
-    hipEventElapsedTime(&ms, b->ev0, b->ev1);
-    hipEventElapsedTime(&kms, b->ev2, b->ev1);
+    if (hipEventElapsedTime(&ms, b->ev0, b->ev1) != hipSuccess) ms = 0;
+    if (hipEventElapsedTime(&kms, b->ev2, b->ev1) != hipSuccess) kms = 0;
     b->stats.kernel_ms = ms;
     b->stats.main_kernel_ms = kms;
     return MOTEX_OK;
@@ -675,7 +675,6 @@ static uint32_t plain_blocks(uint32_t n)
 
 int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32_t>> &pairs, double *ms_out)
 {
-    motex_ctx *c = b->ctx;
     hipStream_t s = bstream(b);
     hipError_t e;
     const SeqSet &ss = b->ss;
@@ -719,7 +718,7 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
     if ((e = hd_alloc_hitq(hitq, max_count, 4)) != hipSuccess) return hip_fail(e, "alloc plain hit queues");
     HdPlan shape;
     shape.rows_max = all.rows_max;
-    hipEventRecord(b->ev0, s);
+    if ((e = hipEventRecord(b->ev0, s)) != hipSuccess) return hip_fail(e, "record plain start");
     for (const Launch &L : launches) {
         shape.wl = hd_lds_words(L.blocks, shape.rows_max);
         HdArgs a = hd_args(b, shape, items, 0);
@@ -730,10 +729,10 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
         a.hitq = (uint32_t *)hitq.p;
         if ((e = launch_hd(a, L.blocks, L.count, s)) != hipSuccess) return hip_fail(e, "launch plain kernel");
     }
-    hipEventRecord(b->ev1, s);
+    if ((e = hipEventRecord(b->ev1, s)) != hipSuccess) return hip_fail(e, "record plain end");
     if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return hip_fail(e, "plain kernel");
     float ms = 0;
-    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    if ((e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) return hip_fail(e, "plain timing");
     if (ms_out) *ms_out = ms;
     return MOTEX_OK;
 }
@@ -842,7 +841,7 @@ int motex::batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *l
 extern "C" int motex_batch_run(motex_batch *b)
 {
     if (!b) return set_error(MOTEX_EINVAL, "batch is NULL");
-    hipSetDevice(b->ctx->device);
+    if (hipError_t e = hipSetDevice(b->ctx->device); e != hipSuccess) return hip_fail(e, "hipSetDevice");
     int rc = batch_launch_graph(b);
     return rc ? rc : batch_finish(b);
 }
@@ -850,7 +849,7 @@ extern "C" int motex_batch_run(motex_batch *b)
 extern "C" int motex_batch_clear(motex_batch *b)
 {
     if (!b) return set_error(MOTEX_EINVAL, "batch is NULL");
-    hipSetDevice(b->ctx->device);
+    if (hipError_t e = hipSetDevice(b->ctx->device); e != hipSuccess) return hip_fail(e, "hipSetDevice");
     int rc = batch_clear(b);
     if (rc) return rc;
     hipError_t e = hipStreamSynchronize(bstream(b));
@@ -860,7 +859,7 @@ extern "C" int motex_batch_clear(motex_batch *b)
 extern "C" int motex_batch_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x)
 {
     if (!b) return set_error(MOTEX_EINVAL, "batch is NULL");
-    hipSetDevice(b->ctx->device);
+    if (hipError_t e = hipSetDevice(b->ctx->device); e != hipSuccess) return hip_fail(e, "hipSetDevice");
     if (b->params.nb_gaps) return struct_run_plain(b, first_x, num_x);
     if (b->params.distance == 0) return hd_run_plain(b, first_x, num_x);
     return ed_run_plain(b, first_x, num_x);
@@ -869,7 +868,7 @@ extern "C" int motex_batch_run_plain(motex_batch *b, uint32_t first_x, uint32_t 
 extern "C" int motex_batch_download(motex_batch *b, uint32_t *const *u, uint32_t *const *v)
 {
     if (!b || !u || !v) return set_error(MOTEX_EINVAL, "NULL argument");
-    hipSetDevice(b->ctx->device);
+    if (hipError_t e = hipSetDevice(b->ctx->device); e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const SeqSet &ss = b->ss;
     uint64_t n = ss.uv_off[ss.num];
     std::vector<uint32_t> hu(n), hv(n);
@@ -898,7 +897,7 @@ extern "C" int motex_batch_stats(const motex_batch *b, motex_stats *out)
 extern "C" void motex_batch_destroy(motex_batch *b)
 {
     if (!b) return;
-    hipSetDevice(b->ctx->device);
+    (void)hipSetDevice(b->ctx->device);  // destroy: nothing to report to
     delete b;
 }
 

@@ -243,7 +243,6 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
                      uint32_t persistent, hipStream_t s)
 {
     hipError_t e;
-    const SeqSet &ss = b->ss;
     bool wide = ed_wide(b->params);
     uint32_t stream_words = ed_stream_words(b);
     uint64_t need = (uint64_t)grid * 2 * stream_words * 4;
@@ -297,7 +296,6 @@ int ed_launch_all(motex_batch *b)
 // Plain form: one launch per pair (one work item, one wave).
 static int ed_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32_t>> &pairs, double *ms_out)
 {
-    motex_ctx *c = b->ctx;
     hipStream_t s = bstream(b);
     hipError_t e;
     const SeqSet &ss = b->ss;
@@ -312,15 +310,15 @@ static int ed_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t,
     if ((e = items.alloc(all.size() * sizeof(EdItemDev))) != hipSuccess) return hip_fail(e, "alloc items");
     if ((e = hipMemcpyAsync(items.p, all.data(), all.size() * sizeof(EdItemDev), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
-    hipEventRecord(b->ev0, s);
+    if ((e = hipEventRecord(b->ev0, s)) != hipSuccess) return hip_fail(e, "record plain start");
     for (size_t t = 0; t < all.size(); ++t) {
         int rc = ed_launch(b, (const EdItemDev *)items.p + t, 1, 1, 0, s);
         if (rc) return rc;
     }
-    hipEventRecord(b->ev1, s);
+    if ((e = hipEventRecord(b->ev1, s)) != hipSuccess) return hip_fail(e, "record plain end");
     if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return hip_fail(e, "plain edit kernel");
     float ms = 0;
-    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    if ((e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) return hip_fail(e, "plain timing");
     if (ms_out) *ms_out = ms;
     return MOTEX_OK;
 }

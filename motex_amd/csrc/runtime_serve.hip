@@ -90,7 +90,7 @@ static void finish_job(motex_ctx *c, Job &j)
 
 static void dispatcher(motex_ctx *c)
 {
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);  // the context's creation selected this device already
     Serving *sv = serving_of(c);
     std::vector<std::shared_ptr<Job>> running;
     std::unique_lock<std::mutex> lk(c->mu);
@@ -184,7 +184,7 @@ static int ensure_worker(motex_ctx *c)
     for (int s = 0; s < kSlots; ++s) {
         hipError_t e = hipStreamCreateWithFlags(&sv->streams[s], hipStreamNonBlocking);
         if (e != hipSuccess) {
-            for (int t = 0; t < s; ++t) hipStreamDestroy(sv->streams[t]);
+            for (int t = 0; t < s; ++t) (void)hipStreamDestroy(sv->streams[t]);
             delete sv;
             return hip_fail(e, "serving stream");
         }
@@ -216,7 +216,7 @@ void serving_shutdown(motex_ctx *c)
         g_serving.erase(c);
     }
     if (sv) {
-        for (int s = 0; s < kSlots; ++s) hipStreamDestroy(sv->streams[s]);
+        for (int s = 0; s < kSlots; ++s) (void)hipStreamDestroy(sv->streams[s]);
         delete sv;
     }
 }

@@ -235,7 +235,6 @@ int struct_launch_all(motex_batch *b)
 
 static int struct_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32_t>> &pairs, double *ms_out)
 {
-    motex_ctx *c = b->ctx;
     hipStream_t s = bstream(b);
     hipError_t e;
     std::vector<StructItem> items;
@@ -246,15 +245,15 @@ static int struct_launch_pairs(motex_batch *b, const std::vector<std::pair<uint3
     if ((e = d.alloc(items.size() * sizeof(StructItem))) != hipSuccess) return hip_fail(e, "alloc items");
     if ((e = hipMemcpyAsync(d.p, items.data(), items.size() * sizeof(StructItem), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
-    hipEventRecord(b->ev0, s);
+    if ((e = hipEventRecord(b->ev0, s)) != hipSuccess) return hip_fail(e, "record plain start");
     for (size_t t = 0; t < items.size(); ++t) {
         int rc = struct_launch(b, (const StructItem *)d.p + t, 1, 1, 0, s);
         if (rc) return rc;
     }
-    hipEventRecord(b->ev1, s);
+    if ((e = hipEventRecord(b->ev1, s)) != hipSuccess) return hip_fail(e, "record plain end");
     if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return hip_fail(e, "plain structured kernel");
     float ms = 0;
-    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    if ((e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) return hip_fail(e, "plain timing");
     if (ms_out) *ms_out = ms;
     return MOTEX_OK;
 }

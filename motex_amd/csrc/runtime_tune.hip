@@ -125,7 +125,7 @@ extern "C" int motex_autotune(motex_ctx *c, const char *const *seqs, const uint3
     if (!seqs || !lens || !num_seqs) return set_error(MOTEX_EINVAL, "autotune needs a sample of sequences");
     int rc = validate_params(p);
     if (rc) return rc;
-    hipSetDevice(c->device);
+    if (hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(e, "hipSetDevice");
     uint32_t nb_structs = 1;
     enumerate_structs(p, &nb_structs);
     SeqSet ss;
