@@ -81,7 +81,8 @@ typedef struct motex_stats {
     uint32_t tile_blocks;   /* autotuned tile shape used by the batched kernel */
     uint32_t tile_rows;
     uint32_t waves_per_cu;
-    uint32_t kernel_variant;
+    uint32_t kernel_variant;  /* edit distance: 0/1 one text per lane (32/64-bit words),
+                                 2 text-parallel one wave per item, 3 text-parallel pipelined */
     uint32_t graph_replay;  /* 1 if the last run replayed the batch's HIP graph */
     uint32_t symmetric;     /* 1 if each unordered pair is computed once and credited to
                                both orders (Hamming; cells still count every ordered pair) */
