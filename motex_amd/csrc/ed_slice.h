@@ -50,6 +50,7 @@ struct Slice {
     uint32_t d[SliceW<K>::W];
     uint32_t pc[kSlicePc];  // popcount of the word
     uint32_t pm[kSlicePc];  // popcount of bits 0 .. K-2
+    uint32_t hi;            // any of bits K-1 .. W-1 (slice_cell_masked, which keeps only this)
 };
 
 // sel ? t : f, per bit (one 3-input logic op on gfx950: the compiler's own choice for this
@@ -263,24 +264,27 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
     if (dropped) *dropped |= slice_mux(sel, c[W], d[W]);  // host check of the width bound
     for (int q = 0; q < kSlicePc; ++q) D.pc[q] = slice_mux(sel, pcc[q], pd[q]);
     slice_popcount<K - 1>(D.d, D.pm);
+    D.hi = 0;  // kept by the masked cell only
 }
 
 // The cell for columns j > K, where left and diag enter truncated: their candidates have no
 // plane at or above K, so only pm of left and diag and pc of up are needed, and the result's
-// pm is left to the reader (slice_pm).  up: d, pc; left, diag: d, pm.  Returns d, pc.
+// pm is left to the reader (slice_pm).  up: planes 0..K-2, uhi, pc; left, diag: planes
+// 0..K-2, pm.  Returns planes 0..K-1, dhi and pc.
 //
 // Planes K..W of the candidates are up's alone (left and diag are zero there), so both
 // comparisons decide those planes from one OR of them: in the ripple, a plane where the
-// other side is zero turns "better so far" off wherever up's plane is set, and the planes of
-// the first choice d are up's masked by "not left".  The selected word's planes K..W-1 are
-// up's where neither left nor diag was chosen.
+// other side is zero turns "better so far" off wherever up's plane is set.  That OR is uhi,
+// the OR of up's planes K-1..W-1, and it is all the cell needs of them: the selected word's
+// planes K..W-1 are up's planes K-1..W-2 where neither left nor diag won, and up's plane W-1
+// is never kept (selected words stay below 2K+1 bits), so the result's OR of planes
+// K-1..W-1 is dhi = plane K-1 | (uhi where neither won).  Words past the first strip thus
+// carry K planes and one OR instead of 2K planes.
 template <int K>
-MOTEX_HD void slice_cell_masked(const uint32_t *ud, const uint32_t *upc, const uint32_t *ld,
+MOTEX_HD void slice_cell_masked(const uint32_t *ud, uint32_t uhi, const uint32_t *upc, const uint32_t *ld,
                                 const uint32_t *lpm, const uint32_t *gd, const uint32_t *gpm,
-                                uint32_t delta, uint32_t *dd, uint32_t *dpc,
-                                uint32_t *dropped = nullptr)
+                                uint32_t delta, uint32_t *dd, uint32_t &dhi, uint32_t *dpc)
 {
-    constexpr int W = SliceW<K>::W;
     // candidate planes 0..K-1: a = shift(up) | 1, b = shiftc(left) | 1, c = shiftc(diag) | delta
     uint32_t a[K], b[K], c[K];
     a[0] = ~0u;
@@ -291,16 +295,13 @@ MOTEX_HD void slice_cell_masked(const uint32_t *ud, const uint32_t *upc, const u
         b[p] = ld[p - 1];
         c[p] = gd[p - 1];
     }
-    // any of a's planes K..W (= up's planes K-1..W-1)
-    uint32_t hi = 0;
-    for (int p = K - 1; p < W; p += 2) hi = slice_or3(hi, ud[p], p + 1 < W ? ud[p + 1] : 0u);
     uint32_t pcc[kSlicePc];
     slice_inc(gpm, delta, pcc);
     // min*: start from a, replace where b then c is strictly better.  a and b both set plane
     // 0 and add one to their source's popcount, so they compare as their sources do.
     uint32_t r = 0;
     for (int p = 1; p < K; ++p) r = slice_gt_step(b[p], a[p], r);
-    r = slice_andnot(r, hi);
+    r = slice_andnot(r, uhi);
     for (int q = 0; q < kSlicePc; ++q) r = slice_lt_step(lpm[q], upc[q], r);
     const uint32_t sel = r;
     uint32_t d[K], pd[kSlicePc], ps[kSlicePc];
@@ -311,12 +312,11 @@ MOTEX_HD void slice_cell_masked(const uint32_t *ud, const uint32_t *upc, const u
     // c against d: d's planes K..W are a's where b was not chosen
     r = 0;
     for (int p = 0; p < K; ++p) r = slice_gt_step(c[p], d[p], r);
-    r = slice_andnot_unless(r, hi, sel);
+    r = slice_andnot_unless(r, uhi, sel);
     for (int q = 0; q < kSlicePc; ++q) r = slice_lt_step(pcc[q], pd[q], r);
     const uint32_t sel2 = r;
     for (int p = 0; p < K; ++p) dd[p] = slice_mux(sel2, c[p], d[p]);
-    for (int p = K; p < W; ++p) dd[p] = slice_andnot2(ud[p - 1], sel, sel2);
-    if (dropped) *dropped |= slice_andnot2(ud[W - 1], sel, sel2);
+    dhi = dd[K - 1] | slice_andnot2(uhi, sel, sel2);
     for (int q = 0; q < kSlicePc; ++q) dpc[q] = slice_mux(sel2, pcc[q], pd[q]);
 }
 
@@ -339,6 +339,7 @@ MOTEX_HD void slice_row0(uint32_t j, Slice<K> &D)
         D.pc[q] = ((r >> q) & 1) ? ~0u : 0u;
         D.pm[q] = ((rm >> q) & 1) ? ~0u : 0u;
     }
+    D.hi = r >= (uint32_t)K ? ~0u : 0u;
 }
 
 // Column 0: the zero word.

@@ -173,7 +173,7 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
             if constexpr (FIRST) {
                 slice_cell<K>(U, L, G, ne, full, D);
             } else {
-                slice_cell_masked<K>(U.d, U.pc, L.d, L.pm, G.d, G.pm, ne, D.d, D.pc);
+                slice_cell_masked<K>(U.d, U.hi, U.pc, L.d, L.pm, G.d, G.pm, ne, D.d, D.hi, D.pc);
             }
             if (report) {
                 const uint32_t hit = slice_le_planes(D.pc, eb) & t[5];
@@ -321,7 +321,7 @@ __device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uin
         if constexpr (FIRST) {
             slice_cell<K>(c.U, L, c.G, ne, c.full, D);
         } else {
-            slice_cell_masked<K>(c.U.d, c.U.pc, L.d, L.pm, c.G.d, c.G.pm, ne, D.d, D.pc);
+            slice_cell_masked<K>(c.U.d, c.U.hi, c.U.pc, L.d, L.pm, c.G.d, c.G.pm, ne, D.d, D.hi, D.pc);
         }
         if (c.report) {
             const uint32_t hit = slice_le_planes(D.pc, c.eb) & t[5];

@@ -78,13 +78,20 @@ static int run(uint32_t e, uint32_t m, uint32_t nmin, uint32_t nmax, uint32_t si
                 pc_bad |= cnt != held || cnt > (uint32_t)K;
             }
             if (j > (uint32_t)K) {
-                // the truncated-column form must give the same word and popcounts
+                // the truncated-column form must give the same low planes, popcounts and OR of
+                // planes K-1..W-1, from up's low planes and that OR alone
+                constexpr int W = SliceW<K>::W;
+                uint32_t uhi = 0, dhi_full = 0;
+                for (int p = K - 1; p < W; ++p) {
+                    uhi |= prev[j].d[p];
+                    dhi_full |= cur[j].d[p];
+                }
                 Slice<K> alt;
-                uint32_t dr2 = 0;
-                slice_cell_masked<K>(prev[j].d, prev[j].pc, cur[j - 1].d, cur[j - 1].pm, prev[j - 1].d,
-                                     prev[j - 1].pm, delta, alt.d, alt.pc, &dr2);
+                slice_cell_masked<K>(prev[j].d, uhi, prev[j].pc, cur[j - 1].d, cur[j - 1].pm, prev[j - 1].d,
+                                     prev[j - 1].pm, delta, alt.d, alt.hi, alt.pc);
                 slice_pm<K>(alt.d, alt.pm);
-                for (int p = 0; p < SliceW<K>::W; ++p) masked_bad |= alt.d[p] != cur[j].d[p];
+                for (int p = 0; p < K; ++p) masked_bad |= alt.d[p] != cur[j].d[p];
+                masked_bad |= alt.hi != dhi_full;
                 for (int q = 0; q < kSlicePc; ++q)
                     masked_bad |= alt.pc[q] != cur[j].pc[q] || alt.pm[q] != cur[j].pm[q];
             }
