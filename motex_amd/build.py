@@ -27,9 +27,6 @@ LIB_SOURCES = ["hd_kernels.hip", "ed_kernels.hip", "ed_slice_kernels.hip", "ed_s
                "ed_slice_kernels_k23.hip", "struct_kernels.hip", "runtime.hip",
                "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip"]
 CLI_SOURCES = ["cli/main.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/switches.cpp"]
-# Stand-alone micro-benchmarks that calibrate the roofline (motex_amd/bin/<name>).
-PROBES = os.path.join(HERE, "probes")
-PROBE_SOURCES = ["valu_issue_rate.hip", "atomic_rate.hip"]
 
 
 def _run(cmd):
@@ -56,12 +53,6 @@ def _compile_cli(src):
     return out
 
 
-def _compile_probe(src):
-    out = os.path.join(os.path.dirname(BIN), os.path.splitext(src)[0])
-    _run([HIPCC, "-O3", "-std=c++17", "-Wno-unused-result", *DEVICE, os.path.join(PROBES, src), "-o", out])
-    return out
-
-
 def build(verbose: bool = False) -> tuple[str, str]:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
@@ -72,7 +63,6 @@ def build(verbose: bool = False) -> tuple[str, str]:
     with cf.ThreadPoolExecutor(workers) as ex:
         lib_objs = list(ex.map(_compile_lib, lib_srcs))
         cli_objs = list(ex.map(_compile_cli, cli_srcs))
-        list(ex.map(_compile_probe, [p for p in PROBE_SOURCES if os.path.exists(os.path.join(PROBES, p))]))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *lib_objs, "-lpthread"])
     os.replace(tmp, LIB)

@@ -240,13 +240,15 @@ void hd_batched_kernel(HdArgs a)
     const uint32_t wl = a.wl;
     // LDS: NE window (sigma rows; at offset 0 so that the block offsets fit the read
     //      instructions' immediates) | per-row v, sequence mask | range sequence starts |
-    //      per-row NE byte offsets (2 spare entries for the prefetch) | per-row u (16 bit)
+    //      per-row NE byte offsets (2 spare entries for the prefetch) | per-row u
     uint32_t *ne_l = smem;
     uint32_t *acc_v = ne_l + a.sigma * wl;
     uint32_t *ymask = acc_v + a.rows_max;
     uint32_t *starts = ymask + a.rows_max;
     uint32_t *xb = starts + kMaxRangeSeqs + 1;
-    uint16_t *acc_u = reinterpret_cast<uint16_t *>(xb + a.rows_max + 2);
+    // u counts distinct texts per row over all of the item's ranges: 32 bits, since an item
+    // may span more texts than 16 bits can count (short sequences in large inputs)
+    uint32_t *acc_u = xb + a.rows_max + 2;
     const uint32_t lane4 = lane * 4;
     // this wave's hit queue: tags, then B words per entry (16-byte aligned)
     uint32_t *hq_meta = a.hitq + (size_t)blockIdx.x * a.hitq_cap * (B + 1);
@@ -418,7 +420,7 @@ void hd_batched_kernel(HdArgs a)
                 const uint32_t m = ymask[r];
                 if (m) {
                     if (I.split != kNoSplit) acc_u[r] = 1;
-                    else acc_u[r] = (uint16_t)(acc_u[r] + __builtin_popcount(m));
+                    else acc_u[r] += __builtin_popcount(m);
                     ymask[r] = 0;
                 }
             }
@@ -492,9 +494,9 @@ static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream
 
 size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max)
 {
-    // NE window | v, sequence mask | range starts | row offsets (+2) | u (16 bit)
+    // NE window | v, sequence mask | range starts | row offsets (+2) | u
     return (size_t)sigma * wl * 4 + (size_t)rows_max * 8 + (kMaxRangeSeqs + 1) * 4 +
-           ((size_t)rows_max + 2) * 4 + (size_t)rows_max * 2;
+           ((size_t)rows_max + 2) * 4 + (size_t)rows_max * 4;
 }
 
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)

@@ -72,18 +72,26 @@ static size_t footprint(const Job &j, const motex_ctx *c)
     return (size_t)(bytes * 1.25);
 }
 
-static void finish_job(motex_ctx *c, Job &j)
+// Waits for a running job, accumulates its u/v into the caller's arrays and frees its
+// device memory -- all without the context lock -- then publishes the outcome under the lock
+// held by `lk`, so that motex_poll/motex_wait/motex_job_times (which read these fields under
+// the same lock) see the caller's u/v written before the state becomes 2.
+static void finish_job(motex_ctx *c, Job &j, std::unique_lock<std::mutex> &lk)
 {
     Serving *sv = serving_of(c);
+    lk.unlock();
     int rc = batch_finish(j.batch);
     if (!rc) rc = motex_batch_download(j.batch, j.u.data(), j.v.data());
-    j.device_ms = j.batch->stats.kernel_ms;
-    j.cells = j.batch->stats.cells;
+    const double device_ms = j.batch->stats.kernel_ms, cells = j.batch->stats.cells;
+    std::string err = rc ? motex_last_error() : "";
     motex_batch_destroy(j.batch);
+    lk.lock();
     j.batch = nullptr;
     sv->busy[j.slot] = false;
+    j.device_ms = device_ms;
+    j.cells = cells;
     j.rc = rc;
-    if (rc) j.err = motex_last_error();
+    j.err = err;
     j.t_done = now_ms();
     j.state = rc ? 3 : 2;
 }
@@ -101,9 +109,7 @@ static void dispatcher(motex_ctx *c)
         for (size_t t = 0; t < running.size();) {
             Job &j = *running[t];
             if (hipEventQuery(j.batch->ev1) == hipSuccess) {
-                lk.unlock();
-                finish_job(c, j);
-                lk.lock();
+                finish_job(c, j, lk);
                 c->resident_bytes -= j.footprint;
                 running.erase(running.begin() + t);
                 progressed = true;
@@ -154,11 +160,7 @@ static void dispatcher(motex_ctx *c)
             c->cv.wait_for(lk, std::chrono::microseconds(200));
     }
     // shut down: wait for running jobs, fail pending ones
-    for (auto &jp : running) {
-        lk.unlock();
-        finish_job(c, *jp);
-        lk.lock();
-    }
+    for (auto &jp : running) finish_job(c, *jp, lk);
     for (auto &jp : c->pending) {
         jp->state = 3;
         jp->rc = MOTEX_EDEVICE;
@@ -177,8 +179,11 @@ static Serving *serving_of(motex_ctx *c)
     return g_serving[c];
 }
 
+static std::mutex g_start_mu;  // serialises the first submits of a context
+
 static int ensure_worker(motex_ctx *c)
 {
+    std::lock_guard<std::mutex> g0(g_start_mu);
     if (c->worker_started) return MOTEX_OK;
     Serving *sv = new Serving();
     for (int s = 0; s < kSlots; ++s) {
@@ -201,6 +206,7 @@ static int ensure_worker(motex_ctx *c)
 
 void serving_shutdown(motex_ctx *c)
 {
+    std::lock_guard<std::mutex> g0(g_start_mu);
     if (!c->worker_started) return;
     {
         std::lock_guard<std::mutex> lk(c->mu);

@@ -73,3 +73,25 @@ def test_quorum_zero_prints_every_factor(tmp_path, oracle):
     w = synth.dna50_hamming(scale=0.1)
     w.q, w.N = 0, 2000000000
     run_both(tmp_path, oracle, w, smile=False)
+
+
+def test_long_mode_flag(tmp_path, oracle):
+    # -L 1 on a 20-knt prefix of the long-sequence workload, every distinct factor printed
+    w = synth.dna4m_long(scale=0.005)
+    w.q, w.N = 0, 2000000000
+    assert "-L" in w.cli_args()
+    run_both(tmp_path, oracle, w, smile=False)
+
+
+def test_reference_structured_example(tmp_path, oracle):
+    # data/README step 6, structured: -d 0 -k 8 -e 1 -q 7 -s data/boxes.txt (on the first
+    # 250 B. subtilis regions)
+    from motex_amd import refdata
+    w = synth.Workload("subtilis", "DNA", refdata.subtilis()[:250], 0, 8, 1, 7, boxes=refdata.boxes())
+    run_both(tmp_path, oracle, w)
+
+
+def test_hsapiens_genes(tmp_path, oracle):
+    from motex_amd import refdata
+    w = refdata.hsapiens(100)
+    run_both(tmp_path, oracle, w)

@@ -122,3 +122,23 @@ def test_invalid_params_raise(ctx):
         ctx.extract_all_pairs([b"ACGT"], MotexParams(k=4, e=4))
     with pytest.raises(motex_amd.MotexError):
         ctx.extract_all_pairs([b"ACGT"], MotexParams(k=4, e=1, distance=2))
+
+
+def test_u_counts_beyond_16_bits(ctx):
+    """100,000 copies of AAAA, k=4, e=0: the one factor of every sequence occurs once in every
+    sequence, so u = v = N exactly.  Above 65,535 sequences the ordered plan runs, and one
+    work item then spans more than 65,535 texts: the per-row distinct-text count must not
+    wrap (the reference keeps u as unsigned int, motex.cc:495-529)."""
+    N = 100_000
+    seqs = [b"AAAA"] * N
+    print(f"[hd u-width] n={N} k=4 e=0", flush=True)
+    b = ctx.batch(seqs, MotexParams(k=4, e=0))
+    st = b.run()
+    u, v, off = b.download()
+    b.close()
+    assert st["symmetric"] == 0
+    u = u.reshape(N, 4)
+    v = v.reshape(N, 4)
+    assert np.all(u[:, :3] == 0) and np.all(v[:, :3] == 0)
+    assert np.all(u[:, 3] == N), np.unique(u[:, 3])
+    assert np.all(v[:, 3] == N), np.unique(v[:, 3])
