@@ -125,6 +125,25 @@ int motex_extract_all_pairs(motex_ctx *ctx, const char *const *seqs, const uint3
 int motex_extract_pair(motex_ctx *ctx, const char *x, uint32_t m, const char *y, uint32_t n,
                        const motex_params *params, uint32_t *u, uint32_t *v);
 
+/* --- extraction without device allocation (no counterpart in the reference) -------- */
+
+/* Bytes of the per-device scratch array the no-allocation entry point works in (64 MiB). */
+#define MOTEX_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* motex_extract_all_pairs for Hamming single motifs (distance 0, nb_gaps 0) that calls no
+ * hipMalloc/hipFree and no allocating device library routine: every device buffer of the
+ * run comes from one static array of MOTEX_NOALLOC_SCRATCH_BYTES per device, which each GPU
+ * holds from the context's creation until the process ends.  For callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, accumulation and results as motex_extract_all_pairs; MOTEX_ENOMEM when the
+ * input needs more scratch than the array holds (about 4 M nt in 1000-nt sequences),
+ * MOTEX_EUNSUPPORTED for edit distance or structured motifs.  The array keeps the last call's
+ * contents; calls on one device share it, so the library runs them one at a time within a
+ * process, and calls from several processes on one GPU race. */
+int motex_extract_all_pairs_noalloc(motex_ctx *ctx, const char *const *seqs, const uint32_t *lens,
+                                    uint32_t num_seqs, const motex_params *params,
+                                    uint32_t *const *u, uint32_t *const *v);
+
 /* --- device-resident batches (the timed form: inputs already in HBM) --------------- */
 
 /* Uploads and packs the sequences, builds the work queue and picks tile shapes. */

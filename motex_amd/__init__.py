@@ -63,6 +63,8 @@ SIGNATURES = [
                                           C.POINTER(Params), C.POINTER(u32p), C.POINTER(u32p)]),
     ("motex_extract_pair", C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32,
                                      C.POINTER(Params), u32p, u32p]),
+    ("motex_extract_all_pairs_noalloc", C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), u32p, C.c_uint32,
+                                                  C.POINTER(Params), C.POINTER(u32p), C.POINTER(u32p)]),
     ("motex_batch_create", C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), u32p, C.c_uint32,
                                      C.POINTER(Params), C.POINTER(C.c_void_p)]),
     ("motex_batch_run", C.c_int, [C.c_void_p]),
@@ -179,6 +181,18 @@ class Context:
         p, keep = params.to_c()
         _check(lib().motex_extract_all_pairs(self.h, sa, lens.ctypes.data_as(u32p), len(seqs),
                                              C.byref(p), _row_ptrs(u, off), _row_ptrs(v, off)))
+        return u[: int(off[-1])], v[: int(off[-1])], off
+
+    def extract_all_pairs_noalloc(self, seqs: list[bytes], params: MotexParams):
+        """Hamming all-pairs form that allocates no device memory (works in the library's
+        static per-device scratch array).  Returns flat (u, v, offsets)."""
+        off = uv_layout(seqs, params.nb_structs)
+        u = np.zeros(max(int(off[-1]), 1), np.uint32)
+        v = np.zeros(max(int(off[-1]), 1), np.uint32)
+        sa, lens = _seq_args(seqs)
+        p, keep = params.to_c()
+        _check(lib().motex_extract_all_pairs_noalloc(self.h, sa, lens.ctypes.data_as(u32p), len(seqs),
+                                                     C.byref(p), _row_ptrs(u, off), _row_ptrs(v, off)))
         return u[: int(off[-1])], v[: int(off[-1])], off
 
     def extract_pair(self, x: bytes, y: bytes, params: MotexParams, u=None, v=None):

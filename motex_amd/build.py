@@ -25,7 +25,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 LIB_SOURCES = ["hd_kernels.hip", "ed_kernels.hip", "ed_slice_kernels.hip", "ed_slice_kernels_k15.hip",
                "ed_slice_kernels_k23.hip", "struct_kernels.hip", "runtime.hip",
-               "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip"]
+               "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip", "runtime_noalloc.hip"]
 CLI_SOURCES = ["cli/main.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/switches.cpp"]
 
 

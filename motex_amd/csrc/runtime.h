@@ -5,6 +5,7 @@
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -193,6 +194,13 @@ uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max);
 int batch_upload_common(motex_batch *b);
 int batch_clear(motex_batch *b);
 int hd_prepare(motex_batch *b);
+// Hamming work-queue plan of an all-pairs run (ranges, items grouped for about 16 items per
+// resident wave, longest first, LDS window words); want_sym decides the pair-symmetric plan
+// from the first ranges pass.
+int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const TileShape &ts,
+                 const std::function<bool(const HdPlan &)> &want_sym, HdPlan &plan, bool *sym_out);
+int hd_table_words(const SeqSet &ss, const HdPlan &plan, uint64_t *Wout);
+double hd_swept_cells(const HdPlan &plan);
 int hd_launch(motex_batch *b);
 int hd_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
 int ed_prepare(motex_batch *b);
@@ -207,5 +215,6 @@ int struct_pair_run(motex_batch *b, uint32_t i, uint32_t j);
 HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t persistent);
 hipError_t hd_alloc_hitq(DevBuf &buf, uint64_t grid, uint32_t blocks);
 void serving_shutdown(motex_ctx *c);
+int noalloc_init(int device);  // looks up the no-allocation scratch array's address on `device`
 
 }  // namespace motex

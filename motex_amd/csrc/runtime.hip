@@ -15,6 +15,7 @@
 #include <memory>
 #include <cmath>
 #include <cstdlib>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -298,11 +299,10 @@ static int64_t hd_wbase(const HdItem &it, const HdRange &R)
     return w + (it.px_start >> 5);
 }
 
-// Upload ranges + items; size the NE tables for every window the items read.
-static int hd_upload_plan(motex_batch *b, HdPlan &plan, DevBuf &d_items, DevBuf &d_ranges, uint64_t *Wout)
+// Words per NE table row that every window the plan's items read fits in.
+int hd_table_words(const SeqSet &ss, const HdPlan &plan, uint64_t *Wout)
 {
-    hipError_t e;
-    uint64_t W = (b->ss.total + kPadBits) / 32 + 2;
+    uint64_t W = (ss.total + kPadBits) / 32 + 2;
     for (const HdItem &it : plan.items)
         for (uint32_t r = it.rb; r < it.re; ++r) {
             int64_t wb = hd_wbase(it, plan.ranges[r]);
@@ -310,6 +310,14 @@ static int hd_upload_plan(motex_batch *b, HdPlan &plan, DevBuf &d_items, DevBuf 
             W = std::max<uint64_t>(W, (uint64_t)wb + plan.wl + 1);
         }
     *Wout = W;
+    return MOTEX_OK;
+}
+
+// Upload ranges + items; size the NE tables for every window the items read.
+static int hd_upload_plan(motex_batch *b, HdPlan &plan, DevBuf &d_items, DevBuf &d_ranges, uint64_t *Wout)
+{
+    hipError_t e;
+    if (int rc = hd_table_words(b->ss, plan, Wout)) return rc;
     if ((e = d_items.alloc(std::max<size_t>(plan.items.size(), 1) * sizeof(HdItem))) != hipSuccess)
         return hip_fail(e, "alloc items");
     if (!plan.items.empty() &&
@@ -355,6 +363,7 @@ extern "C" int motex_ctx_create(int device, size_t budget, motex_ctx **out)
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
     if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return hip_fail(e, "hipEventCreate");
     if ((e = hipEventCreate(&c->ev2)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if (int rc = noalloc_init(device)) return rc;
     *out = c.release();
     return MOTEX_OK;
 }
@@ -422,6 +431,56 @@ static bool hd_symmetric(motex_batch *b, const HdPlan &plan)
     return p < 1e-4;
 }
 
+int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const TileShape &ts,
+                 const std::function<bool(const HdPlan &)> &want_sym, HdPlan &plan, bool *sym_out)
+{
+    PhaseTrace tr("hd_plan");
+    std::vector<uint32_t> xs(ss.num);
+    for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
+    plan_hd_items(ss, p.k, ts.blocks, ts.rows, xs, nullptr, plan, 1u << 30);
+    tr.mark("ranges");
+    const bool sym = want_sym(plan);
+    tr.mark("symmetry estimate");
+    // group size: aim at ~16 items per resident wave so the tail of the queue is short
+    uint64_t pairs_units = 0;  // tiles x ranges
+    {
+        size_t r_first = 0;
+        for (uint32_t i = 0; i < ss.num; ++i) {
+            if (ss.len[i] < p.k) continue;
+            uint64_t ntiles = (ss.len[i] - p.k + 1 + ts.rows - 1) / ts.rows;
+            if (sym)
+                while (r_first < plan.ranges.size() &&
+                       plan.ranges[r_first].j0 + plan.ranges[r_first].ny - 1 < i)
+                    ++r_first;
+            pairs_units += ntiles * (plan.ranges.size() - r_first);
+        }
+    }
+    uint64_t target_items = (uint64_t)c->num_cu * ts.waves_per_cu * 16;
+    uint32_t gmax = (uint32_t)std::max<uint64_t>(1, pairs_units / std::max<uint64_t>(target_items, 1));
+    plan_hd_items(ss, p.k, ts.blocks, ts.rows, xs, nullptr, plan, gmax, sym);
+    tr.mark("items");
+    // longest items first: they set the tail of the persistent schedule
+    std::vector<uint64_t> span(plan.ranges.size() + 1, 0);
+    for (size_t r = 0; r < plan.ranges.size(); ++r) span[r + 1] = span[r] + (plan.ranges[r].ye - plan.ranges[r].ys);
+    {
+        std::vector<uint64_t> cost(plan.items.size());
+        for (size_t t = 0; t < cost.size(); ++t) {
+            const HdItem &x = plan.items[t];
+            cost[t] = (uint64_t)(x.r1 - x.px_start) * (span[x.re] - span[x.rb] + (x.re - x.rb) * (x.r1 - x.r0));
+        }
+        const std::vector<uint32_t> order = order_by_cost_desc(cost);
+        std::vector<HdItem> sorted(order.size());
+        for (size_t t = 0; t < order.size(); ++t) sorted[t] = plan.items[order[t]];
+        plan.items.swap(sorted);
+    }
+    plan.wl = hd_lds_words(ts.blocks, plan.rows_max);
+    if (hd_lds_bytes(ss.sigma, plan.wl, plan.rows_max) > 65536 || (uint64_t)ss.sigma * plan.wl >= 65536)
+        return set_error(MOTEX_EUNSUPPORTED, "alphabet of %u symbols does not fit the kernel's LDS window", ss.sigma);
+    tr.mark("sort");
+    *sym_out = sym;
+    return MOTEX_OK;
+}
+
 int hd_prepare(motex_batch *b)
 {
     motex_ctx *c = b->ctx;
@@ -429,50 +488,13 @@ int hd_prepare(motex_batch *b)
     PhaseTrace tr("hd_prepare");
     TileShape ts = choose_tile(c, b->params, ss);
     b->tile = ts;
-    std::vector<uint32_t> xs(ss.num);
-    for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
-    // group size: aim at ~16 items per resident wave so the tail of the queue is short
-    plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, 1u << 30);
-    tr.mark("ranges");
-    const bool sym = hd_symmetric(b, b->hd);
-    tr.mark("symmetry estimate");
-    uint64_t pairs_units = 0;  // tiles x ranges
-    {
-        size_t r_first = 0;
-        for (uint32_t i = 0; i < ss.num; ++i) {
-            if (ss.len[i] < b->params.k) continue;
-            uint64_t ntiles = (ss.len[i] - b->params.k + 1 + ts.rows - 1) / ts.rows;
-            if (sym)
-                while (r_first < b->hd.ranges.size() &&
-                       b->hd.ranges[r_first].j0 + b->hd.ranges[r_first].ny - 1 < i)
-                    ++r_first;
-            pairs_units += ntiles * (b->hd.ranges.size() - r_first);
-        }
-    }
-    uint64_t target_items = (uint64_t)c->num_cu * ts.waves_per_cu * 16;
-    uint32_t gmax = (uint32_t)std::max<uint64_t>(1, pairs_units / std::max<uint64_t>(target_items, 1));
-    plan_hd_items(ss, b->params.k, ts.blocks, ts.rows, xs, nullptr, b->hd, gmax, sym);
-    tr.mark("items");
-    // longest items first: they set the tail of the persistent schedule
-    std::vector<uint64_t> span(b->hd.ranges.size() + 1, 0);
-    for (size_t r = 0; r < b->hd.ranges.size(); ++r) span[r + 1] = span[r] + (b->hd.ranges[r].ye - b->hd.ranges[r].ys);
-    {
-        std::vector<uint64_t> cost(b->hd.items.size());
-        for (size_t t = 0; t < cost.size(); ++t) {
-            const HdItem &x = b->hd.items[t];
-            cost[t] = (uint64_t)(x.r1 - x.px_start) * (span[x.re] - span[x.rb] + (x.re - x.rb) * (x.r1 - x.r0));
-        }
-        const std::vector<uint32_t> order = order_by_cost_desc(cost);
-        std::vector<HdItem> sorted(order.size());
-        for (size_t t = 0; t < order.size(); ++t) sorted[t] = b->hd.items[order[t]];
-        b->hd.items.swap(sorted);
-    }
-    b->hd.wl = hd_lds_words(ts.blocks, b->hd.rows_max);
-    if (hd_lds_bytes(ss.sigma, b->hd.wl, b->hd.rows_max) > 65536 || (uint64_t)ss.sigma * b->hd.wl >= 65536)
-        return set_error(MOTEX_EUNSUPPORTED, "alphabet of %u symbols does not fit the kernel's LDS window", ss.sigma);
-    tr.mark("sort");
+    bool sym = false;
+    int rc = hd_make_plan(c, ss, b->params, ts, [b](const HdPlan &pl) { return hd_symmetric(b, pl); },
+                          b->hd, &sym);
+    if (rc) return rc;
+    tr.mark("plan");
     uint64_t W = 0;
-    int rc = hd_upload_plan(b, b->hd, b->d_items, b->d_ranges, &W);
+    rc = hd_upload_plan(b, b->hd, b->d_items, b->d_ranges, &W);
     if (rc) return rc;
     tr.mark("upload plan");
     // the plain form's windows (one block, short row tiles) must fit as well
@@ -489,9 +511,7 @@ int hd_prepare(motex_batch *b)
     if ((e = hd_alloc_hitq(b->d_hitq, grid, ts.blocks)) != hipSuccess) return hip_fail(e, "alloc hit queues");
     tr.mark("hit queues");
     b->stats.symmetric = sym ? 1u : 0u;
-    b->stats.swept_cells = 0;
-    for (const HdItem &it : b->hd.items)
-        b->stats.swept_cells += (double)(it.r1 - it.px_start) * (it.re - it.rb) * kDiagPerBlock * ts.blocks;
+    b->stats.swept_cells = hd_swept_cells(b->hd);
     if (sym) {
         b->f_words = (uint32_t)((ss.total + 31) / 32 + 1);
         if ((e = b->d_fbits.alloc((uint64_t)ss.num * b->f_words * 4)) != hipSuccess) return hip_fail(e, "alloc text bits");
@@ -503,6 +523,14 @@ int hd_prepare(motex_batch *b)
     if ((e = hipStreamSynchronize(bstream(b))) != hipSuccess) return hip_fail(e, "NE build sync");
     tr.mark("sync");
     return MOTEX_OK;
+}
+
+double hd_swept_cells(const HdPlan &plan)
+{
+    double s = 0;
+    for (const HdItem &it : plan.items)
+        s += (double)(it.r1 - it.px_start) * (it.re - it.rb) * kDiagPerBlock * plan.blocks;
+    return s;
 }
 
 hipError_t hd_alloc_hitq(DevBuf &buf, uint64_t grid, uint32_t blocks)
