@@ -25,8 +25,16 @@ sys.path.insert(0, ROOT)
 
 # MI355X published peaks
 HBM_PEAK_GBS = 8000.0
-VALU_INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # 39.3 T lane-ops/s
-OPS_PER_CELL = {0: 8.0, 1: 22.0}                  # algorithmic int32 ops per cell (HD, ED)
+# The published FP32 vector rate, 157.3 TFLOP/s, read as 32-bit lane operations (an FMA counts
+# two flops): 78.6 T lane-ops/s = 256 CUs x 128 lanes x 2.4 GHz.  The kernels' issued VALU
+# lane-ops (SQ_INSTS_VALU x 64) are measured against it.
+VALU_LANE_OPS_PEAK_TOPS = 157.3 / 2
+# Algorithmic view: the reference recurrence's int32 ops per cell (Hamming 8, edit 22) against
+# one 32-bit op per lane per clock (256 x 64 x 2.4 GHz = 39.3 T).  The kernels are bit-sliced
+# (32 cells per lane op) and, for Hamming, compute each unordered pair once, so this figure
+# exceeds 1; it is reported beside the roofline, never as its fraction.
+VALU_INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+OPS_PER_CELL = {0: 8.0, 1: 22.0}
 # Issue-rate ceilings measured on MI355X with a dependency-free micro-benchmark of one
 # instruction kind at 32 waves per CU (T lane-ops/s); see profiles/valu_issue_rate.md.
 VALU_CEILINGS_TOPS = {"v_xor_b32": 61.9, "v_bitop3_b32": 59.3, "v_alignbit_b32": 37.9}
@@ -38,11 +46,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="dna1200", choices=["dna50", "dna1200", "prot500", "struct1000", "long4m"],
+    ap.add_argument("--workload", default="dna1200",
+                    choices=["dna50", "dna1200", "prot500", "struct1000", "long4m", "hsapiens1200"],
                     help="dna50: 50 x 500 nt Hamming k=10 e=1; dna1200: 1200 x 1000 nt Hamming k=12 "
                          "e=2 (the metric's workload); prot500: 500 proteins of U[200,2000], edit k=8 "
                          "e=1; struct1000: 1000 x 600 nt, structured 8+[10,30]+8; long4m: one 4 Mnt "
-                         "sequence, Hamming k=10 e=2")
+                         "sequence, Hamming k=10 e=2; hsapiens1200: the reference's 1200 H. sapiens "
+                         "genes x 1000 nt, Hamming k=12 e=2")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--plain-x", type=int, default=4,
                     help="x sequences timed in the plain one-launch-per-pair form")
@@ -90,6 +100,37 @@ def traffic_from_profile(kernel: str, workload_tag: str):
             "source": os.path.relpath(path, ROOT)}
 
 
+def valu_from_profile(kernel: str, workload_tag: str):
+    """VALU wave-instructions of one dispatch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc summary (the SQ_INSTS_VALU row), or None."""
+    path = os.path.join(PROFILES, f"{kernel}_{workload_tag}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 2 and cells[0] == "SQ_INSTS_VALU":
+            try:
+                return {"valu_insts": int(cells[1].replace(",", "")), "source": os.path.relpath(path, ROOT)}
+            except ValueError:
+                return None
+    return None
+
+
+def calibration(distance: int, threads: int):
+    """The oracle's time over motexOMP's on the same input and host (oracle/calibrate.py),
+    for this distance at the nearest thread count, or None."""
+    path = os.path.join(ROOT, "oracle", "calibration_hsapiens120.json")
+    if not os.path.exists(path):
+        return None
+    cal = json.load(open(path))
+    runs = [r for r in cal["runs"] if r["distance"] == distance]
+    if not runs:
+        return None
+    r = min(runs, key=lambda r: abs(r["threads"] - threads))
+    return {"ratio_oracle_over_motexomp": r["ratio_oracle_over_motexomp"], "threads": r["threads"],
+            "input": cal["input"], "host": cal["host"], "source": "oracle/calibration_hsapiens120.json"}
+
+
 def cpu_baseline(w, gpu_u, gpu_v, off, nx):
     """Times the oracle (OpenMP over x, host cores) on x sequences [0, nx) against every
     y and checks the GPU rows of the same sample bit for bit."""
@@ -104,10 +145,21 @@ def cpu_baseline(w, gpu_u, gpu_v, off, nx):
     parity = bool(np.array_equal(ou[:end], gpu_u[:end]) and np.array_equal(ov[:end], gpu_v[:end]))
     L = np.array(w.lens, dtype=np.int64)
     cells = float(np.sum(L[:nx] - w.k + 1) * np.sum(L))
-    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": cores, "kind": "port",
-            "sample": f"x sequences 0..{nx - 1} against all {len(w.seqs)} texts "
-                      f"({cells:.3e} cells, {dt:.1f} s, OpenMP over x)",
-            "parity_on_sample": parity}
+    out = {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": cores, "kind": "port",
+           "sample": f"x sequences 0..{nx - 1} against all {len(w.seqs)} texts "
+                     f"({cells:.3e} cells, {dt:.1f} s, OpenMP over x)",
+           "parity_on_sample": parity}
+    return with_calibration(out, w.distance if not w.boxes else None, cores)
+
+
+def with_calibration(out, distance, cores):
+    """Adds the motexOMP-equivalent rate: the port's rate times its time ratio to motexOMP on
+    the reference's own timed input (structured runs have no recorded motexOMP time)."""
+    cal = calibration(distance, cores) if distance is not None else None
+    if cal:
+        out["calibration"] = cal
+        out["motexomp_equivalent_value"] = round(out["value"] * cal["ratio_oracle_over_motexomp"], 4)
+    return out
 
 
 def cpu_baseline_prefix(ctx, w, params, seconds=12.0):
@@ -122,10 +174,11 @@ def cpu_baseline_prefix(ctx, w, params, seconds=12.0):
     dt = time.perf_counter() - t0
     gu, gv, _ = ctx.extract_all_pairs(pre, params)
     cells = float((P - w.k + 1) * P)
-    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
-            "sample": f"self pair of the first {P} symbols ({cells:.3e} cells, {dt:.1f} s; one "
-                      "thread, as the reference gives a single sequence one thread)",
-            "parity_on_sample": bool(np.array_equal(ou, gu) and np.array_equal(ov, gv))}
+    out = {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
+           "sample": f"self pair of the first {P} symbols ({cells:.3e} cells, {dt:.1f} s; one "
+                     "thread, as the reference gives a single sequence one thread)",
+           "parity_on_sample": bool(np.array_equal(ou, gu) and np.array_equal(ov, gv))}
+    return with_calibration(out, w.distance, 1)
 
 
 def main():
@@ -141,7 +194,11 @@ def main():
     import motex_amd
     from motex_amd import MotexParams, synth
 
-    w = synth.WORKLOADS[a.workload](a.scale)
+    if a.workload == "hsapiens1200":
+        from motex_amd import refdata
+        w = refdata.hsapiens(max(2, int(round(1200 * a.scale))))
+    else:
+        w = synth.WORKLOADS[a.workload](a.scale)
     params = MotexParams(k=w.k, e=w.e, distance=w.distance, alphabet=w.alphabet,
                          boxes=list(w.boxes), long_mode=w.long_mode)
     ctx = motex_amd.Context(0)
@@ -182,28 +239,38 @@ def main():
 
     kms = float(np.mean(kernel_ms))
     ops = OPS_PER_CELL[w.distance]
-    achieved = cells * ops / (kms / 1e3) / 1e12
+    algo = cells * ops / (kms / 1e3) / 1e12
     kernel = ("struct_kernel" if w.boxes else
               "hd_batched_kernel" if w.distance == 0 else
               {2: "ed_slice_kernel", 3: "ed_slice_pipe_kernel"}.get(st["kernel_variant"], "ed_kernel"))
     tr = traffic_from_profile(kernel, a.workload) if a.scale == 1.0 else None
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_INT32_PEAK_TOPS,
-                "unit": "TOP/s", "frac": round(achieved / VALU_INT32_PEAK_TOPS, 4),
+    vp = valu_from_profile(kernel, a.workload) if a.scale == 1.0 else None
+    # issued VALU lane-ops of one dispatch (rocprof SQ_INSTS_VALU x 64) over the live kernel time
+    achieved = vp["valu_insts"] * 64 / (kms / 1e3) / 1e12 if vp else None
+    roofline = {"bound": "valu", "achieved": round(achieved, 3) if vp else None,
+                "peak": VALU_LANE_OPS_PEAK_TOPS, "unit": "T lane-ops/s",
+                "frac": round(achieved / VALU_LANE_OPS_PEAK_TOPS, 4) if vp else None,
                 "traffic": tr["bytes"] if tr else None,
                 "traffic_detail": tr,
+                "valu_insts_per_dispatch": vp,
                 "kernel": kernel,
                 "kernel_ms": round(kms, 4),
-                "ops_per_cell": ops,
                 "cells_per_s": round(cells / (kms / 1e3), 1),
                 "swept_cells_per_s": round(st["swept_cells"] / (kms / 1e3), 1),
                 "symmetric": bool(st["symmetric"]),
                 "graph_replay": bool(st["graph_replay"]),
                 "valu_ceilings_measured": VALU_CEILINGS_TOPS,
-                "note": "achieved = algorithmic int32 ops (the reference's per-cell MaxShift "
-                        "count x every ordered pair's cells) / kernel time (HIP events).  The "
-                        "kernel is bit-sliced (32 diagonals per lane op) and, for Hamming, "
-                        "computes each unordered pair once, so frac exceeds 1; the issued VALU "
-                        "rate against the measured ceilings is in profiles/"}
+                "note": "achieved = issued VALU lane-ops of one dispatch (SQ_INSTS_VALU x 64, "
+                        "from the committed rocprofv3 --pmc summary of this workload) / the "
+                        "kernel time measured here with HIP events; peak = the published 157.3 "
+                        "TFLOP/s FP32 vector rate read as 78.6 T lane-ops/s",
+                "algorithmic": {"ops_per_cell": ops, "ops_per_s_T": round(algo, 3),
+                                "int32_peak_T": VALU_INT32_PEAK_TOPS,
+                                "ratio_to_int32_peak": round(algo / VALU_INT32_PEAK_TOPS, 4),
+                                "note": "the reference recurrence's ops per cell over every "
+                                        "ordered pair's cells; above 1 because one lane op "
+                                        "advances 32 cells and Hamming pairs are computed once "
+                                        "per unordered pair"}}
 
     out = {
         "metric": "GCUPS batched vs plain-per-pair on 1 GPU at k=12,e=2; p50/p99 job latency",
@@ -217,13 +284,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic (seeded SplitMix64 generator, motex_amd/synth.py)",
+        "data": ("the reference's data/hsapiens.1200_genes.fa (tests/golden/refdata)" if a.workload == "hsapiens1200"
+                 else "synthetic (seeded SplitMix64 generator, motex_amd/synth.py)"),
         "config": {"workload": w.name, "num_seqs": len(w.seqs), "total_nt": int(L.sum()),
                    "cells": cells, "k": w.k, "e": w.e, "distance": w.distance,
                    "tile": {"blocks": st["tile_blocks"], "rows": st["tile_rows"],
                             "waves_per_cu": st["waves_per_cu"]},
                    "work_items": st["work_items"], "upload_s": round(t_up, 3)},
         "roofline": roofline,
+        "edit_semantics": ("MaxShift recurrence of the reference's in-tree code (functions.cc:437-447, "
+                           "bitminmax 923-967); equality with libflasm's flasm_ed is unpinned"
+                           if w.distance == 1 else None),
         "plain": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain and plain["value"] else None,
     }

@@ -33,3 +33,19 @@ def test_workload_choices_are_the_generators():
     assert r.returncode == 0
     for name in synth.WORKLOADS:
         assert name in r.stdout
+
+
+def test_roofline_fraction_from_issued_valu_is_a_fraction():
+    vp = bench.valu_from_profile("hd_batched_kernel", "dna1200")
+    assert vp is not None and vp["valu_insts"] > 0
+    # the committed trace's kernel time: issued lane-ops over it must stay under the peak
+    frac = vp["valu_insts"] * 64 / 8.5e-3 / 1e12 / bench.VALU_LANE_OPS_PEAK_TOPS
+    assert 0 < frac <= 1
+    assert bench.valu_from_profile("hd_batched_kernel", "no_such_workload") is None
+
+
+def test_calibration_against_motexomp_is_recorded():
+    for d in (0, 1):
+        cal = bench.calibration(d, 16)
+        assert cal is not None and cal["ratio_oracle_over_motexomp"] > 0
+    assert bench.calibration(0, 1)["threads"] == 1
