@@ -144,6 +144,16 @@ int motex_extract_all_pairs_noalloc(motex_ctx *ctx, const char *const *seqs, con
                                     uint32_t num_seqs, const motex_params *params,
                                     uint32_t *const *u, uint32_t *const *v);
 
+/* Cross form, for re-searching motifs in a set of sequences (-I, motex.cc:883-1137): every
+ * x sequence (a motif string read back from an earlier run's unmatched-motif file) against
+ * every text y, with the per-pair operator of the loop at motex.cc:940-990
+ * (motifs_extraction_{hd,ed}, x = fseqs[i], y = seqs[j]).  u[i], v[i] hold xlens[i] words,
+ * accumulated; a text shorter than k is skipped as motex.cc:948-952 does.  Single motifs only
+ * (the reference refuses -I with -s, motex.cc:193-199). */
+int motex_extract_cross(motex_ctx *ctx, const char *const *xs, const uint32_t *xlens, uint32_t num_x,
+                        const char *const *ys, const uint32_t *ylens, uint32_t num_y,
+                        const motex_params *params, uint32_t *const *u, uint32_t *const *v);
+
 /* --- device-resident batches (the timed form: inputs already in HBM) --------------- */
 
 /* Uploads and packs the sequences, builds the work queue and picks tile shapes. */

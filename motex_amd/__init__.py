@@ -63,6 +63,9 @@ SIGNATURES = [
                                           C.POINTER(Params), C.POINTER(u32p), C.POINTER(u32p)]),
     ("motex_extract_pair", C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32,
                                      C.POINTER(Params), u32p, u32p]),
+    ("motex_extract_cross", C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), u32p, C.c_uint32,
+                                      C.POINTER(C.c_char_p), u32p, C.c_uint32, C.POINTER(Params),
+                                      C.POINTER(u32p), C.POINTER(u32p)]),
     ("motex_extract_all_pairs_noalloc", C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), u32p, C.c_uint32,
                                                   C.POINTER(Params), C.POINTER(u32p), C.POINTER(u32p)]),
     ("motex_batch_create", C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), u32p, C.c_uint32,
@@ -181,6 +184,20 @@ class Context:
         p, keep = params.to_c()
         _check(lib().motex_extract_all_pairs(self.h, sa, lens.ctypes.data_as(u32p), len(seqs),
                                              C.byref(p), _row_ptrs(u, off), _row_ptrs(v, off)))
+        return u[: int(off[-1])], v[: int(off[-1])], off
+
+    def extract_cross(self, xs: list[bytes], ys: list[bytes], params: MotexParams):
+        """Every x against every text y only (the -I re-search).  Returns flat (u, v, offsets)
+        over the x sequences."""
+        off = uv_layout(xs, params.nb_structs)
+        u = np.zeros(max(int(off[-1]), 1), np.uint32)
+        v = np.zeros(max(int(off[-1]), 1), np.uint32)
+        xa, xl = _seq_args(xs)
+        ya, yl = _seq_args(ys)
+        p, keep = params.to_c()
+        _check(lib().motex_extract_cross(self.h, xa, xl.ctypes.data_as(u32p), len(xs), ya,
+                                         yl.ctypes.data_as(u32p), len(ys), C.byref(p),
+                                         _row_ptrs(u, off), _row_ptrs(v, off)))
         return u[: int(off[-1])], v[: int(off[-1])], off
 
     def extract_all_pairs_noalloc(self, seqs: list[bytes], params: MotexParams):

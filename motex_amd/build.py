@@ -26,7 +26,23 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 LIB_SOURCES = ["hd_kernels.hip", "ed_kernels.hip", "ed_slice_kernels.hip", "ed_slice_kernels_k15.hip",
                "ed_slice_kernels_k23.hip", "struct_kernels.hip", "runtime.hip",
                "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip", "runtime_noalloc.hip"]
-CLI_SOURCES = ["cli/main.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/switches.cpp"]
+CLI_SOURCES = ["cli/main.cpp", "cli/app.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/writer_back.cpp", "cli/stats.cpp",
+               "cli/switches.cpp"]
+# MPFR (with GMP) for the command line's statistics modes (-b/-u/-U, -I), whose p-values the
+# reference computes with 200-bit MPFR arithmetic: the system's runtime libraries (libmpfr6,
+# libgmp10, which the compiler itself depends on) with the API headers of the conda
+# installation.  Without them those modes are refused.
+MPFR_INCLUDE = os.environ.get("MOTEX_MPFR_INCLUDE", "/opt/conda/include")
+MPFR_LIBS = ["libmpfr.so.6", "libgmp.so.10"]
+SYSTEM_LIB = "/usr/lib/x86_64-linux-gnu"
+
+
+def mpfr_flags() -> tuple[list[str], list[str]]:
+    """(compile flags, link flags) for MPFR, or empty lists when it is not installed."""
+    if not (os.path.exists(os.path.join(MPFR_INCLUDE, "mpfr.h")) and os.path.exists(os.path.join(MPFR_INCLUDE, "gmp.h"))
+            and all(os.path.exists(os.path.join(SYSTEM_LIB, l)) for l in MPFR_LIBS)):
+        return [], []
+    return ["-DMOTEX_HAVE_MPFR", f"-I{MPFR_INCLUDE}"], [f"-l:{l}" for l in MPFR_LIBS]
 
 
 def _run(cmd):
@@ -49,7 +65,7 @@ def _compile_lib(src):
 def _compile_cli(src):
     out = _obj(src)
     _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
-          f"-I{CSRC}", "-c", os.path.join(CSRC, src), "-o", out])
+          f"-I{CSRC}", *mpfr_flags()[0], "-c", os.path.join(CSRC, src), "-o", out])
     return out
 
 
@@ -68,7 +84,7 @@ def build(verbose: bool = False) -> tuple[str, str]:
     os.replace(tmp, LIB)
     if cli_objs:
         _run(["g++", "-o", BIN, *cli_objs, f"-L{os.path.dirname(LIB)}", "-lmotex",
-              "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"])
+              "-Wl,-rpath,$ORIGIN/../lib", *mpfr_flags()[1], "-lpthread"])
     if verbose:
         print("built", LIB, BIN if cli_objs else "")
     return LIB, BIN

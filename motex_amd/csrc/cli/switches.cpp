@@ -108,8 +108,12 @@ void usage()
             "                                      runs on one GPU (default: 4).\n"
             "  -L, --long-sequences      <int>     1 for a few very long sequences; a scheduling\n"
             "                                      hint that never changes results (default: 0).\n"
-            "  -b, -u, -I, -U                      Background statistics and re-search modes:\n"
-            "                                      validated, not implemented by this tool.\n");
+            "  -b, --background-in-file  <str>     A .motex output of an earlier run, used as the\n"
+            "                                      background: each motif gets a binomial p-value.\n"
+            "  -u, --unmatched-out-file  <str>     Output file of the motifs not found in the -b file.\n"
+            "  -U, --unmatched-SMILE-out-file <str> SMILE-like form of the -u file (needs -S).\n"
+            "  -I, --unmatched-in-file   <str>     A -u file to search again in this input, with\n"
+            "                                      foreground p-values.\n");
 }
 
 }  // namespace motex_cli

@@ -20,14 +20,14 @@ static const char *kDNA = "ACGTN";
 static const char *kPROT = "ARNDCQEGHILKMFPSTWYVOUBZJX*";
 static const char *kUSR = "0123456789";
 
-static const char *alphabet_letters(const std::string &name)
+const char *alphabet_letters(const std::string &name)
 {
     if (name == "DNA") return kDNA;
     if (name == "PROT") return kPROT;
     return kUSR;
 }
 
-static void write_header(FILE *f, const Switches &sw, unsigned int num_seqs, double exectime, int P)
+void write_header(FILE *f, const Switches &sw, unsigned int num_seqs, double exectime, int P)
 {
     time_t t;
     time(&t);
@@ -60,7 +60,7 @@ static void write_header(FILE *f, const Switches &sw, unsigned int num_seqs, dou
     fprintf(f, "####################################\n\n");
 }
 
-static bool passes(const Switches &sw, unsigned int num_seqs, uint32_t u, uint32_t v)
+bool passes(const Switches &sw, unsigned int num_seqs, uint32_t u, uint32_t v)
 {
     double r = ((double)u / num_seqs) * 100;
     return r >= sw.q && v >= sw.n && r <= sw.Q && v <= sw.N;
@@ -68,7 +68,7 @@ static bool passes(const Switches &sw, unsigned int num_seqs, uint32_t u, uint32
 
 // The printed motif of entry (j, s) of sequence seq: the k-mer ending at j, or for
 // structured motifs every box joined by '_' (functions.cc:1191-1231).
-static std::string motif_text(const Switches &sw, const std::string &seq, unsigned int j, unsigned int s)
+std::string motif_text(const Switches &sw, const std::string &seq, unsigned int j, unsigned int s)
 {
     unsigned int jj = j - sw.l + 1;
     std::string out = seq.substr(jj, sw.l);
@@ -122,7 +122,7 @@ int write_motifs(const Switches &sw, const std::vector<std::string> &seqs,
     return 1;
 }
 
-static void write_smile_header(FILE *f, const Switches &sw, unsigned int num_seqs, const char *astr)
+void write_smile_header(FILE *f, const Switches &sw, unsigned int num_seqs, const char *astr)
 {
     std::string line1;
     char buf[512];
@@ -162,6 +162,21 @@ static void write_smile_header(FILE *f, const Switches &sw, unsigned int num_seq
     fputc('\n', f);
 }
 
+// SMILE digit id of a motif: each letter's index in the alphabet string ('0' + index; the
+// index one past the end for a letter outside it), and '-' for the '_' box separators of
+// structured motifs (functions.cc:1424-1432, 1564-1577).
+std::string smile_id(const Switches &sw, const std::string &mot, const char *astr)
+{
+    const size_t alen = strlen(astr);
+    std::string id(mot.size(), '0');
+    for (size_t c = 0; c < mot.size(); ++c) {
+        size_t a = 0;
+        while (a < alen && mot[c] != astr[a]) ++a;
+        id[c] = (sw.nb_boxes > 1 && a >= alen) ? '-' : (char)('0' + a);
+    }
+    return id;
+}
+
 int write_motifs_smile(const Switches &sw, const std::vector<std::string> &seqs,
                        const std::vector<std::vector<uint32_t>> &u,
                        const std::vector<std::vector<uint32_t>> &v, double exectime)
@@ -175,7 +190,6 @@ int write_motifs_smile(const Switches &sw, const std::vector<std::string> &seqs,
     unsigned int num = (unsigned int)seqs.size(), valid = 0;
     write_smile_header(f, sw, num, astr);
     std::unordered_set<std::string> printed;
-    size_t alen = strlen(astr);
     for (unsigned int i = 0; i < num; ++i) {
         unsigned int m = (unsigned int)seqs[i].size();
         for (unsigned int j = sw.l - 1; j < m; ++j) {
@@ -184,12 +198,7 @@ int write_motifs_smile(const Switches &sw, const std::vector<std::string> &seqs,
                 if (!passes(sw, num, u[i][o], v[i][o])) continue;
                 std::string mot = motif_text(sw, seqs[i], j, s);
                 if (!printed.insert(mot).second) continue;
-                std::string id(mot.size(), '0');
-                for (size_t c = 0; c < mot.size(); ++c) {
-                    size_t a = 0;
-                    while (a < alen && mot[c] != astr[a]) ++a;
-                    id[c] = (sw.nb_boxes > 1 && a >= alen) ? '-' : (char)('0' + a);
-                }
+                const std::string id = smile_id(sw, mot, astr);
                 fprintf(f, "%s %s %d\t%d\n", mot.c_str(), id.c_str(), (int)u[i][o], (int)v[i][o]);
                 valid++;
             }

@@ -84,6 +84,10 @@ struct SeqSet {
     std::vector<uint32_t> len, start;   // start has num+1 entries
     std::vector<uint8_t> codes;
     std::vector<uint64_t> uv_off;       // num+1 entries, nb_structs * len prefix sums
+    // Sequences [0, nx) are the x side (their factors are searched) and [y0, num) the texts.
+    // All-pairs runs: nx = num, y0 = 0.  Cross runs (x = motifs, y = input sequences):
+    // nx = y0 = the number of x sequences.
+    uint32_t nx = 0, y0 = 0;
     int build(const char *const *seqs, const uint32_t *lens, uint32_t n, uint32_t nb_structs);
 };
 
@@ -164,8 +168,10 @@ struct motex_batch {
 namespace motex {
 
 inline hipStream_t bstream(const motex_batch *b) { return b->stream ? b->stream : b->ctx->stream; }
+// num_x < num: a cross run, sequences [0, num_x) against the texts [num_x, num) only.
 int batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num,
-                 const motex_params *p, hipStream_t stream, motex_batch **out, uint32_t grid_share = 1);
+                 const motex_params *p, hipStream_t stream, motex_batch **out, uint32_t grid_share = 1,
+                 uint32_t num_x = ~0u);
 int batch_launch(motex_batch *b);   // enqueue clear + batched kernel, no wait
 int batch_launch_graph(motex_batch *b);  // same, replayed from a HIP graph after the first run
 hipError_t batch_realloc_scratch(motex_batch *b, uint64_t bytes);  // grows d_aux, drops the graph

@@ -150,6 +150,8 @@ int SeqSet::build(const char *const *seqs, const uint32_t *lens, uint32_t n, uin
                          (unsigned long long)tot);
     start[n] = (uint32_t)tot;
     total = tot;
+    nx = n;
+    y0 = 0;
     int map[256];
     for (int c = 0; c < 256; ++c) map[c] = -1;
     sigma = 0;
@@ -232,7 +234,7 @@ void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
     if (only_y) {
         for (int32_t j : *only_y) add_seq_ranges((uint32_t)j);
     } else {
-        uint32_t j = 0;
+        uint32_t j = ss.y0;
         while (j < ss.num) {
             if (ss.len[j] < k) { ++j; continue; }
             if ((int64_t)ss.len[j] > cap) { add_seq_ranges(j); ++j; continue; }
@@ -413,7 +415,7 @@ int batch_upload_common(motex_batch *b)
 static bool hd_symmetric(motex_batch *b, const HdPlan &plan)
 {
     const SeqSet &ss = b->ss;
-    if (plan.split_slots || b->nb_structs != 1 || ss.num < 2 || ss.num > 65535) return false;
+    if (plan.split_slots || b->nb_structs != 1 || ss.num < 2 || ss.num > 65535 || ss.nx != ss.num) return false;
     const uint64_t f_words = (ss.total + 31) / 32 + 1;
     if ((uint64_t)ss.num * f_words * 4 > (4ull << 30)) return false;
     if (const char *env = getenv("MOTEX_SYM")) return env[0] == '1';
@@ -435,8 +437,8 @@ int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const Ti
                  const std::function<bool(const HdPlan &)> &want_sym, HdPlan &plan, bool *sym_out)
 {
     PhaseTrace tr("hd_plan");
-    std::vector<uint32_t> xs(ss.num);
-    for (uint32_t i = 0; i < ss.num; ++i) xs[i] = i;
+    std::vector<uint32_t> xs(ss.nx);
+    for (uint32_t i = 0; i < ss.nx; ++i) xs[i] = i;
     plan_hd_items(ss, p.k, ts.blocks, ts.rows, xs, nullptr, plan, 1u << 30);
     tr.mark("ranges");
     const bool sym = want_sym(plan);
@@ -445,7 +447,7 @@ int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const Ti
     uint64_t pairs_units = 0;  // tiles x ranges
     {
         size_t r_first = 0;
-        for (uint32_t i = 0; i < ss.num; ++i) {
+        for (uint32_t i = 0; i < ss.nx; ++i) {
             if (ss.len[i] < p.k) continue;
             uint64_t ntiles = (ss.len[i] - p.k + 1 + ts.rows - 1) / ts.rows;
             if (sym)
@@ -769,9 +771,9 @@ int hd_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x)
 {
     const SeqSet &ss = b->ss;
     double total = 0;
-    for (uint32_t i = first_x; i < first_x + num_x && i < ss.num; ++i) {
+    for (uint32_t i = first_x; i < first_x + num_x && i < ss.nx; ++i) {
         std::vector<std::pair<uint32_t, uint32_t>> pairs;
-        for (uint32_t j = 0; j < ss.num; ++j) pairs.push_back({i, j});
+        for (uint32_t j = ss.y0; j < ss.num; ++j) pairs.push_back({i, j});
         double ms = 0;
         int rc = hd_launch_pairs(b, pairs, &ms);
         if (rc) return rc;
@@ -800,11 +802,13 @@ double pair_cells(const SeqSet &ss, const motex_params &p, uint64_t *pairs, uint
 {
     uint32_t tot = total_motif_length(&p);
     double sm = 0, sn = 0;
-    uint64_t ok = 0;
-    for (uint32_t i = 0; i < ss.num; ++i)
-        if (ss.len[i] >= tot) { sm += ss.len[i] - p.k + 1; sn += ss.len[i]; ++ok; }
-    *pairs = ok * ok;
-    *skipped = (uint64_t)ss.num * ss.num - ok * ok;
+    uint64_t okx = 0, oky = 0;
+    for (uint32_t i = 0; i < ss.nx; ++i)
+        if (ss.len[i] >= tot) { sm += ss.len[i] - p.k + 1; ++okx; }
+    for (uint32_t j = ss.y0; j < ss.num; ++j)
+        if (ss.len[j] >= tot) { sn += ss.len[j]; ++oky; }
+    *pairs = okx * oky;
+    *skipped = (uint64_t)ss.nx * (ss.num - ss.y0) - okx * oky;
     return sm * sn;
 }
 
@@ -817,7 +821,8 @@ extern "C" int motex_batch_create(motex_ctx *c, const char *const *seqs, const u
 }
 
 int motex::batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *lens, uint32_t num,
-                        const motex_params *p, hipStream_t stream, motex_batch **out, uint32_t grid_share)
+                        const motex_params *p, hipStream_t stream, motex_batch **out, uint32_t grid_share,
+                        uint32_t num_x)
 {
     if (!c || !out) return set_error(MOTEX_EINVAL, "ctx/out is NULL");
     *out = nullptr;
@@ -846,6 +851,10 @@ int motex::batch_create(motex_ctx *c, const char *const *seqs, const uint32_t *l
     PhaseTrace tr("batch_create");
     b->S = enumerate_structs(&b->params, &b->nb_structs);
     if ((rc = b->ss.build(seqs, lens, num, b->nb_structs))) return rc;
+    if (num_x < num) {
+        if (p->nb_gaps) return set_error(MOTEX_EUNSUPPORTED, "cross runs take single motifs only");
+        b->ss.nx = b->ss.y0 = num_x;
+    }
     b->stats.cells = pair_cells(b->ss, b->params, &b->stats.pairs, &b->stats.skipped_pairs);
     tr.mark("pack");
     if ((rc = batch_upload_common(b.get()))) return rc;
@@ -940,6 +949,37 @@ extern "C" int motex_extract_all_pairs(motex_ctx *c, const char *const *seqs, co
     if (!rc) rc = motex_batch_download(b, u, v);
     motex_batch_destroy(b);
     return rc;
+}
+
+extern "C" int motex_extract_cross(motex_ctx *c, const char *const *xs, const uint32_t *xlens, uint32_t num_x,
+                                   const char *const *ys, const uint32_t *ylens, uint32_t num_y,
+                                   const motex_params *p, uint32_t *const *u, uint32_t *const *v)
+{
+    if (!c) return set_error(MOTEX_EINVAL, "ctx is NULL");
+    int rc = validate_params(p);
+    if (rc) return rc;
+    if (p->nb_gaps) return set_error(MOTEX_EINVAL, "cross runs take single motifs only (no structure)");
+    if ((num_x && (!xs || !xlens || !u || !v)) || (num_y && (!ys || !ylens)))
+        return set_error(MOTEX_EINVAL, "NULL argument");
+    if (!num_x) return MOTEX_OK;
+    std::vector<const char *> all(xs, xs + num_x);
+    std::vector<uint32_t> lens(xlens, xlens + num_x);
+    all.insert(all.end(), ys, ys + num_y);
+    lens.insert(lens.end(), ylens, ylens + num_y);
+    motex_batch *b = nullptr;
+    if ((rc = batch_create(c, all.data(), lens.data(), num_x + num_y, p, nullptr, &b, 1, num_x))) return rc;
+    std::unique_ptr<motex_batch, void (*)(motex_batch *)> guard(b, motex_batch_destroy);
+    if ((rc = motex_batch_run(b))) return rc;
+    // the texts' rows stay zero in a cross run; they are downloaded into scratch rows
+    std::vector<std::vector<uint32_t>> yu(num_y), yv(num_y);
+    std::vector<uint32_t *> up(u, u + num_x), vp(v, v + num_x);
+    for (uint32_t j = 0; j < num_y; ++j) {
+        yu[j].assign(std::max<uint32_t>(ylens[j], 1), 0);
+        yv[j].assign(std::max<uint32_t>(ylens[j], 1), 0);
+        up.push_back(yu[j].data());
+        vp.push_back(yv[j].data());
+    }
+    return motex_batch_download(b, up.data(), vp.data());
 }
 
 extern "C" int motex_extract_pair(motex_ctx *c, const char *x, uint32_t m, const char *y, uint32_t n,

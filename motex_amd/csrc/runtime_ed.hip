@@ -34,7 +34,7 @@ static void plan_ed(const SeqSet &ss, uint32_t k, const std::vector<uint32_t> &x
     if (only_y) {
         for (uint32_t j : *only_y) ranges.push_back({j, 1});
     } else {
-        uint32_t j = 0;
+        uint32_t j = ss.y0;
         while (j < ss.num) {
             uint32_t j0 = j;
             uint64_t acc = 0;
@@ -101,9 +101,10 @@ static bool ed_slice_ok(const motex_batch *b)
     // latency is long; with very few items the one-text-per-lane kernel, whose items shrink
     // to (x, one text), finishes first (40-100 protein sequences: 4.5-9 ms against 10-16 ms
     // pipelined and 31-51 ms with one wave per item).
-    uint64_t nx = 0;
-    for (uint32_t i = 0; i < b->ss.num; ++i) nx += b->ss.len[i] >= k;
-    const uint64_t items = nx * ((nx + 31) / 32);
+    uint64_t nx = 0, ny = 0;
+    for (uint32_t i = 0; i < b->ss.nx; ++i) nx += b->ss.len[i] >= k;
+    for (uint32_t j = b->ss.y0; j < b->ss.num; ++j) ny += b->ss.len[j] >= k;
+    const uint64_t items = nx * ((ny + 31) / 32);
     return items >= (uint64_t)b->ctx->num_cu * kSliceItemsPerCu;
 }
 
@@ -112,7 +113,7 @@ static int ed_prepare_slice(motex_batch *b)
     const SeqSet &ss = b->ss;
     const uint32_t k = b->params.k;
     std::vector<uint32_t> texts;
-    for (uint32_t j = 0; j < ss.num; ++j)
+    for (uint32_t j = ss.y0; j < ss.num; ++j)
         if (ss.len[j] >= k) texts.push_back(j);
     std::stable_sort(texts.begin(), texts.end(), [&](uint32_t x, uint32_t z) { return ss.len[x] > ss.len[z]; });
     const uint32_t ngroups = (uint32_t)((texts.size() + 31) / 32);
@@ -131,7 +132,7 @@ static int ed_prepare_slice(motex_batch *b)
         }
     }
     std::vector<EdSliceItem> items;
-    for (uint32_t i = 0; i < ss.num; ++i) {
+    for (uint32_t i = 0; i < ss.nx; ++i) {
         if (ss.len[i] < k) continue;
         for (uint32_t g = 0; g < ngroups; ++g)
             items.push_back({ss.start[i], ss.len[i], (uint32_t)ss.uv_off[i], g});
@@ -209,9 +210,9 @@ int ed_prepare(motex_batch *b)
 {
     if (ed_slice_ok(b)) return ed_prepare_slice(b);
     const SeqSet &ss = b->ss;
-    std::vector<uint32_t> xs(ss.num);
+    std::vector<uint32_t> xs(ss.nx);
     uint64_t nx = 0;
-    for (uint32_t i = 0; i < ss.num; ++i) {
+    for (uint32_t i = 0; i < ss.nx; ++i) {
         xs[i] = i;
         nx += ss.len[i] >= b->params.k;
     }
@@ -327,9 +328,9 @@ int ed_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x)
 {
     const SeqSet &ss = b->ss;
     double total = 0;
-    for (uint32_t i = first_x; i < first_x + num_x && i < ss.num; ++i) {
+    for (uint32_t i = first_x; i < first_x + num_x && i < ss.nx; ++i) {
         std::vector<std::pair<uint32_t, uint32_t>> pairs;
-        for (uint32_t j = 0; j < ss.num; ++j) pairs.push_back({i, j});
+        for (uint32_t j = ss.y0; j < ss.num; ++j) pairs.push_back({i, j});
         double ms = 0;
         int rc = ed_launch_pairs(b, pairs, &ms);
         if (rc) return rc;

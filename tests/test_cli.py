@@ -58,9 +58,21 @@ def test_validation_errors(tmp_path, fa, change, msg):
     assert rc == 1 and msg in err
 
 
-def test_background_mode_is_refused(tmp_path, fa):
-    rc, out, err = run(BASE + ["-i", fa, "-b", "bg", "-u", "un"], tmp_path)
-    assert rc == 1 and "not supported" in err
+def test_research_mode_needs_its_motif_file(tmp_path, fa):
+    # -I reads the FASTA input, then the unmatched-motif file, before any extraction
+    # (motex.cc:883-889)
+    rc, out, err = run(BASE + ["-i", fa, "-I", "nope.motex"], tmp_path)
+    assert rc == 1 and "Cannot open file nope.motex" in err
+
+
+@pytest.mark.parametrize("extra,msg", [
+    (["-I", "un.motex", "-b", "bg.motex", "-u", "un2"], "`-I' option cannot be used with `-b' option"),
+    (["-I", "un.motex", "-s", "boxes.txt"], "`-I' option cannot be used with `-s' option"),
+    (["-b", "bg.motex", "-u", "un", "-U", "un.smile"], "`-U' option must be used with `-S' option"),
+])
+def test_statistics_option_combinations(tmp_path, fa, extra, msg):
+    rc, out, err = run(BASE + ["-i", fa] + extra, tmp_path)
+    assert rc == 1 and msg in err
 
 
 def test_missing_input(tmp_path):

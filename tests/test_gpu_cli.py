@@ -4,6 +4,8 @@ date and timing lines)."""
 import os
 import subprocess
 
+import numpy as np
+
 import pytest
 
 import motex_amd
@@ -95,3 +97,49 @@ def test_hsapiens_genes(tmp_path, oracle):
     from motex_amd import refdata
     w = refdata.hsapiens(100)
     run_both(tmp_path, oracle, w)
+
+
+def test_background_and_research_modes(tmp_path, oracle):
+    """-b/-u/-U on the GPU against a background from an earlier run, then -I with the
+    unmatched motifs, each equal to the oracle's writers on the oracle's u/v."""
+    from oracle import stats as S
+    bg = synth.dna50_hamming(scale=0.6)          # 30 x 500 nt, k=10 e=1
+    fg = synth.dna50_hamming(scale=0.5)
+    fg.seqs = [s[100:400] for s in fg.seqs]       # 25 x 300 nt, sharing the planted motifs
+    for w in (bg, fg):
+        w.k, w.e, w.q = 7, 1, 30
+    (tmp_path / "bg.fa").write_bytes(bg.fasta())
+    (tmp_path / "fg.fa").write_bytes(fg.fasta())
+    base = ["-a", "DNA", "-d", "0", "-k", "7", "-e", "1", "-q", "30"]
+
+    def motex(args):
+        print("[cli] " + " ".join(args), flush=True)
+        r = subprocess.run([motex_amd.BIN_PATH] + args, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+
+    motex(["-i", "bg.fa", "-o", "bg.motex"] + base)
+    motex(["-i", "fg.fa", "-o", "fg.motex", "-b", "bg.motex", "-u", "un.motex", "-S", "fg.smile",
+           "-U", "un.smile"] + base)
+    bg_text = (tmp_path / "bg.motex").read_text()
+    fu, fv, foff, _ = oracle.all_pairs(fg.seqs, 0, 7, 1)
+    sw = {"input_filename": "fg.fa", "output_filename": "fg.motex", "d": 0, "q": 30, "k": 7, "e": 1}
+    want_out, want_un = S.write_back(sw, fg.seqs, fu, fv, foff, bg_text)
+    assert masked(tmp_path / "fg.motex") == [l for l in want_out.splitlines() if not l.startswith(MASK)]
+    assert masked(tmp_path / "un.motex") == [l for l in want_un.splitlines() if not l.startswith(MASK)]
+    # -I: the unmatched motifs searched again in the background input
+    motex(["-i", "bg.fa", "-o", "fore.motex", "-d", "0", "-k", "7", "-e", "1", "-q", "5", "-a", "DNA",
+           "-I", "un.motex"])
+    un_text = (tmp_path / "un.motex").read_text()
+    recs = S.read_motif_records(un_text)
+    assert recs
+    xs = [r[0].encode() for r in recs]
+    off = np.zeros(len(xs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in xs])
+    u = np.zeros(int(off[-1]), np.uint32)
+    v = np.zeros(int(off[-1]), np.uint32)
+    for i, x in enumerate(xs):
+        for y in bg.seqs:
+            oracle.pair(0, x, y, 7, 1, u[int(off[i]):int(off[i + 1])], v[int(off[i]):int(off[i + 1])])
+    sw2 = {"input_filename": "bg.fa", "output_filename": "fore.motex", "d": 0, "q": 5, "k": 7, "e": 1}
+    want = S.write_fore(sw2, recs, u, v, off, len(bg.seqs))
+    assert masked(tmp_path / "fore.motex") == [l for l in want.splitlines() if not l.startswith(MASK)]
