@@ -38,7 +38,7 @@ extern "C" {
 #define MOTEX_ENOMEM (-2)    /* host or device allocation failed, or over the HBM budget */
 #define MOTEX_EDEVICE (-3)   /* HIP runtime error */
 #define MOTEX_EUNSUPPORTED (-4) /* parameters outside what the kernels implement (Hamming k > 1024,
-                                  structured boxes > 64, edit k > 32, ...) */
+                                  structured boxes > 64, edit k > 64, ...) */
 #define MOTEX_EBUSY (-5)     /* serving: job not finished yet */
 #define MOTEX_ENOTFOUND (-6) /* serving: unknown job id */
 
@@ -81,7 +81,7 @@ typedef struct motex_stats {
     uint32_t tile_blocks;   /* autotuned tile shape used by the batched kernel */
     uint32_t tile_rows;
     uint32_t waves_per_cu;
-    uint32_t kernel_variant;  /* edit distance: 0/1 one text per lane (32/64-bit words),
+    uint32_t kernel_variant;  /* edit distance: 0/1/4 one text per lane (32/64/128-bit words),
                                  2 text-parallel one wave per item, 3 text-parallel pipelined */
     uint32_t graph_replay;  /* 1 if the last run replayed the batch's HIP graph */
     uint32_t symmetric;     /* 1 if each unordered pair is computed once and credited to

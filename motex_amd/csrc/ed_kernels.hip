@@ -14,9 +14,10 @@
 // the left neighbour's word arrives by one DPP wave shift per step.  Strips of one x run
 // one after another; lane 63's last column of every row is collected 64 rows at a time
 // and becomes the next strip's left boundary.  Words are 32-bit for k <= 14 (every
-// selected word is narrower than 2k+1 bits) and 64-bit up to k = 32.  min* uses one
-// unsigned key per candidate, popcount in the top bits above the complemented word, so a
-// single v_min3 selects it.
+// selected word is narrower than 2k+1 bits), 64-bit up to k = 32 and 128-bit up to k = 64
+// (four 32-bit pieces per DPP shift).  On 32-bit words min* uses one unsigned key per
+// candidate, popcount in the top bits above the complemented word, so a single v_min3
+// selects it.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "motex_internal.h"
@@ -27,7 +28,7 @@ namespace motex {
 // Occurrence counts go straight to global u/v with one atomic pair per (strip, text,
 // column with an occurrence): occurrences are rare, and keeping per-item accumulators in
 // LDS (16 KB for 2,000-symbol x) limited the kernel to 10 waves per CU, 23% slower.
-template <int C, bool WIDE>
+template <int C, int W>
 __global__ __launch_bounds__(64) void ed_kernel(EdArgs a)
 {
     const uint32_t lane = threadIdx.x;
@@ -51,21 +52,24 @@ __global__ __launch_bounds__(64) void ed_kernel(EdArgs a)
             const uint32_t ys = a.seq_start[yj];
             const uint32_t n = a.seq_start[yj + 1] - ys;
             if (n < k) continue;  // the pair is skipped (motex.cc:536-540)
-            ed_pair<C, WIDE, false>(a.codes + I.x_start, I.m, a.codes + ys, n, k, e, buf0, buf1,
+            ed_pair<C, W, false>(a.codes + I.x_start, I.m, a.codes + ys, n, k, e, buf0, buf1,
                                     a.u + I.uv_base, a.v + I.uv_base, none);
         }
     }
 }
 
-hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, hipStream_t s)
+hipError_t launch_ed(const EdArgs &a, int width, uint32_t grid, hipStream_t s)
 {
-    if (wide)
-        hipLaunchKernelGGL((ed_kernel<2, true>), dim3(grid), dim3(64), 0, s, a);
+    if (width == 2)
+        hipLaunchKernelGGL((ed_kernel<ed_cols(2), 2>), dim3(grid), dim3(64), 0, s, a);
+    else if (width == 1)
+        hipLaunchKernelGGL((ed_kernel<ed_cols(1), 1>), dim3(grid), dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL((ed_kernel<4, false>), dim3(grid), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((ed_kernel<ed_cols(0), 0>), dim3(grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-uint32_t ed_strip_cols(bool wide) { return wide ? 128 : 256; }
+uint32_t ed_strip_cols(int width) { return 64 * ed_cols(width); }
+uint32_t ed_stream_row_words(int width) { return width == 0 ? 1 : (width == 1 ? 2 : 4); }
 
 }  // namespace motex

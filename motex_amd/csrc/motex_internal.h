@@ -10,7 +10,7 @@ constexpr uint32_t kPadBits = 4096 + 2048;
 constexpr uint32_t kDiagPerBlock = 2048;     // 64 lanes x 32 diagonals
 constexpr uint32_t kMaxHdK = 64;             // structured Hamming boxes: counters of up to 7 planes
 constexpr uint32_t kMaxHdLongK = 1024;       // Hamming factors: 11 counter planes, single-block tiles
-constexpr uint32_t kMaxEdK = 32;             // edit kernels: bit-vectors stay below 64 bits
+constexpr uint32_t kMaxEdK = 64;             // edit kernels: bit-vectors stay below 128 bits
 // Text-parallel edit kernel: k up to 30, so that a candidate's popcount (at most k + 1) fits
 // its 5 bit-sliced planes.
 constexpr uint32_t kMaxSliceK = 30;

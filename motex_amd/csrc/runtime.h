@@ -188,8 +188,9 @@ TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss);
 hipError_t launch_ne_build(const uint8_t *codes, uint32_t total, uint32_t sigma, uint32_t W,
                            uint32_t *ne, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t val, hipStream_t s);
-hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, hipStream_t s);
-uint32_t ed_strip_cols(bool wide);
+hipError_t launch_ed(const EdArgs &a, int width, uint32_t grid, hipStream_t s);
+uint32_t ed_strip_cols(int width);          // columns per strip of the one-text-per-lane kernel
+uint32_t ed_stream_row_words(int width);    // boundary-stream words per row
 hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_t s);
 hipError_t launch_fold_fbits(const uint32_t *fbits, uint32_t nx, uint32_t f_words, uint32_t total,
                              uint32_t *u, hipStream_t s);

@@ -5,22 +5,22 @@
 #include <cstdlib>
 #include <vector>
 
+#include "ed_device.h"
 #include "runtime.h"
 
 namespace motex {
 
-hipError_t launch_ed(const EdArgs &a, bool wide, uint32_t grid, hipStream_t s);
 hipError_t launch_ed_slice(const EdSliceArgs &a, uint32_t grid, bool pipe, hipStream_t s);
 uint32_t ed_slice_stream_row_words(uint32_t k);
-uint32_t ed_strip_cols(bool wide);
 
 static const uint64_t kEdTextPerItem = 8192;  // text symbols per work item, at most
 
-static bool ed_wide(const motex_params &p)
+// Word width of the one-text-per-lane kernel: 32, 64 or 128 bits (ed_device.h).
+static int ed_word_width(const motex_params &p)
 {
     uint32_t longest = p.k;
     for (uint32_t g = 0; g < p.nb_gaps; ++g) longest = std::max(longest, p.box_len[g]);
-    return longest > 14;
+    return ed_width(longest);
 }
 
 // Items: (x, run of consecutive texts of about text_per_item symbols).
@@ -49,9 +49,9 @@ static void plan_ed(const SeqSet &ss, uint32_t k, const std::vector<uint32_t> &x
     }
 }
 
-static uint64_t ed_item_cost(const SeqSet &ss, const EdItemDev &it, bool wide)
+static uint64_t ed_item_cost(const SeqSet &ss, const EdItemDev &it, int width)
 {
-    uint32_t sc = ed_strip_cols(wide);
+    uint32_t sc = ed_strip_cols(width);
     uint64_t strips = (it.m + sc - 1) / sc, steps = 0;
     for (uint32_t j = it.j0; j < it.j0 + it.ny; ++j) steps += ss.len[j] + 63;
     return strips * steps;
@@ -62,7 +62,7 @@ static uint32_t ed_stream_words(const motex_batch *b)
 {
     uint32_t maxn = 0;
     for (uint32_t j = 0; j < b->ss.num; ++j) maxn = std::max(maxn, b->ss.len[j]);
-    return (maxn + 128) * (ed_wide(b->params) ? 2 : 1);
+    return (maxn + 128) * ed_stream_row_words(ed_word_width(b->params));
 }
 
 static uint32_t ed_grid(const motex_batch *b)
@@ -219,13 +219,13 @@ int ed_prepare(motex_batch *b)
     b->tile = choose_tile(b->ctx, b->params, ss);
     std::vector<EdItemDev> items;
     plan_ed(ss, b->params.k, xs, nullptr, ed_text_per_item(b, nx), items);
-    bool wide = ed_wide(b->params);
+    const int width = ed_word_width(b->params);
     std::vector<uint64_t> cost(items.size());
-    for (size_t t = 0; t < items.size(); ++t) cost[t] = ed_item_cost(ss, items[t], wide);
+    for (size_t t = 0; t < items.size(); ++t) cost[t] = ed_item_cost(ss, items[t], width);
     const std::vector<uint32_t> order = order_by_cost_desc(cost);
     std::vector<EdItemDev> sorted(items.size());
     for (size_t t = 0; t < items.size(); ++t) sorted[t] = items[order[t]];
-    b->tile.variant = wide ? 1 : 0;
+    b->tile.variant = width == 0 ? 0 : (width == 1 ? 1 : 4);
     hipError_t e;
     uint64_t ni = std::max<size_t>(sorted.size(), 1);
     if ((e = b->d_items.alloc(ni * sizeof(EdItemDev))) != hipSuccess) return hip_fail(e, "alloc items");
@@ -244,7 +244,7 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
                      uint32_t persistent, hipStream_t s)
 {
     hipError_t e;
-    bool wide = ed_wide(b->params);
+    const int width = ed_word_width(b->params);
     uint32_t stream_words = ed_stream_words(b);
     uint64_t need = (uint64_t)grid * 2 * stream_words * 4;
     if (b->d_aux.bytes < need && (e = batch_realloc_scratch(b, need)) != hipSuccess) return hip_fail(e, "alloc edit streams");
@@ -261,7 +261,7 @@ static int ed_launch(motex_batch *b, const void *items, uint32_t nitems, uint32_
     a.k = b->params.k;
     a.e = b->params.e;
     a.persistent = persistent;
-    if ((e = launch_ed(a, wide, grid, s)) != hipSuccess) return hip_fail(e, "launch edit kernel");
+    if ((e = launch_ed(a, width, grid, s)) != hipSuccess) return hip_fail(e, "launch edit kernel");
     return MOTEX_OK;
 }
 

@@ -170,7 +170,7 @@ static int struct_fill_args(motex_batch *b, StructArgs &a)
     a.gcap = kStructHitCap;
     a.gather_scratch = pair_lists ? 1u : 0u;
     a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
-    a.stream_words = (maxlen + 128) * 2;
+    a.stream_words = (maxlen + 128) * 4;  // up to four words per row (128-bit edit words)
     a.lcap = b->params.distance == 0 ? kStructListCap : 0;
     // bitmaps (Hamming: and their transposes), edit streams, counted bits, lists, gather list
     a.scratch_words = (b->params.distance == 0 ? 2ull : 1ull) * a.ndist * a.bm_words + 2ull * a.stream_words +

@@ -364,11 +364,14 @@ void struct_kernel(StructArgs a)
                 } else {
                     EdOut eo{out, m};
                     if (len <= 14)
-                        ed_pair<4, false, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
-                                                buf0, buf1, nullptr, nullptr, eo);
+                        ed_pair<ed_cols(0), 0, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
+                                                     buf0, buf1, nullptr, nullptr, eo);
+                    else if (len <= 32)
+                        ed_pair<ed_cols(1), 1, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
+                                                     buf0, buf1, nullptr, nullptr, eo);
                     else
-                        ed_pair<2, true, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
-                                               buf0, buf1, nullptr, nullptr, eo);
+                        ed_pair<ed_cols(2), 2, true>(a.codes + I.x_start, m, a.codes + I.y_start, n, len, err,
+                                                     buf0, buf1, nullptr, nullptr, eo);
                 }
             }
             __threadfence_block();

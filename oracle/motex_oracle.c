@@ -140,36 +140,74 @@ static int hd_hits(const unsigned char *x, uint32_t m, const unsigned char *t, u
     return 0;
 }
 
+/* Edit-distance words: 128 bits.  A selected word is narrower than 2k+1 bits, so 128 bits
+ * carry factors of up to k = 64 exactly.  For k <= 32 every word fits the reference's 64-bit
+ * WORD1 and the results are those of its 64-bit arithmetic; for 32 < k <= 64 the 64-bit
+ * restatement would drop the high bits of the unmasked "up" candidate, and the full word is
+ * kept, as libflasm's multi-word bit arrays (libflasm.h:44-54) keep a factor's errors. */
+typedef unsigned __int128 wide_t;
+
+static inline unsigned pcw(wide_t w) { return pc64((word_t)w) + pc64((word_t)(w >> 64)); }
+static inline wide_t onesw(unsigned r) { return r >= 128 ? ~(wide_t)0 : (((wide_t)1 << r) - 1); }
+
+/* bitminmax over 128-bit words: the same case analysis as orc_bitminmax. */
+static wide_t bitminmax_w(wide_t a, wide_t b, wide_t c)
+{
+    unsigned x = pcw(a), y = pcw(b), z = pcw(c);
+    unsigned lo = x < y ? x : y;
+    if (z < lo) return c;
+    if (z == lo && x != y) {
+        wide_t other = x < y ? a : b;
+        return c > other ? c : other;
+    }
+    if (z == x && x == y) {
+        wide_t mbc = b > c ? b : c;
+        return a > mbc ? a : mbc;
+    }
+    if (z > lo && x != y) return x < y ? a : b;
+    return a > b ? a : b;
+}
+
+/* for tests: bitminmax_w on words given as (lo, hi) halves, result's halves in out[2] */
+void orc_bitminmax_w(uint64_t alo, uint64_t ahi, uint64_t blo, uint64_t bhi, uint64_t clo, uint64_t chi,
+                     uint64_t *out)
+{
+    wide_t r = bitminmax_w(((wide_t)ahi << 64) | alo, ((wide_t)bhi << 64) | blo, ((wide_t)chi << 64) | clo);
+    out[0] = (uint64_t)r;
+    out[1] = (uint64_t)(r >> 64);
+}
+
 /* MaxShift edit distance, functions.cc:437-447 with bitminmax (functions.cc:923-967).
  * up = D[i-1][j], left = D[i][j-1], diag = D[i-1][j-1]. */
 static int ed_hits(const unsigned char *x, uint32_t m, const unsigned char *t, uint32_t n,
                    uint32_t ell, uint32_t err, hit_fn fn, void *ctx)
 {
     if (ell == 0 || ell > 64) return -1;
-    word_t ymask = ones(ell - 1);
-    word_t *prev = (word_t *)malloc((size_t)(m + 1) * sizeof(word_t));
-    word_t *cur = (word_t *)malloc((size_t)(m + 1) * sizeof(word_t));
+    wide_t ymask = onesw(ell - 1);
+    wide_t *prev = (wide_t *)malloc((size_t)(m + 1) * sizeof(wide_t));
+    wide_t *cur = (wide_t *)malloc((size_t)(m + 1) * sizeof(wide_t));
     if (!prev || !cur) { free(prev); free(cur); return -2; }
     prev[0] = 0;
-    for (uint32_t j = 1; j <= m; ++j) prev[j] = ones(j < ell ? j : ell);
+    for (uint32_t j = 1; j <= m; ++j) prev[j] = onesw(j < ell ? j : ell);
     for (uint32_t i = 1; i <= n; ++i) {
         unsigned char tc = t[i - 1];
         cur[0] = 0;
         for (uint32_t j = 1; j <= m; ++j) {
-            word_t dl = (word_t)(tc != x[j - 1]);
-            word_t d;
+            wide_t dl = (wide_t)(tc != x[j - 1]);
+            wide_t d;
             if (j <= ell)
-                d = orc_bitminmax((prev[j] << 1) | 1, (cur[j - 1] << 1) | 1, (prev[j - 1] << 1) | dl);
+                d = bitminmax_w((prev[j] << 1) | 1, (cur[j - 1] << 1) | 1, (prev[j - 1] << 1) | dl);
             else
-                d = orc_bitminmax(((cur[j - 1] & ymask) << 1) | 1, (prev[j] << 1) | 1,
-                                  ((prev[j - 1] & ymask) << 1) | dl);
+                d = bitminmax_w(((cur[j - 1] & ymask) << 1) | 1, (prev[j] << 1) | 1,
+                                ((prev[j - 1] & ymask) << 1) | dl);
+            if (ell < 64 && (d >> (2 * ell))) abort();  /* wider than 2k bits: the width bound failed */
             cur[j] = d;
             if (j >= ell) {
-                unsigned p = pc64(d);
+                unsigned p = pcw(d);
                 if (p <= err) fn(ctx, i - 1, j - 1, p);
             }
         }
-        word_t *tmp = prev; prev = cur; cur = tmp;
+        wide_t *tmp = prev; prev = cur; cur = tmp;
     }
     free(prev); free(cur);
     return 0;

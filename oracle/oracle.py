@@ -51,6 +51,7 @@ def lib():
         L.orc_all_pairs_range.restype = C.c_int64
         L.orc_bitminmax.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_bitminmax.restype = C.c_uint64
+        L.orc_bitminmax_w.argtypes = [C.c_uint64] * 6 + [u64p]
         L.orc_write.argtypes = [C.c_void_p, C.c_char_p, u64p, u32p, C.c_uint32, u64p, u32p, u32p,
                                 C.c_double, C.c_int, C.c_char_p, C.c_char_p]
         L.orc_write.restype = C.c_int64

@@ -81,6 +81,10 @@ def test_pipelined_strips_of_long_x(ctx, oracle, monkeypatch):
     (8, 3, 2, b"ACGT", 8, 2, 70),          # e = k-1, texts shorter than one wave
     (9, 10, 9, b"A", 5, 10, 200),          # one symbol: widest words, every tie
     (10, 4, 1, b"ARNDCQEGHILKMFPSTWYV", 40, 4, 300),  # protein, more than one text group
+    (14, 33, 6, b"AC", 5, 30, 300),        # 128-bit words from k = 33
+    (15, 48, 12, b"ACGT", 5, 40, 400),
+    (16, 64, 20, b"AC", 4, 60, 300),       # longest edit factor
+    (17, 64, 63, b"A", 3, 60, 200),        # one symbol, e = k - 1: widest 128-bit words
 ])
 def test_random_edges(ctx, oracle, kernel, seed, k, e, alpha, n, lo, hi):
     rng = np.random.default_rng(100 + seed)

@@ -85,3 +85,15 @@ def test_plain_form_equals_batched(ctx):
     up, vp, _ = b.download()
     assert np.array_equal(ub, up) and np.array_equal(vb, vp)
     b.close()
+
+
+def test_edit_boxes_with_128_bit_words(ctx, oracle):
+    rng = np.random.default_rng(41)
+    a = np.frombuffer(b"AC", np.uint8)
+    seqs = [a[rng.integers(0, 2, int(rng.integers(150, 260)))].tobytes() for _ in range(5)]
+    p = MotexParams(k=36, e=8, distance=1, boxes=[(3, 5, 34, 7)])
+    print(f"[struct] edit boxes 36/8 + [3,5] + 34/7 n={len(seqs)}", flush=True)
+    u, v, _ = ctx.extract_all_pairs(seqs, p)
+    ou, ov, _, _ = oracle.all_pairs(seqs, 1, 36, 8, boxes=[(3, 5, 34, 7)])
+    assert np.array_equal(u, ou) and np.array_equal(v, ov)
+    assert v.sum() > 0

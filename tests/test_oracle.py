@@ -235,3 +235,36 @@ def test_dna50_generator_is_deterministic():
     h = hashlib.sha256(b"".join(a.seqs)).hexdigest()
     golden = os.path.join(os.path.dirname(__file__), "golden", "dna50_sha256.txt")
     assert open(golden).read().strip() == h
+
+
+def test_wide_bitminmax_equals_the_64_bit_one(oracle):
+    # the 128-bit restatement used for edit factors up to k = 64 against the 64-bit one, on
+    # words that fit 64 bits, and on words whose high halves alone decide
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    out = np.zeros(2, np.uint64)
+    for _ in range(3000):
+        a, b, c = (int(w) for w in rng.integers(0, 1 << 20, 3, dtype=np.uint64))
+        a &= int(rng.integers(0, 1 << 20))
+        L.orc_bitminmax_w(a, 0, b, 0, c, 0, out.ctypes.data_as(oracle.u64p))
+        assert int(out[0]) == L.orc_bitminmax(a, b, c) and int(out[1]) == 0
+        L.orc_bitminmax_w(0, a, 0, b, 0, c, out.ctypes.data_as(oracle.u64p))
+        assert int(out[1]) == L.orc_bitminmax(a, b, c) and int(out[0]) == 0
+
+
+@pytest.mark.parametrize("k", [33, 48, 64])
+def test_e0_edit_hits_are_exact_occurrences_for_long_factors(oracle, k):
+    # factors past one 64-bit word: every e=0 hit is an exact occurrence, and long exact
+    # repeats are found
+    rng = np.random.default_rng(k)
+    base = rand_seqs(rng, 1, 3 * k, 3 * k, b"AC")[0]
+    seqs = [base, base[k // 2:] + base[: k // 2], rand_seqs(rng, 1, 2 * k, 4 * k, b"AC")[0]]
+    total = 0
+    for x in seqs:
+        for y in seqs:
+            ex = {(t + k - 1, p) for p in range(k - 1, len(x)) for t in range(len(y) - k + 1)
+                  if y[t:t + k] == x[p - k + 1:p + 1]}
+            h = {(int(t), int(p)) for t, p, _ in oracle.hits(1, x, y, k, 0)}
+            assert h <= ex
+            total += len(h)
+    assert total >= 3 * (2 * k + 1)   # at least the three self pairs' exact windows
