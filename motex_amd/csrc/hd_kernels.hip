@@ -27,12 +27,14 @@
 
 namespace motex {
 
+// pad: bit offset of text position 0 (kPadBits for the Hamming kernel; structured runs,
+// whose pair windows reach a whole sequence length to the left, use more)
 __global__ void ne_build_kernel(const uint8_t *__restrict__ codes, uint32_t total,
-                                uint32_t sigma, uint32_t W, uint32_t *__restrict__ ne)
+                                uint32_t sigma, uint32_t W, uint32_t *__restrict__ ne, uint32_t pad)
 {
     uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= W) return;
-    int64_t p0 = (int64_t)w * 32 - (int64_t)kPadBits;
+    int64_t p0 = (int64_t)w * 32 - (int64_t)pad;
     uint8_t cs[32];
 #pragma unroll
     for (int t = 0; t < 32; ++t) {
@@ -505,10 +507,10 @@ uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)
 }
 
 hipError_t launch_ne_build(const uint8_t *codes, uint32_t total, uint32_t sigma, uint32_t W,
-                           uint32_t *ne, hipStream_t s)
+                           uint32_t *ne, hipStream_t s, uint32_t pad)
 {
     hipLaunchKernelGGL(ne_build_kernel, dim3((W + 255) / 256), dim3(256), 0, s, codes, total,
-                       sigma, W, ne);
+                       sigma, W, ne, pad);
     return hipGetLastError();
 }
 

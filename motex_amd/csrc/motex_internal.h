@@ -8,7 +8,6 @@ namespace motex {
 // Positions before 0 and after the end are padding whose bits are never reported.
 constexpr uint32_t kPadBits = 4096 + 2048;
 constexpr uint32_t kDiagPerBlock = 2048;     // 64 lanes x 32 diagonals
-constexpr uint32_t kMaxHdK = 64;             // structured Hamming boxes: counters of up to 7 planes
 constexpr uint32_t kMaxHdLongK = 1024;       // Hamming factors: 11 counter planes, single-block tiles
 constexpr uint32_t kMaxEdK = 64;             // edit kernels: bit-vectors stay below 128 bits
 // Text-parallel edit kernel: k up to 30, so that a candidate's popcount (at most k + 1) fits
@@ -121,9 +120,11 @@ struct EdArgs {
     uint32_t persistent;      // 1: pull items from *queue; 0: item = blockIdx.x
 };
 
-constexpr uint32_t kMaxGaps = 4;
+constexpr uint32_t kMaxGaps = 8;
 constexpr uint32_t kMaxBoxes = kMaxGaps + 1;
-constexpr uint32_t kStructMaxLen = 4096;    // longest sequence of the structured kernels
+// Longest sequence of the structured kernels: occurrence lists pack (pos_x, pos_t) as two
+// 16-bit fields (the LDS window and x table must also fit, checked at batch creation).
+constexpr uint32_t kStructMaxLen = 65535;
 
 // One (x, y) pair of the structured-motif work queue.
 struct StructItem {
@@ -159,6 +160,10 @@ struct StructArgs {
     uint32_t lcap;                              // Hamming boxes: occurrence list entries per shape
     uint32_t gcap, gather_scratch;              // gathered box-0 hits: entries, and whether they
                                                 // live in scratch (after the lists) or in LDS
+    uint32_t pad_bits;                          // bit offset of text position 0 in the NE tables
+    uint32_t run_words;                         // LDS words per shape for its list's sorted runs
+                                                // (count + bounds: 2 + the 2048-diagonal blocks
+                                                // of the longest pair)
 };
 
 }  // namespace motex

@@ -186,7 +186,7 @@ TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss);
 
 // kernels (hd_kernels.hip, ed_kernels.hip)
 hipError_t launch_ne_build(const uint8_t *codes, uint32_t total, uint32_t sigma, uint32_t W,
-                           uint32_t *ne, hipStream_t s);
+                           uint32_t *ne, hipStream_t s, uint32_t pad = kPadBits);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t val, hipStream_t s);
 hipError_t launch_ed(const EdArgs &a, int width, uint32_t grid, hipStream_t s);
 uint32_t ed_strip_cols(int width);          // columns per strip of the one-text-per-lane kernel

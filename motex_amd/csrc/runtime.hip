@@ -122,7 +122,7 @@ int validate_params(const motex_params *p)
             return set_error(MOTEX_EINVAL, "gap %u: max below min", g + 1);
         if (p->box_len[g] == 0) return set_error(MOTEX_EINVAL, "box %u has length 0", g + 1);
     }
-    const uint32_t kmax = p->distance ? kMaxEdK : (p->nb_gaps ? kMaxHdK : kMaxHdLongK);
+    const uint32_t kmax = p->distance ? kMaxEdK : kMaxHdLongK;
     uint32_t longest = p->k;
     for (uint32_t g = 0; g < p->nb_gaps; ++g) longest = std::max(longest, p->box_len[g]);
     if (longest > kmax)

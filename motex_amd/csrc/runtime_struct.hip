@@ -20,10 +20,19 @@ static const uint32_t kStructPairListLds = 512;
 static const uint32_t kStructListCap = 8192; // Hamming occurrences listed per pair and box shape
 
 static int struct_alloc_scratch(motex_batch *b);
+static int struct_fill_args(motex_batch *b, StructArgs &a);
+
+// NE-table bit offset of text position 0 for sequences of up to maxlen symbols: the sweep of a
+// pair starts at diagonal pos_t - pos_x = -(m - 1), rounded down to a multiple of 32.
+static uint32_t struct_pad_bits(uint32_t maxlen)
+{
+    return std::max<uint32_t>(kPadBits, (maxlen + 63) / 32 * 32);
+}
 
 static size_t struct_lds_bytes(const StructArgs &a)
 {
-    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + 1 + kMaxBoxes + 8 * kMaxBoxes) * 4;
+    return ((size_t)(a.sigma + 1) * a.wl + (a.m_max + 1) / 2 + a.hcap + 1 + kMaxBoxes +
+            (size_t)a.run_words * kMaxBoxes) * 4;
 }
 
 // Fills the shape fields of StructArgs from the parameters.
@@ -96,19 +105,30 @@ int struct_prepare(motex_batch *b)
     for (uint32_t i = 0; i < ss.num; ++i) maxlen = std::max(maxlen, ss.len[i]);
     if (maxlen > kStructMaxLen)
         return set_error(MOTEX_EUNSUPPORTED, "structured motifs: sequences longer than %u", kStructMaxLen);
+
     hipError_t e;
     if (b->params.distance == 0) {
         // NE tables over the whole text; every pair's diagonal window must fit inside
         uint32_t wl = 64 + ((maxlen - 1) >> 5) + 3;
-        uint64_t W = (ss.total + kPadBits) / 32 + kDiagPerBlock / 32 + wl + 8;
+        // a pair's first block starts up to m - 1 + 31 positions left of the text's start
+        const uint32_t pad = struct_pad_bits(maxlen);
+        uint64_t W = (ss.total + pad) / 32 + kDiagPerBlock / 32 + wl + 8;
         if ((uint64_t)(ss.sigma + 1) * wl >= 65536 || ss.sigma * wl >= 65536)
             return set_error(MOTEX_EUNSUPPORTED, "alphabet too large for the structured kernel");
         b->hd.W = (uint32_t)W;
         b->hd.wl = wl;
         if ((e = b->d_ne.alloc(W * ss.sigma * 4)) != hipSuccess) return hip_fail(e, "alloc NE tables");
         if ((e = launch_ne_build((const uint8_t *)b->d_codes.p, (uint32_t)ss.total, ss.sigma, (uint32_t)W,
-                                 (uint32_t *)b->d_ne.p, bstream(b))) != hipSuccess)
+                                 (uint32_t *)b->d_ne.p, bstream(b), pad)) != hipSuccess)
             return hip_fail(e, "NE build");
+    }
+    {
+        StructArgs t{};
+        if ((rc = struct_fill_args(b, t))) return rc;
+        if (struct_lds_bytes(t) > 65536)
+            return set_error(MOTEX_EUNSUPPORTED,
+                             "structured motifs: sequences of %u symbols over %u letters exceed the kernel's LDS window",
+                             maxlen, ss.sigma);
     }
     std::vector<StructItem> items;
     b->stats.symmetric = struct_symmetric(b) ? 1u : 0u;
@@ -162,6 +182,7 @@ static int struct_fill_args(motex_batch *b, StructArgs &a)
     a.sigma = ss.sigma;
     a.wl = b->params.distance == 0 ? b->hd.wl : 0;
     a.m_max = std::max<uint32_t>(maxlen, 1);
+    a.pad_bits = struct_pad_bits(maxlen);
     const bool pair_lists = b->params.distance == 0 && b->params.nb_gaps == 1;
     // list merge: LDS keeps the counted bits (nb_structs x m_max), up to kStructPairListLds
     // words; larger ones go to scratch
@@ -172,6 +193,9 @@ static int struct_fill_args(motex_batch *b, StructArgs &a)
     a.bm_words = (uint64_t)((maxlen + 31) / 32) * maxlen;
     a.stream_words = (maxlen + 128) * 4;  // up to four words per row (128-bit edit words)
     a.lcap = b->params.distance == 0 ? kStructListCap : 0;
+    // a pair's diagonals q = pt - px run over (-m, n), from a multiple of 32 below: at most
+    // (m + n + 31) / 2048 + 1 blocks
+    a.run_words = 2 + (2 * a.m_max + 31) / kDiagPerBlock + 1;
     // bitmaps (Hamming: and their transposes), edit streams, counted bits, lists, gather list
     a.scratch_words = (b->params.distance == 0 ? 2ull : 1ull) * a.ndist * a.bm_words + 2ull * a.stream_words +
                       ((uint64_t)a.nb_structs * a.m_max + 31) / 32 + 1 + (uint64_t)a.ndist * a.lcap +

@@ -51,11 +51,12 @@ __device__ __forceinline__ uint32_t bm_window(const uint32_t *bm, uint32_t m, ui
 
 // Number of y-side gap combinations that complete the structured motif whose box 0 ends
 // at (pos_x = px0, pos_t = pt0) with x-side combination kx (functions.cc:653-685).
-__device__ uint32_t struct_walk(const StructArgs &a, uint32_t *const bm[kMaxBoxes], uint32_t m,
+template <int NB>
+__device__ uint32_t struct_walk(const StructArgs &a, uint32_t *const bm[NB], uint32_t m,
                                 uint32_t n, uint32_t kx, uint32_t px0, uint32_t pt0)
 {
     const uint32_t G = a.nb_gaps;
-    uint32_t jx[kMaxGaps];
+    uint32_t jx[NB - 1];
     uint32_t px = px0;
     for (uint32_t g = 0; g < G; ++g) {
         px += a.S[(size_t)kx * G + g] + a.blen[g + 1];
@@ -63,7 +64,7 @@ __device__ uint32_t struct_walk(const StructArgs &a, uint32_t *const bm[kMaxBoxe
         jx[g] = px;
     }
     // odometer over the y-side gaps 0 .. G-2; the last gap is a window popcount
-    uint32_t s[kMaxGaps], pt[kMaxGaps + 1];
+    uint32_t s[NB - 1], pt[NB];
     uint32_t total = 0;
     int g = 0;
     pt[0] = pt0;
@@ -126,7 +127,7 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
     // LDS atomic per occurrence); with bitmaps, the LDS counter
     uint32_t nl = __builtin_amdgcn_readfirstlane(*lcount);
     for (int64_t qb = q0; qb < qend; qb += kDiagPerBlock) {
-        const uint32_t wbase = (uint32_t)((qb + (int64_t)kPadBits) / 32);
+        const uint32_t wbase = (uint32_t)((qb + (int64_t)a.pad_bits) / 32);
         // text position of lane 0's flag 0 at row 0: within (-m - 32, n + 2048), so 32 bits
         const int32_t rb = (int32_t)(qb - (int64_t)I.y_start);
         __syncthreads();
@@ -208,7 +209,9 @@ __device__ void struct_hd_box(const StructArgs &a, const StructItem &I, uint32_t
 template <bool DENSE>
 constexpr int struct_waves_per_eu() { return DENSE ? 8 : 1; }
 
-template <int P, bool DENSE>
+// NB: boxes the per-pair arrays hold (5 up to four gaps, kMaxBoxes beyond): larger arrays
+// of bitmap pointers and walk state live in scratch and slow the common shapes down.
+template <int P, bool DENSE, int NB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(struct_waves_per_eu<DENSE>())))
 void struct_kernel(StructArgs a)
 {
@@ -222,7 +225,7 @@ void struct_kernel(StructArgs a)
     uint32_t *hits = smem + (a.sigma + 1) * wl + (a.m_max + 1) / 2;
     uint32_t *hcount = hits + a.hcap;
     uint32_t *lcount = hcount + 1;  // per box shape: occurrences listed this pair
-    uint32_t *runs = lcount + kMaxBoxes;  // per box shape: sorted runs of its list (8 words)
+    uint32_t *runs = lcount + kMaxBoxes;  // per box shape: sorted runs of its list (run_words)
     for (uint32_t w = lane; w < wl; w += 64) ne_l[a.sigma * wl + w] = 0;
     // scratch: box bitmaps | (Hamming) their transposes | two edit boundary streams |
     //          counted (kx, pos_x) bits | (Hamming) occurrence lists
@@ -262,7 +265,7 @@ void struct_kernel(StructArgs a)
             const uint32_t d0 = a.dist_of[0], d1 = a.dist_of[1];
             const uint32_t *l0 = lists + (size_t)d0 * a.lcap, *l1 = lists + (size_t)d1 * a.lcap;
             const uint32_t c0 = __builtin_amdgcn_readfirstlane(lcount[d0]);
-            const uint32_t *r1 = runs + 8 * d1;
+            const uint32_t *r1 = runs + a.run_words * d1;
             const uint32_t nr1 = __builtin_amdgcn_readfirstlane(r1[0]);
             const uint32_t lo = a.gmin[0] + a.blen[1], hi = a.gmax[0] + a.blen[1];
             // The binary searches probe box 1's pos_x keys from LDS when they fit where the x
@@ -314,14 +317,14 @@ void struct_kernel(StructArgs a)
         auto pair = [&](const StructItem &I, bool sym) -> bool {
             const uint32_t m = I.m, n = I.n;
             const uint32_t wn = (n + 31) / 32;
-            uint32_t *bmd[kMaxBoxes];
+            uint32_t *bmd[NB];
             for (uint32_t d = 0; d < a.ndist; ++d) bmd[d] = scr + (size_t)d * wn * m;
             // transposed bitmaps of a pair-symmetric item: bmT[(pos_x >> 5) * n + pos_t]
             const uint32_t wm = (m + 31) / 32;
-            uint32_t *bmTd[kMaxBoxes], *bmT[kMaxBoxes];
+            uint32_t *bmTd[NB], *bmT[NB];
             for (uint32_t d = 0; d < a.ndist; ++d) bmTd[d] = scrT + (size_t)d * wm * n;
             for (uint32_t b = 0; b <= a.nb_gaps; ++b) bmT[b] = bmTd[a.dist_of[b]];
-            uint32_t *bm[kMaxBoxes];
+            uint32_t *bm[NB];
             for (uint32_t b = 0; b <= a.nb_gaps; ++b) bm[b] = bmd[a.dist_of[b]];
             __syncthreads();
             if (a.distance == 0) {
@@ -357,7 +360,7 @@ void struct_kernel(StructArgs a)
                     __syncthreads();
                     if (pairlist)
                         struct_hd_box<P, false>(a, I, len, err, out, sym ? bmTd[d] : nullptr, ne_l, xoff,
-                                                lists + (size_t)d * a.lcap, lcount + d, runs + 8 * d);
+                                                lists + (size_t)d * a.lcap, lcount + d, runs + a.run_words * d);
                     else
                         struct_hd_box<P, true>(a, I, len, err, out, sym ? bmTd[d] : nullptr, ne_l, xoff,
                                                lists + (size_t)d * a.lcap, lcount + d, nullptr);
@@ -434,7 +437,7 @@ void struct_kernel(StructArgs a)
                         const uint32_t pt0 = orient ? (e >> 16) : (e & 0xFFFFu);
     #pragma unroll 4
                         for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
-                            const uint32_t c = struct_walk(a, bmo, mo, no, kx, px0, pt0);
+                            const uint32_t c = struct_walk<NB>(a, bmo, mo, no, kx, px0, pt0);
                             if (c) {
                                 const uint32_t idx = kx * mo + px0, bit = 1u << (idx & 31);
                                 const uint32_t old = atomicOr(cnt_bits + (idx >> 5), bit);
@@ -482,7 +485,7 @@ void struct_kernel(StructArgs a)
                     for (uint32_t h = lane; h < have; h += 64) {
                         uint32_t e = gather[h];
                         uint32_t px0 = e >> 16, pt0 = e & 0xFFFFu;
-                        uint32_t c = struct_walk(a, bm, m, n, kx, px0, pt0);
+                        uint32_t c = struct_walk<NB>(a, bm, m, n, kx, px0, pt0);
                         if (c) {
                             uint32_t idx = kx * m + px0, bit = 1u << (idx & 31);
                             uint32_t old = atomicOr(counted + (idx >> 5), bit);
@@ -543,11 +546,11 @@ void struct_kernel(StructArgs a)
 template <bool DENSE>
 static void launch_struct_p(const StructArgs &a, uint32_t planes, uint32_t grid, size_t lds, hipStream_t s)
 {
-    if (planes <= 3) hipLaunchKernelGGL((struct_kernel<3, DENSE>), dim3(grid), dim3(64), lds, s, a);
-    else if (planes == 4) hipLaunchKernelGGL((struct_kernel<4, DENSE>), dim3(grid), dim3(64), lds, s, a);
-    else if (planes == 5) hipLaunchKernelGGL((struct_kernel<5, DENSE>), dim3(grid), dim3(64), lds, s, a);
-    else if (planes == 6) hipLaunchKernelGGL((struct_kernel<6, DENSE>), dim3(grid), dim3(64), lds, s, a);
-    else hipLaunchKernelGGL((struct_kernel<7, DENSE>), dim3(grid), dim3(64), lds, s, a);
+    if (planes <= 3) hipLaunchKernelGGL((struct_kernel<3, DENSE, 5>), dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 4) hipLaunchKernelGGL((struct_kernel<4, DENSE, 5>), dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 5) hipLaunchKernelGGL((struct_kernel<5, DENSE, 5>), dim3(grid), dim3(64), lds, s, a);
+    else if (planes == 6) hipLaunchKernelGGL((struct_kernel<6, DENSE, 5>), dim3(grid), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((struct_kernel<7, DENSE, 5>), dim3(grid), dim3(64), lds, s, a);
 }
 
 hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, bool dense, hipStream_t s)
@@ -557,9 +560,17 @@ hipError_t launch_struct(const StructArgs &a, uint32_t grid, size_t lds, bool de
         uint32_t p = hd_counter_planes(a.dlen[d], a.derr[d]);
         planes = p > planes ? p : planes;
     }
-    if (dense) launch_struct_p<true>(a, planes, grid, lds, s);
-    else launch_struct_p<false>(a, planes, grid, lds, s);
+    if (planes > 7 || a.nb_gaps > 4) {
+        // boxes longer than 64 (up to 1024) or more than four gaps: one general form with
+        // eleven planes, the threshold set at run time
+        hipLaunchKernelGGL((struct_kernel<11, false, kMaxBoxes>), dim3(grid), dim3(64), lds, s, a);
+    } else if (dense) {
+        launch_struct_p<true>(a, planes, grid, lds, s);
+    } else {
+        launch_struct_p<false>(a, planes, grid, lds, s);
+    }
     return hipGetLastError();
 }
+
 
 }  // namespace motex

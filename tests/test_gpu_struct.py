@@ -97,3 +97,43 @@ def test_edit_boxes_with_128_bit_words(ctx, oracle):
     ou, ov, _, _ = oracle.all_pairs(seqs, 1, 36, 8, boxes=[(3, 5, 34, 7)])
     assert np.array_equal(u, ou) and np.array_equal(v, ov)
     assert v.sum() > 0
+
+
+def check_struct(ctx, oracle, seqs, d, k, e, boxes, tag):
+    print(f"[struct] {tag} n={len(seqs)} lens={[len(s) for s in seqs]} d={d} k={k} e={e} boxes={boxes}", flush=True)
+    p = MotexParams(k=k, e=e, distance=d, boxes=boxes)
+    u, v, _ = ctx.extract_all_pairs(seqs, p)
+    ou, ov, _, _ = oracle.all_pairs(seqs, d, k, e, boxes=boxes)
+    bad = np.nonzero((u != ou) | (v != ov))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}"
+    return u, v
+
+
+@pytest.mark.parametrize("d", [0, 1])
+def test_sequences_beyond_4096(ctx, oracle, sym, d):
+    # 5-10 knt sequences (the reference's structured operator has no length limit)
+    rng = np.random.default_rng(50 + d)
+    seqs = rand_seqs(rng, 3, 5000, 10000 if d == 0 else 6000, b"ACGT")
+    u, v = check_struct(ctx, oracle, seqs, d, 8, 1, [(10, 20, 8, 1)], "long")
+    assert v.sum() > 0
+
+
+def test_five_gaps(ctx, oracle, sym):
+    rng = np.random.default_rng(55)
+    seqs = rand_seqs(rng, 6, 60, 200, b"AC")
+    u, v = check_struct(ctx, oracle, seqs, 0, 4, 1, [(2, 3, 4, 1), (1, 2, 3, 0), (2, 2, 4, 1), (0, 1, 3, 1),
+                                                    (1, 3, 4, 2)], "five gaps")
+    assert v.sum() > 0
+
+
+def test_eight_gaps_edit(ctx, oracle):
+    rng = np.random.default_rng(56)
+    seqs = rand_seqs(rng, 4, 60, 120, b"AC")
+    check_struct(ctx, oracle, seqs, 1, 3, 1, [(1, 2, 3, 1)] * 8, "eight gaps, edit")
+
+
+def test_boxes_longer_than_64(ctx, oracle, sym):
+    rng = np.random.default_rng(57)
+    seqs = rand_seqs(rng, 4, 300, 600, b"AC")
+    u, v = check_struct(ctx, oracle, seqs, 0, 100, 30, [(5, 9, 80, 24)], "long boxes")
+    assert v.sum() > 0
