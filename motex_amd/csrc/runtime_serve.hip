@@ -3,9 +3,11 @@
 // A dispatcher thread per context owns kSlots HIP streams.  Jobs wait in FIFO order until
 // a stream is free and their estimated HBM footprint fits beside the jobs already
 // resident (the first job always fits so the queue cannot stall).  An admitted job's
-// sequences are packed and uploaded on its stream through the stream-ordered allocator
-// (no device-wide synchronisation), and its persistent kernel is enqueued with a grid
-// capped to half the device, so jobs that arrive while others run overlap on the GPU.
+// sequences are packed, planned and uploaded on its stream through the stream-ordered
+// allocator (no device-wide synchronisation) by a set-up thread of its own, so that the
+// dispatcher keeps detecting completions and admitting jobs while a large job's host
+// set-up runs; its persistent kernel is enqueued with a grid capped to half the device, so
+// jobs that arrive while others run overlap on the GPU.
 // Completion is detected by the job's end event; u and v are then accumulated into the
 // caller's arrays and the job's device memory is returned to the pool.
 #include <hip/hip_runtime.h>
@@ -44,6 +46,9 @@ struct Job {
 struct Serving {
     hipStream_t streams[kSlots] = {};
     bool busy[kSlots] = {};
+    // guarded by the context's mutex:
+    std::vector<std::shared_ptr<Job>> running;  // launched jobs, in launch order
+    int setups = 0;                             // set-up threads still working
 };
 
 static Serving *serving_of(motex_ctx *c);
@@ -96,71 +101,90 @@ static void finish_job(motex_ctx *c, Job &j, std::unique_lock<std::mutex> &lk)
     j.state = rc ? 3 : 2;
 }
 
+// Host set-up of an admitted job (packing, planning, upload) and its launch, on a thread of
+// its own so that the dispatcher keeps detecting completions and admitting jobs meanwhile.
+// Hands the launched job to the dispatcher's running list (or fails it) under the lock.
+static void setup_job(motex_ctx *c, std::shared_ptr<Job> jp)
+{
+    Serving *sv = serving_of(c);
+    (void)hipSetDevice(c->device);  // the context's creation selected this device already
+    motex_batch *b = nullptr;
+    int rc = batch_create(c, jp->seqs.data(), jp->lens.data(), (uint32_t)jp->seqs.size(), &jp->params,
+                          sv->streams[jp->slot], &b, 2);
+    if (!rc) {
+        b->grid_cap = std::max<uint32_t>(c->num_cu, (uint32_t)c->num_cu * b->tile.waves_per_cu / 2);
+        rc = batch_launch(b);
+    }
+    std::string err = rc ? motex_last_error() : "";
+    if (rc && b) {
+        motex_batch_destroy(b);
+        b = nullptr;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (rc) {
+        jp->rc = rc;
+        jp->err = err;
+        jp->state = 3;
+        jp->t_done = now_ms();
+        sv->busy[jp->slot] = false;
+        c->resident_bytes -= jp->footprint;
+    } else {
+        jp->batch = b;
+        sv->running.push_back(jp);
+    }
+    sv->setups--;
+    c->cv.notify_all();
+}
+
 static void dispatcher(motex_ctx *c)
 {
     (void)hipSetDevice(c->device);  // the context's creation selected this device already
     Serving *sv = serving_of(c);
-    std::vector<std::shared_ptr<Job>> running;
     std::unique_lock<std::mutex> lk(c->mu);
     for (;;) {
         if (c->stop) break;
-        bool progressed = false;
-        // completions
-        for (size_t t = 0; t < running.size();) {
-            Job &j = *running[t];
-            if (hipEventQuery(j.batch->ev1) == hipSuccess) {
-                finish_job(c, j, lk);
-                c->resident_bytes -= j.footprint;
-                running.erase(running.begin() + t);
-                progressed = true;
+        // completions: taken off the running list first, since finishing a job drops the
+        // lock and set-up threads append to the list meanwhile
+        std::vector<std::shared_ptr<Job>> done;
+        for (size_t t = 0; t < sv->running.size();) {
+            if (hipEventQuery(sv->running[t]->batch->ev1) == hipSuccess) {
+                done.push_back(sv->running[t]);
+                sv->running.erase(sv->running.begin() + t);
             } else {
                 ++t;
             }
         }
-        if (progressed) c->cv.notify_all();
+        for (auto &jp : done) {
+            finish_job(c, *jp, lk);
+            c->resident_bytes -= jp->footprint;
+        }
+        if (!done.empty()) c->cv.notify_all();
         // admissions, FIFO
         while (!c->pending.empty()) {
             std::shared_ptr<Job> jp = c->pending.front();
             int slot = -1;
             for (int s = 0; s < kSlots; ++s) if (!sv->busy[s]) { slot = s; break; }
             if (slot < 0) break;
-            if (!running.empty() && c->resident_bytes + jp->footprint > c->budget) break;
+            const bool resident = !sv->running.empty() || sv->setups > 0;
+            if (resident && c->resident_bytes + jp->footprint > c->budget) break;
             c->pending.pop_front();
             sv->busy[slot] = true;
             jp->slot = slot;
-            c->resident_bytes += jp->footprint;
-            lk.unlock();
-            motex_batch *b = nullptr;
-            int rc = batch_create(c, jp->seqs.data(), jp->lens.data(), (uint32_t)jp->seqs.size(), &jp->params,
-                                  sv->streams[slot], &b, 2);
-            if (!rc) {
-                jp->batch = b;
-                b->grid_cap = std::max<uint32_t>(c->num_cu, (uint32_t)c->num_cu * b->tile.waves_per_cu / 2);
-                rc = batch_launch(b);
-            }
-            lk.lock();
-            if (rc) {
-                if (b) { lk.unlock(); motex_batch_destroy(b); lk.lock(); }
-                jp->batch = nullptr;
-                jp->rc = rc;
-                jp->err = motex_last_error();
-                jp->state = 3;
-                jp->t_done = now_ms();
-                sv->busy[slot] = false;
-                c->resident_bytes -= jp->footprint;
-                c->cv.notify_all();
-                continue;
-            }
             jp->state = 1;
-            running.push_back(jp);
+            c->resident_bytes += jp->footprint;
+            sv->setups++;
+            std::thread(setup_job, c, jp).detach();
         }
-        if (running.empty())
+        if (sv->running.empty() && sv->setups == 0)
             c->cv.wait(lk, [&] { return c->stop || !c->pending.empty(); });
         else
             c->cv.wait_for(lk, std::chrono::microseconds(200));
     }
-    // shut down: wait for running jobs, fail pending ones
-    for (auto &jp : running) finish_job(c, *jp, lk);
+    // shut down: let set-ups end, wait for running jobs, fail pending ones
+    c->cv.wait(lk, [&] { return sv->setups == 0; });
+    std::vector<std::shared_ptr<Job>> left;
+    left.swap(sv->running);
+    for (auto &jp : left) finish_job(c, *jp, lk);
     for (auto &jp : c->pending) {
         jp->state = 3;
         jp->rc = MOTEX_EDEVICE;
