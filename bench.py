@@ -240,7 +240,7 @@ def main():
     kms = float(np.mean(kernel_ms))
     ops = OPS_PER_CELL[w.distance]
     algo = cells * ops / (kms / 1e3) / 1e12
-    kernel = ("struct_kernel" if w.boxes else
+    kernel = (("struct_range_kernel" if st["kernel_variant"] == 1 else "struct_kernel") if w.boxes else
               "hd_batched_kernel" if w.distance == 0 else
               {2: "ed_slice_kernel", 3: "ed_slice_pipe_kernel"}.get(st["kernel_variant"], "ed_kernel"))
     tr = traffic_from_profile(kernel, a.workload) if a.scale == 1.0 else None
@@ -290,7 +290,8 @@ def main():
                    "cells": cells, "k": w.k, "e": w.e, "distance": w.distance,
                    "tile": {"blocks": st["tile_blocks"], "rows": st["tile_rows"],
                             "waves_per_cu": st["waves_per_cu"]},
-                   "work_items": st["work_items"], "upload_s": round(t_up, 3)},
+                   "work_items": st["work_items"], "upload_s": round(t_up, 3),
+                   **({"handed_pairs": st["handed_pairs"]} if w.boxes else {})},
         "roofline": roofline,
         "edit_semantics": ("MaxShift recurrence of the reference's in-tree code (functions.cc:437-447, "
                            "bitminmax 923-967); equality with libflasm's flasm_ed is unpinned"

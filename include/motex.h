@@ -82,12 +82,15 @@ typedef struct motex_stats {
     uint32_t tile_rows;
     uint32_t waves_per_cu;
     uint32_t kernel_variant;  /* edit distance: 0/1/4 one text per lane (32/64/128-bit words),
-                                 2 text-parallel one wave per item, 3 text-parallel pipelined */
+                                 2 text-parallel one wave per item, 3 text-parallel pipelined;
+                                 structured: 0 one pair per item, 1 range-batched (Hamming boxes) */
     uint32_t graph_replay;  /* 1 if the last run replayed the batch's HIP graph */
     uint32_t symmetric;     /* 1 if each unordered pair is computed once and credited to
                                both orders (Hamming; cells still count every ordered pair) */
     double swept_cells;     /* cells the kernel actually sweeps (Hamming: wave rows x
                                diagonals of their windows, warm-up and padding included) */
+    uint64_t handed_pairs;  /* range-batched structured runs: pairs whose range overflowed its
+                               hit queue and went to the one-pair-per-item kernel */
 } motex_stats;
 
 /* A serving job.  The caller keeps seqs/lens/params and the output arrays alive until

@@ -41,7 +41,7 @@ class Stats(C.Structure):
                 ("tile_blocks", C.c_uint32), ("tile_rows", C.c_uint32),
                 ("waves_per_cu", C.c_uint32), ("kernel_variant", C.c_uint32),
                 ("graph_replay", C.c_uint32), ("symmetric", C.c_uint32),
-                ("swept_cells", C.c_double)]
+                ("swept_cells", C.c_double), ("handed_pairs", C.c_uint64)]
 
 
 class Tile(C.Structure):

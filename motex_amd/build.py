@@ -24,7 +24,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{os.path.join(ROOT, 'include')}", f"
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB_SOURCES = ["hd_kernels.hip", "ed_kernels.hip", "ed_slice_kernels.hip", "ed_slice_kernels_k15.hip",
-               "ed_slice_kernels_k23.hip", "struct_kernels.hip", "runtime.hip",
+               "ed_slice_kernels_k23.hip", "struct_kernels.hip", "struct_range_kernels.hip", "runtime.hip",
                "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip", "runtime_noalloc.hip"]
 CLI_SOURCES = ["cli/main.cpp", "cli/app.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/writer_back.cpp", "cli/stats.cpp",
                "cli/switches.cpp"]

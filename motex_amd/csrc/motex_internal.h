@@ -164,6 +164,82 @@ struct StructArgs {
     uint32_t run_words;                         // LDS words per shape for its list's sorted runs
                                                 // (count + bounds: 2 + the 2048-diagonal blocks
                                                 // of the longest pair)
+    const uint32_t *nitems_dev;                 // when set, the item count is read from here
+                                                // (at most nitems): the pairs the range kernel
+                                                // handed over
+};
+
+// Range-batched structured extraction (Hamming boxes).  A unit is every row of one x sequence
+// against one text range -- whole texts packed into one window, as for the Hamming kernel.  The
+// sweep kernel writes a unit's box hits to its slot; the merge kernel joins them into chains.
+struct SrItem {
+    uint32_t x_start, m;    // x: text position and length
+    uint32_t uv_base;       // u/v index of (structure 0, pos_x 0) of x
+    uint32_t r;             // text range
+    uint32_t xi;            // sequence index of x
+    uint32_t pad;
+};
+
+// One x of a range-batched run, for the fold of its accumulators.
+struct SrX {
+    uint32_t xi, m;
+    uint32_t uv_base;
+    uint32_t pad;
+};
+
+constexpr uint32_t kSrBlocks = 4;                 // 2048-diagonal blocks of a window (8192)
+constexpr uint32_t kSrMaxGaps = 4;                // gaps the merge nests (more: per-pair kernel)
+constexpr uint32_t kSrMaxBoxes = kSrMaxGaps + 1;
+constexpr uint32_t kSrHead = 8;                   // slot header: [0] 1 if the unit overflowed,
+                                                  // [1 + d] end of shape d's hits
+constexpr uint32_t kSrMergeThreads = 256;
+constexpr uint32_t kSrNoSelf = 0xFFFFFFFFu;
+
+struct SrFoldArgs {
+    const SrX *xs;
+    const uint32_t *self_ovf;
+    const uint32_t *S;
+    const unsigned long long *uv;
+    uint32_t *u, *v;
+    uint32_t nb_gaps, nb_structs;
+    uint32_t blen[kSrMaxBoxes];
+};
+constexpr uint32_t kSrKeyBits = 11;               // u key set of a unit: 2048 slots in LDS
+constexpr uint32_t kSrKeyProbes = 32;
+constexpr uint32_t kSrNoKey = 0xFFFFFFFFu;
+
+struct SrSweepArgs {
+    const uint32_t *ne;         // Hamming NE tables, text position 0 at bit pad_bits
+    const uint8_t *codes;
+    const uint32_t *seq_start;  // num_seqs + 1
+    const SrItem *items;
+    const HdRange *ranges;
+    uint32_t *slots;            // per unit of the launch: kSrHead + hcap words; hits are
+                                // px << 16 | (pt - px - q0), shape by shape, rows in order
+    uint32_t *queue;            // unit counter
+    StructItem *ovf;            // pairs handed to the per-pair kernel (a unit that overflowed)
+    uint32_t *ovf_count;        // [0] pairs handed over, [1] set when the list was full
+    uint32_t *self_ovf;         // per sequence: 1 if the unit holding its own pair overflowed
+    uint32_t ovf_cap;
+    uint32_t W, sigma, pad_bits;
+    uint32_t wl;                // NE window words per symbol: 4 blocks of a row's longest span
+    uint32_t first, nitems;     // units [first, first + nitems); unit first + i uses slot i
+    uint32_t m_max, hcap, ndist;
+    uint32_t dlen[kSrMaxBoxes], derr[kSrMaxBoxes];
+};
+
+struct SrMergeArgs {
+    const uint32_t *slots;
+    const SrItem *items;
+    const HdRange *ranges;
+    const uint32_t *seq_start;
+    const uint32_t *S;          // nb_structs x nb_gaps gap lengths
+    unsigned long long *uv;     // per u/v index: v in the low word, u in the high word
+    uint32_t first;             // unit of slot 0 (one workgroup per slot)
+    uint32_t m_max, hcap, ndist, nbk;  // nbk: position buckets of 16, (window + m_max + 15) / 16
+    uint32_t nb_gaps, nb_structs;
+    uint32_t blen[kSrMaxBoxes], dist_of[kSrMaxBoxes];
+    uint32_t gmin[kSrMaxGaps], gmax[kSrMaxGaps];
 };
 
 }  // namespace motex

@@ -161,8 +161,17 @@ struct motex_batch {
     bool ed_slice = false;           // edit runs on the text-parallel kernel
     bool ed_pipe = false;            // ... in its pipelined form (kSlicePipe waves per item)
     motex::DevBuf d_gplanes, d_goff; // its text groups
+    // range-batched structured plan (Hamming boxes): its units (ranges in d_ranges), the unit
+    // slots of one launch pair, per-launch unit counters, the list of pairs it hands to the
+    // per-pair kernel (whose count and full flag are words 2 and 3 of d_queue), the merge's
+    // 64-bit u/v accumulators, the x list of the fold and per-sequence self-pair flags
+    bool struct_range = false;
+    motex::DevBuf d_sr_items, d_sr_slots, d_sr_queue, d_ovf, d_sr_uv, d_sr_xs, d_sr_self;
+    uint32_t sr_nx = 0;
+    uint32_t sr_nitems = 0, sr_chunk = 0, sr_nchunks = 0, sr_grid = 0, sr_hcap = 0, sr_ovf_cap = 0, sr_ovf_grid = 0;
+    uint32_t sr_wl = 0, sr_nbk = 0, sr_m_max = 0;
     uint32_t ed_slice_stream_words = 0;
-    uint64_t work_items() const { return hd.items.size() + ed_nitems; }
+    uint64_t work_items() const { return hd.items.size() + ed_nitems + sr_nitems; }
 };
 
 namespace motex {
@@ -215,6 +224,7 @@ int ed_launch_all(motex_batch *b);
 int ed_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
 int struct_prepare(motex_batch *b);
 int struct_launch_all(motex_batch *b);
+int struct_range_finish(motex_batch *b);
 int struct_run_plain(motex_batch *b, uint32_t first_x, uint32_t num_x);
 int pair_run(motex_batch *b, uint32_t i, uint32_t j);
 int ed_pair_run(motex_batch *b, uint32_t i, uint32_t j);

@@ -690,7 +690,7 @@ int batch_finish(motex_batch *b)
     if (hipEventElapsedTime(&kms, b->ev2, b->ev1) != hipSuccess) kms = 0;
     b->stats.kernel_ms = ms;
     b->stats.main_kernel_ms = kms;
-    return MOTEX_OK;
+    return b->params.nb_gaps ? struct_range_finish(b) : MOTEX_OK;
 }
 
 // Plain per-pair form: one launch per (x, y) pair.  The launch covers the pair with a few

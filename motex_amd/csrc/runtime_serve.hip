@@ -65,6 +65,11 @@ static size_t footprint(const Job &j, const motex_ctx *c)
     uint64_t bytes = T + 8 * T * ns + 64 * 1024;
     if (j.params.nb_gaps) {
         bytes += grid * (((maxlen + 31) / 32 * maxlen) * (j.params.nb_gaps + 1) + 4 * (maxlen + 128) + ns * maxlen / 32) * 4;
+        if (j.params.distance == 0)  // range-batched plan: unit slots, 64-bit accumulators,
+                                     // overflow list, NE tables
+            bytes += std::min<uint64_t>((uint64_t)j.lens.size() * (T / 4096 + 1) * 32768, 2ull << 30) + 8 * T * ns +
+                     std::min<uint64_t>((uint64_t)j.lens.size() * j.lens.size(), 1u << 22) * 24 +
+                     32 * (T + 2 * maxlen + 16384) / 8;
     } else if (j.params.distance == 0) {
         bytes += 32 * (T + kPadBits + 16384) / 8;                // NE tables (up to 32 symbols)
         bytes += (T / 256 + 1) * (T / 7000 + 1) * sizeof(HdItem); // work items (upper bound)

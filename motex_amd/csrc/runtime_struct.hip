@@ -3,9 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <vector>
 
+#include "hd_device.h"
 #include "runtime.h"
 
 namespace motex {
@@ -95,6 +97,192 @@ static bool struct_symmetric(const motex_batch *b)
     return b->params.distance == 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// Range-batched plan (struct_range_kernels.hip)
+// ---------------------------------------------------------------------------------------
+
+hipError_t launch_struct_range_sweep(const SrSweepArgs &a, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_struct_range_merge(const SrMergeArgs &a, uint32_t units, size_t lds, hipStream_t s);
+size_t struct_range_sweep_lds(uint32_t sigma, uint32_t wl, uint32_t m_max);
+size_t struct_range_merge_lds(uint32_t hcap, uint32_t ndist, uint32_t nbk);
+hipError_t launch_struct_range_fold(const SrFoldArgs &a, uint32_t nx, hipStream_t s);
+
+static const uint32_t kSrWindow = kDiagPerBlock * kSrBlocks;  // diagonals of a range window
+static const uint32_t kSrHcapMin = 1024;
+static const uint32_t kSrHcapMax = 65472;             // hit indexes are 16-bit
+static const uint32_t kSrOvfCap = 1u << 22;           // pairs the overflow list holds
+static const uint64_t kSrSlotBytes = 2ull << 30;      // unit slots of one launch pair
+
+// Waves per CU the sweep kernel's registers allow, by counter planes.
+static uint32_t sr_waves_vgpr(uint32_t planes)
+{
+    return planes <= 4 ? 32 : planes == 5 ? 28 : planes == 6 ? 24 : 20;
+}
+
+// NE window words per symbol of the sweep: a row reads words (px >> 5) + 64 b + lane and the
+// next one, for blocks b < 4.
+static uint32_t sr_window_words(uint32_t maxlen)
+{
+    return kSrBlocks * 64 + 2 + (maxlen + 31) / 32;
+}
+
+// Chance that two windows of len symbols are within err mismatches, letters drawn
+// independently with match probability pm.
+static double sr_hit_probability(double pm, uint32_t len, uint32_t err)
+{
+    if (err >= len) return 1.0;
+    double p = 0, c = 1;  // c = C(len, i)
+    for (uint32_t i = 0; i <= err; ++i) {
+        p += c * std::pow(1 - pm, (double)i) * std::pow(pm, (double)(len - i));
+        c = c * (len - i) / (i + 1);
+    }
+    return std::min(1.0, p);
+}
+
+// Takes the range-batched plan when its kernels can run the batch: Hamming boxes, at most
+// kSrMaxGaps gaps, counters of at most 7 planes, every text and the longest x inside one
+// window, the NE window and a unit's hits in LDS -- and, unless MOTEX_STRUCT_RANGE=1 asks for
+// it regardless, few enough expected hits per unit that units rarely overflow their slots.
+// MOTEX_STRUCT_RANGE=0 turns it off.  Plans units, slots and the overflow list; *taken says
+// whether it did.
+static int struct_range_prepare(motex_batch *b, uint32_t maxlen, bool *taken)
+{
+    *taken = false;
+    const char *env = getenv("MOTEX_STRUCT_RANGE");
+    const bool off = env && env[0] == '0', force = env && env[0] == '1';
+    const SeqSet &ss = b->ss;
+    const uint32_t G = b->params.nb_gaps;
+    if (off || b->params.distance != 0 || ss.nx != ss.num || G > kSrMaxGaps) return MOTEX_OK;
+    StructArgs t{};
+    int rc = struct_shape(b, t);
+    if (rc) return rc;
+    uint32_t planes = 3;
+    for (uint32_t d = 0; d < t.ndist; ++d) planes = std::max(planes, hd_counter_planes(t.dlen[d], t.derr[d]));
+    if (planes > 7 || 2ull * maxlen + 32 > kSrWindow) return MOTEX_OK;
+    const uint32_t m_max = std::max<uint32_t>(maxlen, 1);
+    // the merge's u keys: (kx * m + px) * 32 + text in 32 bits
+    if ((uint64_t)t.nb_structs * m_max >= (1ull << 27)) return MOTEX_OK;
+    const uint32_t wl = sr_window_words(maxlen);
+    const size_t sweep_lds = struct_range_sweep_lds(ss.sigma, wl, m_max);
+    if (sweep_lds > 65536) return MOTEX_OK;
+    const uint32_t nbk = (kSrWindow + m_max + 15) / 16;
+    // a unit's hits take 2 bytes each in the merge's LDS buckets
+    const size_t fixed = struct_range_merge_lds(0, t.ndist, nbk);
+    if (fixed + 2 * kSrHcapMin > 65536) return MOTEX_OK;
+    const uint32_t hcap_lds = (uint32_t)std::min<size_t>(kSrHcapMax, (65536 - fixed) / 2 / 64 * 64);
+
+    // expected hits of a unit: every shape's rows x window x hit chance
+    std::vector<uint64_t> hist(256, 0);
+    for (uint8_t c : ss.codes) hist[c]++;
+    double pm = 0;
+    for (uint64_t h : hist) pm += ss.total ? ((double)h / ss.total) * ((double)h / ss.total) : 0;
+    double expect = 0;
+    for (uint32_t d = 0; d < t.ndist; ++d) {
+        if (maxlen < t.dlen[d]) continue;
+        expect += (double)(maxlen - t.dlen[d] + 1) * kSrWindow * sr_hit_probability(pm, t.dlen[d], t.derr[d]);
+    }
+    const double want = std::ceil(std::max<double>(kSrHcapMin, 2.5 * expect + 512) / 64) * 64;
+    if (want > hcap_lds && !force) return MOTEX_OK;
+    uint32_t hcap = (uint32_t)std::min<double>(hcap_lds, want);
+    // test knobs: smaller slots and overflow list (units that overflow, a full list)
+    if (const char *q = getenv("MOTEX_SR_HCAP")) hcap = std::max(64u, std::min(hcap, (uint32_t)atoi(q) / 64 * 64));
+
+    // ranges: consecutive texts that hold the whole structure, packed up to cap positions
+    const uint32_t tot = total_motif_length(&b->params);
+    const uint32_t cap = kSrWindow - m_max - 32;
+    std::vector<HdRange> ranges;
+    uint64_t texts = 0, text_syms = 0;
+    for (uint32_t j = 0; j < ss.num;) {
+        if (ss.len[j] < tot) { ++j; continue; }
+        HdRange r{ss.start[j], ss.start[j] + ss.len[j], j, 1};
+        uint32_t jj = j + 1;
+        while (jj < ss.num && ss.len[jj] >= tot && jj - j < kMaxRangeSeqs &&
+               (uint64_t)ss.start[jj] + ss.len[jj] - r.ys <= cap) {
+            r.ye = ss.start[jj] + ss.len[jj];
+            r.ny++;
+            ++jj;
+        }
+        ranges.push_back(r);
+        texts += r.ny;
+        text_syms += r.ye - r.ys;
+        j = jj;
+    }
+    // units: every x against every range, longest x first
+    std::vector<uint32_t> xs;
+    for (uint32_t i = 0; i < ss.num; ++i)
+        if (ss.len[i] >= tot) xs.push_back(i);
+    std::stable_sort(xs.begin(), xs.end(), [&](uint32_t p, uint32_t q) { return ss.len[p] > ss.len[q]; });
+    std::vector<SrItem> items;
+    items.reserve(xs.size() * ranges.size());
+    for (uint32_t i : xs)
+        for (uint32_t r = 0; r < ranges.size(); ++r) items.push_back({ss.start[i], ss.len[i], (uint32_t)ss.uv_off[i], r, i, 0});
+    std::vector<SrX> xlist;
+    for (uint32_t i : xs) xlist.push_back({i, ss.len[i], (uint32_t)ss.uv_off[i], 0});
+    const uint64_t slot_bytes = (uint64_t)(kSrHead + hcap) * 4;
+    const uint32_t chunk =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(items.size(), kSrSlotBytes / slot_bytes));
+    const uint32_t nchunks = items.empty() ? 0 : (uint32_t)((items.size() + chunk - 1) / chunk);
+
+    uint32_t wpc = (uint32_t)std::max<size_t>(1, std::min<size_t>(sr_waves_vgpr(planes), (160 * 1024) / sweep_lds));
+    if (const char *w = getenv("MOTEX_SR_WAVES")) wpc = std::max(1, std::min(32, atoi(w)));
+
+    hipError_t e;
+    hipStream_t s = bstream(b);
+    if ((e = b->d_ranges.alloc(std::max<size_t>(ranges.size(), 1) * sizeof(HdRange))) != hipSuccess)
+        return hip_fail(e, "alloc ranges");
+    if (!ranges.empty() &&
+        (e = hipMemcpyAsync(b->d_ranges.p, ranges.data(), ranges.size() * sizeof(HdRange), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(e, "copy ranges");
+    if ((e = b->d_sr_items.alloc(std::max<size_t>(items.size(), 1) * sizeof(SrItem))) != hipSuccess)
+        return hip_fail(e, "alloc units");
+    if (!items.empty() &&
+        (e = hipMemcpyAsync(b->d_sr_items.p, items.data(), items.size() * sizeof(SrItem), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(e, "copy units");
+    if ((e = b->d_sr_slots.alloc(std::max<uint64_t>(1, (uint64_t)chunk * slot_bytes))) != hipSuccess)
+        return hip_fail(e, "alloc unit slots");
+    if ((e = b->d_sr_queue.alloc(std::max<size_t>(nchunks, 1) * 4)) != hipSuccess) return hip_fail(e, "alloc queues");
+    if ((e = b->d_sr_uv.alloc(std::max<uint64_t>(ss.uv_off[ss.num], 1) * 8)) != hipSuccess)
+        return hip_fail(e, "alloc accumulators");
+    if ((e = b->d_sr_self.alloc(std::max<size_t>(ss.num, 1) * 4)) != hipSuccess) return hip_fail(e, "alloc self flags");
+    if ((e = b->d_sr_xs.alloc(std::max<size_t>(xlist.size(), 1) * sizeof(SrX))) != hipSuccess)
+        return hip_fail(e, "alloc x list");
+    if (!xlist.empty() &&
+        (e = hipMemcpyAsync(b->d_sr_xs.p, xlist.data(), xlist.size() * sizeof(SrX), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(e, "copy x list");
+    b->sr_nx = (uint32_t)xlist.size();
+    const uint64_t pairs = (uint64_t)xs.size() * texts;
+    b->sr_ovf_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(pairs, kSrOvfCap));
+    if (const char *o = getenv("MOTEX_SR_OVF_CAP")) b->sr_ovf_cap = std::max(1u, std::min(b->sr_ovf_cap, (uint32_t)atoi(o)));
+    if ((e = b->d_ovf.alloc((uint64_t)b->sr_ovf_cap * sizeof(StructItem))) != hipSuccess)
+        return hip_fail(e, "alloc overflow list");
+    b->struct_range = true;
+    b->sr_nitems = (uint32_t)items.size();
+    b->sr_chunk = chunk;
+    b->sr_nchunks = nchunks;
+    b->sr_grid = (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->num_cu * wpc, std::max<uint32_t>(chunk, 1));
+    b->sr_hcap = hcap;
+    b->sr_wl = wl;
+    b->sr_nbk = nbk;
+    b->sr_m_max = m_max;
+    // the per-pair kernel over handed-over pairs: a small persistent grid
+    b->sr_ovf_grid = std::min<uint32_t>(b->ctx->num_cu * 4, b->sr_ovf_cap);
+    b->tile.variant = 1;
+    b->tile.waves_per_cu = wpc;
+    b->stats.symmetric = 0;
+    // cells: one DP per distinct box shape and ordered pair; swept: every unit's window
+    double cells = 0, swept = 0;
+    for (uint32_t d = 0; d < t.ndist; ++d) {
+        double rows = 0;
+        for (uint32_t i : xs) rows += ss.len[i] >= t.dlen[d] ? ss.len[i] - t.dlen[d] + 1 : 0;
+        cells += rows * (double)text_syms;
+    }
+    for (const SrItem &it : items) swept += (double)it.m * kSrWindow * t.ndist;
+    b->stats.cells = cells;
+    b->stats.swept_cells = swept;
+    *taken = true;
+    return MOTEX_OK;
+}
+
 int struct_prepare(motex_batch *b)
 {
     const SeqSet &ss = b->ss;
@@ -112,7 +300,8 @@ int struct_prepare(motex_batch *b)
         uint32_t wl = 64 + ((maxlen - 1) >> 5) + 3;
         // a pair's first block starts up to m - 1 + 31 positions left of the text's start
         const uint32_t pad = struct_pad_bits(maxlen);
-        uint64_t W = (ss.total + pad) / 32 + kDiagPerBlock / 32 + wl + 8;
+        // (and the range kernel's four-block window)
+        uint64_t W = (ss.total + pad) / 32 + std::max<uint32_t>(kDiagPerBlock / 32 + wl, sr_window_words(maxlen)) + 8;
         if ((uint64_t)(ss.sigma + 1) * wl >= 65536 || ss.sigma * wl >= 65536)
             return set_error(MOTEX_EUNSUPPORTED, "alphabet too large for the structured kernel");
         b->hd.W = (uint32_t)W;
@@ -129,6 +318,18 @@ int struct_prepare(motex_batch *b)
             return set_error(MOTEX_EUNSUPPORTED,
                              "structured motifs: sequences of %u symbols over %u letters exceed the kernel's LDS window",
                              maxlen, ss.sigma);
+    }
+    if ((e = b->d_aux2.alloc(std::max<size_t>(b->S.size(), 1) * 4)) != hipSuccess) return hip_fail(e, "alloc S");
+    if (!b->S.empty() &&
+        (e = hipMemcpyAsync(b->d_aux2.p, b->S.data(), b->S.size() * 4, hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
+        return hip_fail(e, "copy S");
+    b->tile = choose_tile(b->ctx, b->params, ss);
+    bool range = false;
+    if ((rc = struct_range_prepare(b, maxlen, &range))) return rc;
+    if (range) {
+        // the per-pair kernel's scratch, for the pairs of ranges that overflow
+        if ((rc = struct_alloc_scratch(b))) return rc;
+        return hipStreamSynchronize(bstream(b)) == hipSuccess ? MOTEX_OK : set_error(MOTEX_EDEVICE, "NE build");
     }
     std::vector<StructItem> items;
     b->stats.symmetric = struct_symmetric(b) ? 1u : 0u;
@@ -147,11 +348,6 @@ int struct_prepare(motex_batch *b)
         (e = hipMemcpyAsync(b->d_items.p, items.data(), items.size() * sizeof(StructItem), hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
         return hip_fail(e, "copy items");
     b->ed_nitems = (uint32_t)items.size();
-    if ((e = b->d_aux2.alloc(std::max<size_t>(b->S.size(), 1) * 4)) != hipSuccess) return hip_fail(e, "alloc S");
-    if (!b->S.empty() &&
-        (e = hipMemcpyAsync(b->d_aux2.p, b->S.data(), b->S.size() * 4, hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
-        return hip_fail(e, "copy S");
-    b->tile = choose_tile(b->ctx, b->params, ss);
     // cells: one DP per distinct box shape and pair
     double cells = 0;
     for (const StructItem &it : items)
@@ -244,13 +440,135 @@ static int struct_alloc_scratch(motex_batch *b)
     StructArgs a{};
     int rc = struct_fill_args(b, a);
     if (rc) return rc;
-    const uint64_t need = (uint64_t)struct_grid(b, std::max<uint32_t>(b->ed_nitems, 1)) * a.scratch_words * 4;
+    const uint32_t grid = b->struct_range ? b->sr_ovf_grid : struct_grid(b, std::max<uint32_t>(b->ed_nitems, 1));
+    const uint64_t need = (uint64_t)grid * a.scratch_words * 4;
     hipError_t e = batch_realloc_scratch(b, need);
     return e == hipSuccess ? MOTEX_OK : hip_fail(e, "alloc structured scratch");
 }
 
+// Range-batched run: per launch pair, the sweep over a chunk of units into their slots and the
+// merge of those slots; then the per-pair kernel over the pairs of units whose hits overflowed
+// their slots (a count the last launch reads on the device).
+static int struct_range_launch(motex_batch *b)
+{
+    hipStream_t s = bstream(b);
+    hipError_t e;
+    StructArgs t{};
+    int rc = struct_fill_args(b, t);
+    if (rc) return rc;
+    uint32_t *q = (uint32_t *)b->d_queue.p;   // [1] handed pairs' counter, [2..3] their list
+    uint32_t *sq = (uint32_t *)b->d_sr_queue.p;
+    if (b->sr_nchunks && (e = launch_fill_u32(sq, b->sr_nchunks, 0, s)) != hipSuccess) return hip_fail(e, "clear queues");
+    if ((e = launch_fill_u32((uint32_t *)b->d_sr_uv.p, b->d_sr_uv.bytes / 4, 0, s)) != hipSuccess ||
+        (e = launch_fill_u32((uint32_t *)b->d_sr_self.p, b->d_sr_self.bytes / 4, 0, s)) != hipSuccess)
+        return hip_fail(e, "clear accumulators");
+    SrSweepArgs sa{};
+    sa.ne = t.ne;
+    sa.codes = t.codes;
+    sa.seq_start = (const uint32_t *)b->d_seq_start.p;
+    sa.items = (const SrItem *)b->d_sr_items.p;
+    sa.ranges = (const HdRange *)b->d_ranges.p;
+    sa.slots = (uint32_t *)b->d_sr_slots.p;
+    sa.ovf = (StructItem *)b->d_ovf.p;
+    sa.ovf_count = q + 2;
+    sa.ovf_cap = b->sr_ovf_cap;
+    sa.W = t.W;
+    sa.sigma = t.sigma;
+    sa.pad_bits = t.pad_bits;
+    sa.wl = b->sr_wl;
+    sa.m_max = b->sr_m_max;
+    sa.hcap = b->sr_hcap;
+    sa.ndist = t.ndist;
+    SrMergeArgs ma{};
+    ma.slots = sa.slots;
+    ma.items = sa.items;
+    ma.ranges = sa.ranges;
+    ma.seq_start = sa.seq_start;
+    ma.S = t.S;
+    ma.uv = (unsigned long long *)b->d_sr_uv.p;
+    sa.self_ovf = (uint32_t *)b->d_sr_self.p;
+    ma.m_max = b->sr_m_max;
+    ma.hcap = b->sr_hcap;
+    ma.ndist = t.ndist;
+    ma.nbk = b->sr_nbk;
+    ma.nb_gaps = t.nb_gaps;
+    ma.nb_structs = t.nb_structs;
+    for (uint32_t x = 0; x < kSrMaxBoxes; ++x) {
+        sa.dlen[x] = t.dlen[x];
+        sa.derr[x] = t.derr[x];
+        ma.blen[x] = t.blen[x];
+        ma.dist_of[x] = t.dist_of[x];
+    }
+    for (uint32_t g = 0; g < kSrMaxGaps; ++g) {
+        ma.gmin[g] = t.gmin[g];
+        ma.gmax[g] = t.gmax[g];
+    }
+    const size_t sweep_lds = struct_range_sweep_lds(t.sigma, sa.wl, sa.m_max);
+    const size_t merge_lds = struct_range_merge_lds(ma.hcap, ma.ndist, ma.nbk);
+    for (uint32_t c = 0; c < b->sr_nchunks; ++c) {
+        const uint32_t first = c * b->sr_chunk;
+        const uint32_t n = std::min(b->sr_chunk, b->sr_nitems - first);
+        sa.first = first;
+        sa.nitems = n;
+        sa.queue = sq + c;
+        uint32_t grid = std::min(b->sr_grid, n);
+        if (b->grid_cap) grid = std::min(grid, b->grid_cap);
+        if ((e = launch_struct_range_sweep(sa, grid, sweep_lds, s)) != hipSuccess)
+            return hip_fail(e, "launch structured range sweep");
+        ma.first = first;
+        if ((e = launch_struct_range_merge(ma, n, merge_lds, s)) != hipSuccess)
+            return hip_fail(e, "launch structured range merge");
+    }
+    // the accumulators into u and v, with every x against itself's diagonal chains
+    SrFoldArgs fa{};
+    fa.xs = (const SrX *)b->d_sr_xs.p;
+    fa.self_ovf = sa.self_ovf;
+    fa.S = t.S;
+    fa.uv = ma.uv;
+    fa.u = t.u;
+    fa.v = t.v;
+    fa.nb_gaps = t.nb_gaps;
+    fa.nb_structs = t.nb_structs;
+    for (uint32_t x = 0; x < kSrMaxBoxes; ++x) fa.blen[x] = t.blen[x];
+    if (b->sr_nx && (e = launch_struct_range_fold(fa, b->sr_nx, s)) != hipSuccess)
+        return hip_fail(e, "launch structured range fold");
+    // the handed-over pairs
+    t.items = (const StructItem *)b->d_ovf.p;
+    t.nitems = b->sr_ovf_cap;
+    t.nitems_dev = q + 2;
+    t.queue = q + 1;
+    t.persistent = 1;
+    uint32_t og = b->sr_ovf_grid;
+    if (b->grid_cap) og = std::min(og, b->grid_cap);
+    if ((uint64_t)og * t.scratch_words * 4 > b->d_aux.bytes)
+        return set_error(MOTEX_EINVAL, "internal: overflow grid of %u beyond its scratch", og);
+    t.scratch = (uint32_t *)b->d_aux.p;
+    const size_t lds = struct_lds_bytes(t);
+    const bool dense = t.nb_gaps == 1 && lds * 24 <= 160 * 1024;
+    if ((e = launch_struct(t, og, lds, dense, s)) != hipSuccess) return hip_fail(e, "launch structured kernel");
+    return MOTEX_OK;
+}
+
+// After a range-batched run: pairs handed to the per-pair kernel, and whether its list held them.
+int struct_range_finish(motex_batch *b)
+{
+    if (!b->struct_range) return MOTEX_OK;
+    uint32_t w[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(w, (uint32_t *)b->d_queue.p + 2, sizeof w, hipMemcpyDeviceToHost, bstream(b));
+    if (e == hipSuccess) e = hipStreamSynchronize(bstream(b));
+    if (e != hipSuccess) return hip_fail(e, "read overflow count");
+    b->stats.handed_pairs = w[0];
+    if (w[1])
+        return set_error(MOTEX_EDEVICE,
+                         "structured motifs: %u pairs overflowed the range kernel's queues, beyond the %u the "
+                         "per-pair list holds (MOTEX_STRUCT_RANGE=0 runs every pair on the per-pair kernel)",
+                         w[0], b->sr_ovf_cap);
+    return MOTEX_OK;
+}
+
 int struct_launch_all(motex_batch *b)
 {
+    if (b->struct_range) return struct_range_launch(b);
     if (!b->ed_nitems) return MOTEX_OK;
     uint32_t grid = struct_grid(b, b->ed_nitems);
     if (b->grid_cap) grid = std::min(grid, b->grid_cap);

@@ -241,6 +241,8 @@ void struct_kernel(StructArgs a)
     // set, through the occurrence lists, or all of them when a list overflowed); the first
     // pair of a launch clears them in full.
     bool clean = false;
+    // the pairs a range-batched launch handed over: their count is known on the device only
+    const uint32_t nitems = a.nitems_dev ? min(a.nitems, *a.nitems_dev) : a.nitems;
 
     for (uint32_t round = 0;; ++round) {
         uint32_t it;
@@ -249,9 +251,9 @@ void struct_kernel(StructArgs a)
             if (lane == 0) it = atomicAdd(a.queue, 1u);
             it = __builtin_amdgcn_readfirstlane(it);
         } else {
-            it = round == 0 ? blockIdx.x : a.nitems;
+            it = round == 0 ? blockIdx.x : nitems;
         }
-        if (it >= a.nitems) break;
+        if (it >= nitems) break;
         // The merge of functions.cc:647-690 for two boxes from their occurrence lists.  With
         // box 0 ending at (px0, pt0) and box 1 at (px1, pt1), the structured motif with x-side
         // gap gx and y-side gap gy occurs when px1 = px0 + gx + len1 and pt1 = pt0 + gy + len1:

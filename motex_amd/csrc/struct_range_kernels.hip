@@ -1,0 +1,526 @@
+// Structured motifs with Hamming boxes, range-batched (structured_motifs_extraction_hd,
+// functions.cc:564-705).  A unit is every row of one x sequence against one text range: several
+// whole texts packed into one 8192-diagonal window, as the Hamming kernel packs them.
+//
+// struct_range_sweep (one wave per unit, persistent): for every distinct box shape (length,
+// errors) the bit-sliced window counters of hd_device.h run down the rows of x over 4 x 2048
+// diagonals, the NE window in LDS; every hit goes to the unit's slot in HBM as
+// px << 16 | (its diagonal - the window's first), shape by shape, rows in order.  Hits are rare
+// (random DNA, 8 nt with one error: about 4 per row of 8192 diagonals), so a slot takes a few
+// KB and the sweep needs no LDS beyond the NE window and the x table.
+//
+// struct_range_merge (one workgroup of 256 per unit), functions.cc:647-690: every shape's hits
+// of the slot in LDS buckets of 16 text positions.  Each thread takes occurrences (px0, pt0) of
+// box 0 (read from the slot) with the window inside one text j; box b's row is fixed by the
+// x-side gap combination kx, px_b = px_{b-1} + S[kx][b-1] + len_b, and its occurrences are the
+// hits of that row in pt_{b-1} + [gmin + len_b, gmax + len_b] inside text j (one chain per
+// y-side combination), found in the buckets of that position window.  v[kx, px0] counts the
+// chains; u[kx, px0] gets 1 per text j with a chain: the unit keeps the (kx, px0, j) already
+// credited in an LDS key set.  With one gap, each box-1 hit of the window names its kx by its
+// row, so every gap combination is found in one lookup.
+//
+// struct_range_fold: the merge's accumulators (v and u of a u/v index in one 64-bit word, for
+// one atomic a chain) into u and v, adding x against itself's chains on its own diagonal.
+//
+// A unit whose hits do not fit its slot hands its pairs (x, j) to the per-pair kernel
+// (struct_kernels.hip) through an overflow list, which a later launch consumes; its slot is
+// marked and the merge skips it.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hd_device.h"
+#include "motex_internal.h"
+
+#include <climits>
+
+namespace motex {
+
+namespace {
+
+// First diagonal (pt - px) of a unit's window: the range's first position at the last row,
+// rounded down to a multiple of 32.
+__device__ __forceinline__ int32_t sr_window_q0(int32_t ys, uint32_t m)
+{
+    return (ys - (int32_t)m + 1) & ~31;
+}
+
+// One unit as the merge reads it: the slot's hits (HBM) and every shape's hits in position
+// buckets (LDS): entries px << 4 | (position & 15), bucket b of the running sequence over
+// (shape, bucket of 16 positions) holding entries [C[b-1], C[b]).  Positions are pt - q0.
+struct SrUnit {
+    const uint32_t *hits;   // px << 16 | (pt - px - q0); shape d's hits in row order
+    const uint32_t *ends;   // ends[d]: end of shape d's hits
+    const uint16_t *bq;
+    const uint32_t *C;
+    uint32_t nbk;           // buckets per shape
+    uint32_t m;             // rows of x
+    uint32_t self;          // text of the range that is x itself, or kSrNoSelf
+    int32_t diag;           // position - row on x's own diagonal (pt = x_start + px)
+};
+
+// First hit of shape d in row px or later.
+__device__ uint32_t sr_row_lower(const SrUnit &U, uint32_t d, uint32_t px)
+{
+    uint32_t lo = d ? U.ends[d - 1] : 0u, hi = U.ends[d];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((U.hits[mid] >> 16) < px) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// f(px, position) for every hit of shape d at a position in [lo, hi].
+template <class F>
+__device__ __forceinline__ void sr_scan(const SrUnit &U, uint32_t d, int32_t lo, int32_t hi, F &&f)
+{
+    const uint32_t base = d * U.nbk;
+    for (uint32_t bk = (uint32_t)lo >> 4; bk <= ((uint32_t)hi >> 4); ++bk) {
+        const uint32_t b = base + bk;
+        for (uint32_t e = b ? U.C[b - 1] : 0u, ee = U.C[b]; e < ee; ++e) {
+            const uint32_t c = U.bq[e];
+            const int32_t r = (int32_t)((bk << 4) | (c & 15u));
+            if (r >= lo && r <= hi) f(c >> 4, r);
+        }
+    }
+}
+
+// Text of the range holding position r (starts[0 .. ny] are the range's text starts).
+__device__ __forceinline__ uint32_t sr_text_of(const int32_t *starts, uint32_t ny, int32_t r)
+{
+    uint32_t lo = 0, hi = ny - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= r) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Complete chains of boxes B..nb_gaps for gap combination kx, given box B-1 ending at row pxp,
+// position rp, in a text ending before rend: box B's row is fixed by kx, its occurrences lie in
+// rp + [gmin + len_B, gmax + len_B] and inside the text.  Levels nest at compile time (one loop
+// nest, no per-thread stack).
+template <int B>
+__device__ uint32_t sr_chains(const SrMergeArgs &a, const SrUnit &U, uint32_t kx, uint32_t pxp, int32_t rp,
+                              int32_t rend)
+{
+    const uint32_t px = pxp + a.S[(size_t)kx * a.nb_gaps + B - 1] + a.blen[B];
+    if (px >= U.m) return 0;
+    const int32_t lo = rp + (int32_t)(a.gmin[B - 1] + a.blen[B]);
+    const int32_t hi = min(rp + (int32_t)(a.gmax[B - 1] + a.blen[B]), rend - 1);
+    if (lo > hi) return 0;
+    uint32_t total = 0;
+    sr_scan(U, a.dist_of[B], lo, hi, [&](uint32_t pxb, int32_t r) {
+        if (pxb != px) return;
+        if (B == (int)a.nb_gaps) {
+            ++total;
+        } else if constexpr (B < (int)kSrMaxGaps) {
+            total += sr_chains<B + 1>(a, U, kx, px, r, rend);
+        }
+    });
+    return total;
+}
+
+// u is credited once per (gap combination kx, row px0, text): the first chain of the three
+// that reaches the unit's key set (LDS, open addressing, key (kx * m + px0) * 32 + text) takes
+// it.  Returns 1 for a new key, 0 for a known one, 2 when the set is too full to say.
+__device__ __forceinline__ uint32_t sr_key_insert(uint32_t *keys, uint32_t key)
+{
+    uint32_t h = (key * 2654435761u) >> (32 - kSrKeyBits);
+    for (uint32_t probe = 0; probe < kSrKeyProbes; ++probe) {
+        const uint32_t old = atomicCAS(keys + h, kSrNoKey, key);
+        if (old == kSrNoKey) return 1;
+        if (old == key) return 0;
+        h = (h + 1) & ((1u << kSrKeyBits) - 1);
+    }
+    return 2;
+}
+
+// Whether the chain of occurrence (px0, r0) for kx is the least of its (kx, px0, text): no
+// occurrence of row px0 in the text with a smaller position has a chain for kx and, with one
+// gap, no hit of row px1 before r1 completes this one.  (The key set's fallback.)
+__device__ bool sr_first_chain(const SrMergeArgs &a, const SrUnit &U, uint32_t kx, uint32_t px0, int32_t r0,
+                               int32_t ts, int32_t te, uint32_t px1, int32_t r1);
+
+template <int B>
+__device__ uint32_t sr_chains(const SrMergeArgs &a, const SrUnit &U, uint32_t kx, uint32_t pxp, int32_t rp,
+                              int32_t rend);
+
+// Credits c chains of (kx, px0) from occurrence (px0, r0) in text ti: v always, u once per
+// (kx, px0, text) -- the unit's key set, or its fallback when full.  One 64-bit atomic adds
+// both (v in the low word).  x's own text takes no u here: the fold credits it, together with
+// the chain on x's own diagonal.
+__device__ __forceinline__ void sr_credit(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, uint32_t uv_base,
+                                          uint32_t kx, uint32_t px0, int32_t r0, uint32_t ti, int32_t ts,
+                                          int32_t te, uint32_t c, uint32_t px1, int32_t r1)
+{
+    unsigned long long add = c;
+    if (ti != U.self) {
+        const uint32_t k = sr_key_insert(keys, ((kx * U.m + px0) << 5) | ti);
+        if (k == 1 || (k == 2 && sr_first_chain(a, U, kx, px0, r0, ts, te, px1, r1))) add += 1ull << 32;
+    }
+    if (add) atomicAdd(a.uv + (size_t)uv_base + (size_t)kx * U.m + px0, add);
+}
+
+// Occurrence of box 0 at row px0, position r0, in text ti = [ts, te) (positions), any number of
+// gaps: its chains for every gap combination (on x's own diagonal, less the all-diagonal one).
+__device__ void sr_walk(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, uint32_t uv_base, uint32_t px0,
+                        int32_t r0, uint32_t ti, int32_t ts, int32_t te)
+{
+    const bool diag = ti == U.self && r0 - (int32_t)px0 == U.diag;
+    for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
+        uint32_t c = sr_chains<1>(a, U, kx, px0, r0, te);
+        if (!c) continue;
+        c -= diag ? 1u : 0u;
+        sr_credit(a, U, keys, uv_base, kx, px0, r0, ti, ts, te, c, 0, INT32_MIN);
+    }
+}
+
+// Occurrence of box 0, one gap: box 1 ends in row px1 = px0 + len1 + gmin + kx at a position
+// in r0 + [gmin + len1, gmax + len1], inside the text.  The buckets of that window give every
+// such hit, and each is one chain of kx (but the one on x's own diagonal).
+__device__ void sr_join(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, uint32_t uv_base, uint32_t px0,
+                        int32_t r0, uint32_t ti, int32_t ts, int32_t te)
+{
+    const uint32_t len1 = a.blen[1], gmn = a.gmin[0];
+    const uint32_t span = a.gmax[0] - gmn;
+    if (px0 + len1 + gmn >= U.m) return;
+    const int32_t lo = r0 + (int32_t)(gmn + len1), hi = min(r0 + (int32_t)(a.gmax[0] + len1), te - 1);
+    if (lo > hi) return;
+    const bool diag = ti == U.self && r0 - (int32_t)px0 == U.diag;
+    sr_scan(U, a.dist_of[1], lo, hi, [&](uint32_t px1, int32_t r1) {
+        const uint32_t kx = px1 - px0 - len1 - gmn;
+        if (kx > span || (diag && r1 - (int32_t)px1 == U.diag)) return;
+        sr_credit(a, U, keys, uv_base, kx, px0, r0, ti, ts, te, 1, px1, r1);
+    });
+}
+
+__device__ bool sr_first_chain(const SrMergeArgs &a, const SrUnit &U, uint32_t kx, uint32_t px0, int32_t r0,
+                               int32_t ts, int32_t te, uint32_t px1, int32_t r1)
+{
+    const uint32_t d0 = a.dist_of[0];
+    const int32_t from = ts + (int32_t)a.blen[0] - 1;
+    if (a.nb_gaps == 1) {
+        const uint32_t len1 = a.blen[1];
+        const int32_t off_lo = (int32_t)(a.gmin[0] + len1), off_hi = (int32_t)(a.gmax[0] + len1);
+        bool first = true;
+        sr_scan(U, a.dist_of[1], r0 + off_lo, r1 - 1, [&](uint32_t p, int32_t) {
+            if (p == px1) first = false;
+        });
+        for (uint32_t h2 = sr_row_lower(U, d0, px0); first && h2 < U.ends[d0] && (U.hits[h2] >> 16) == px0; ++h2) {
+            const int32_t r2 = (int32_t)(U.hits[h2] & 0xFFFFu) + (int32_t)px0;
+            if (r2 < from || r2 >= r0) continue;
+            sr_scan(U, a.dist_of[1], r2 + off_lo, min(r2 + off_hi, te - 1), [&](uint32_t p, int32_t) {
+                if (p == px1) first = false;
+            });
+        }
+        return first;
+    }
+    for (uint32_t h2 = sr_row_lower(U, d0, px0); h2 < U.ends[d0] && (U.hits[h2] >> 16) == px0; ++h2) {
+        const int32_t r2 = (int32_t)(U.hits[h2] & 0xFFFFu) + (int32_t)px0;
+        if (r2 < from || r2 >= r0) continue;
+        if (sr_chains<1>(a, U, kx, px0, r2, te)) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Sweep
+// ---------------------------------------------------------------------------------------
+
+template <int P>
+__global__ __launch_bounds__(64) void struct_range_sweep(SrSweepArgs a)
+{
+    static_assert(kSrBlocks == 4, "the hit emission names four blocks");
+    extern __shared__ uint32_t smem[];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t wl = a.wl;
+    // LDS: NE window (sigma rows of wl words) | x table (u16 byte offsets of a row's window
+    //      words) | one word to broadcast through
+    uint32_t *ne_l = smem;
+    uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + a.sigma * wl);
+    uint32_t *bcast = smem + a.sigma * wl + (a.m_max + 1) / 2;
+    const uint32_t slot_words = kSrHead + a.hcap;
+
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(a.queue, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= a.nitems) break;
+        const SrItem I = a.items[a.first + it];
+        const HdRange HR = a.ranges[I.r];
+        const uint32_t m = I.m;
+        uint32_t *slot = a.slots + (size_t)it * slot_words;
+        uint32_t *hits = slot + kSrHead;
+        const int32_t q0 = sr_window_q0((int32_t)HR.ys, m);
+        const uint32_t wbase = (uint32_t)((q0 + (int32_t)a.pad_bits) >> 5);
+        __syncthreads();  // the previous unit is done with LDS
+        for (uint32_t r = lane; r < m; r += 64)
+            xoff[r] = (uint16_t)(4 * (a.codes[I.x_start + r] * wl + (r >> 5)));
+        for (uint32_t c = 0; c < a.sigma; ++c) {
+            const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
+            for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
+        }
+        __syncthreads();
+
+        uint32_t nq = 0;  // hits so far (wave-uniform)
+        bool overflow = false;
+        const char *ne_b = reinterpret_cast<const char *>(ne_l) + 4 * lane;
+        for (uint32_t d = 0; d < a.ndist && !overflow; ++d) {
+            const uint32_t len = a.dlen[d], err = a.derr[d];
+            uint32_t hm[P];
+            planes_high_masks<P>(hm, hd_low_planes(err));
+            uint32_t cnt[kSrBlocks][P];
+#pragma unroll
+            for (int b = 0; b < (int)kSrBlocks; ++b) planes_set<P>(cnt[b], hd_bias(err));
+            for (uint32_t px = 0; px < m; ++px) {
+                const uint32_t *wn = reinterpret_cast<const uint32_t *>(ne_b + xoff[px]);
+                uint32_t high[kSrBlocks];
+                if (px >= len) {
+                    const uint32_t *wo = reinterpret_cast<const uint32_t *>(ne_b + xoff[px - len]);
+                    const uint32_t sho = (px - len) & 31;
+#pragma unroll
+                    for (int b = 0; b < (int)kSrBlocks; ++b) {
+                        const uint32_t an = __builtin_amdgcn_alignbit(wn[64 * b + 1], wn[64 * b], px & 31);
+                        const uint32_t bo = __builtin_amdgcn_alignbit(wo[64 * b + 1], wo[64 * b], sho);
+                        planes_slide<P>(cnt[b], an, bo);
+                        high[b] = planes_high_masked<P>(cnt[b], hm);
+                    }
+                } else {
+#pragma unroll
+                    for (int b = 0; b < (int)kSrBlocks; ++b) {
+                        const uint32_t an = __builtin_amdgcn_alignbit(wn[64 * b + 1], wn[64 * b], px & 31);
+                        planes_inc<P>(cnt[b], an);
+                        high[b] = planes_high_masked<P>(cnt[b], hm);
+                    }
+                }
+                if (px + 1 < len) continue;  // no full window in x yet
+                // hits: the clear bits of high, in any block
+                const uint32_t w0 = ~high[0], w1 = ~high[1], w2 = ~high[2], w3 = ~high[3];
+                const uint32_t any = w0 | w1 | w2 | w3;
+                const uint64_t mb = __ballot(any != 0);
+                if (!mb) continue;
+                const uint32_t nh = __builtin_popcount(w0) + __builtin_popcount(w1) + __builtin_popcount(w2) +
+                                    __builtin_popcount(w3);
+                uint64_t bt = __ballot(nh > 1);
+                if (!bt) {
+                    // at most one hit a lane (the common case): slot = lanes with a hit below
+                    const uint32_t tot = (uint32_t)__builtin_popcountll(mb);
+                    if (nq + tot > a.hcap) {
+                        overflow = true;
+                        break;
+                    }
+                    if (any) {
+                        const uint32_t b = ((w1 | w3) ? 1u : 0u) | ((w2 | w3) ? 2u : 0u);
+                        const uint32_t at =
+                            nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
+                        hits[at] = (px << 16) | (kDiagPerBlock * b + 32 * lane + __builtin_ctz(any));
+                    }
+                    nq += tot;
+                    continue;
+                }
+                // some lane has several: slots from the wave's exclusive prefix of the counts,
+                // one ballot per count level
+                uint32_t pre = 0, tot = 0;
+                bt = mb;
+                for (uint32_t t = 1; bt; ++t) {
+                    pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(bt >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bt, pre));
+                    tot += (uint32_t)__builtin_popcountll(bt);
+                    bt = __ballot(nh > t);
+                }
+                if (nq + tot > a.hcap) {
+                    overflow = true;
+                    break;
+                }
+                if (nh) {
+                    uint32_t at = nq + pre;
+                    const uint32_t tag = (px << 16) | (32 * lane);
+#pragma unroll
+                    for (int b = 0; b < (int)kSrBlocks; ++b) {
+                        uint32_t hv = ~high[b];
+                        while (hv) {
+                            hits[at++] = tag + kDiagPerBlock * b + __builtin_ctz(hv);
+                            hv &= hv - 1;
+                        }
+                    }
+                }
+                nq += tot;
+            }
+            if (lane == 0) slot[1 + d] = nq;
+        }
+        if (lane == 0) slot[0] = overflow ? 1u : 0u;
+        if (overflow) {
+            // hand the unit's pairs to the per-pair kernel
+            if (lane == 0) {
+                const uint32_t at = atomicAdd(a.ovf_count, HR.ny);
+                if (at + HR.ny > a.ovf_cap) atomicOr(a.ovf_count + 1, 1u);
+                bcast[0] = at;
+                // x against itself goes with them: the fold must not credit its diagonal
+                if (I.xi >= HR.j0 && I.xi < HR.j0 + HR.ny) a.self_ovf[I.xi] = 1;
+            }
+            __syncthreads();
+            const uint32_t at = bcast[0];
+            for (uint32_t y = lane; y < HR.ny; y += 64) {
+                if (at + y >= a.ovf_cap) break;
+                const uint32_t j = HR.j0 + y;
+                const uint32_t ts = a.seq_start[j], n = a.seq_start[j + 1] - ts;
+                a.ovf[at + y] = StructItem{I.x_start, m, ts, n, I.uv_base, kNoSplit};
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Merge
+// ---------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kSrMergeThreads) void struct_range_merge(SrMergeArgs a)
+{
+    extern __shared__ uint32_t smem[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t *slot = a.slots + (size_t)blockIdx.x * (kSrHead + a.hcap);
+    if (slot[0]) return;  // overflowed: the per-pair kernel takes its pairs
+    const SrItem I = a.items[a.first + blockIdx.x];
+    const HdRange HR = a.ranges[I.r];
+    const uint32_t m = I.m;
+    const int32_t q0 = sr_window_q0((int32_t)HR.ys, m);
+    // LDS: text starts (kMaxRangeSeqs + 1, as positions pt - q0) | shape ends (kSrMaxBoxes) |
+    //      u key set (2^kSrKeyBits) | bucket ends (ndist x nbk) | bucket entries (u16, hcap)
+    int32_t *starts = reinterpret_cast<int32_t *>(smem);
+    uint32_t *ends = smem + kMaxRangeSeqs + 1;
+    uint32_t *keys = ends + kSrMaxBoxes;
+    uint32_t *C = keys + (1u << kSrKeyBits);
+    const uint32_t nbc = a.ndist * a.nbk;
+    uint16_t *bq = reinterpret_cast<uint16_t *>(C + nbc);
+    const uint32_t *hits = slot + kSrHead;
+    if (tid <= HR.ny) starts[tid] = (int32_t)a.seq_start[HR.j0 + tid] - q0;
+    if (tid < a.ndist) ends[tid] = slot[1 + tid];
+    for (uint32_t b = tid; b < nbc; b += kSrMergeThreads) C[b] = 0;
+    for (uint32_t k = tid; k < (1u << kSrKeyBits); k += kSrMergeThreads) keys[k] = kSrNoKey;
+    __syncthreads();
+    const uint32_t nh = ends[a.ndist - 1];
+    // every shape's hits into position buckets: count, scan, scatter
+    for (uint32_t h = tid; h < nh; h += kSrMergeThreads) {
+        const uint32_t hv = hits[h];
+        uint32_t d = 0;
+        while (h >= ends[d]) ++d;
+        atomicAdd(C + d * a.nbk + (((hv & 0xFFFFu) + (hv >> 16)) >> 4), 1u);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t per = (nbc + 63) / 64, c0 = min(tid * per, nbc), c1 = min(c0 + per, nbc);
+        uint32_t sum = 0;
+        for (uint32_t c = c0; c < c1; ++c) sum += C[c];
+        uint32_t incl = sum;
+#pragma unroll
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (tid >= o) incl += t;
+        }
+        uint32_t run = incl - sum;
+        for (uint32_t c = c0; c < c1; ++c) {
+            const uint32_t n = C[c];
+            C[c] = run;
+            run += n;
+        }
+    }
+    __syncthreads();
+    for (uint32_t h = tid; h < nh; h += kSrMergeThreads) {
+        const uint32_t hv = hits[h];
+        uint32_t d = 0;
+        while (h >= ends[d]) ++d;
+        const uint32_t px = hv >> 16, r = (hv & 0xFFFFu) + px;
+        bq[atomicAdd(C + d * a.nbk + (r >> 4), 1u)] = (uint16_t)((px << 4) | (r & 15u));
+    }
+    __syncthreads();
+
+    // occurrences of box 0
+    const uint32_t self = I.xi >= HR.j0 && I.xi < HR.j0 + HR.ny ? I.xi - HR.j0 : kSrNoSelf;
+    const SrUnit U{hits, ends, bq, C, a.nbk, m, self, (int32_t)I.x_start - q0};
+    const int32_t ys = starts[0], ye = starts[HR.ny];
+    const uint32_t len0 = a.blen[0];
+    const uint32_t d0 = a.dist_of[0];
+    for (uint32_t h = (d0 ? ends[d0 - 1] : 0u) + tid, he = ends[d0]; h < he; h += kSrMergeThreads) {
+        const uint32_t hv = hits[h];
+        const uint32_t px0 = hv >> 16;
+        const int32_t r0 = (int32_t)(hv & 0xFFFFu) + (int32_t)px0;
+        if (r0 < ys || r0 >= ye) continue;
+        const uint32_t ti = sr_text_of(starts, HR.ny, r0);
+        const int32_t ts = starts[ti], te = starts[ti + 1];
+        if (r0 - (int32_t)len0 + 1 < ts) continue;  // box 0's window inside the text
+        if (a.nb_gaps == 1)
+            sr_join(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
+        else
+            sr_walk(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Fold: the merge's accumulators into u and v, with x against itself's all-diagonal chains
+// ---------------------------------------------------------------------------------------
+
+// x against itself has, for every gap combination kx and row px0 whose structure fits in x
+// (box 0 ends at px0 >= len0 - 1, the last box at px0 + sum(S[kx][g] + len_{g+1}) < m), the
+// chain that lies on the diagonal -- every box a 0-error occurrence of itself.  It is one chain
+// of v and, x being one text, all of u for that text; the merge leaves both to this pass, unless
+// the unit holding the pair overflowed (then the per-pair kernel computes the pair whole).
+__global__ __launch_bounds__(256) void struct_range_fold(SrFoldArgs a)
+{
+    const SrX X = a.xs[blockIdx.x];
+    const bool self = a.self_ovf[X.xi] == 0;
+    const uint32_t n = a.nb_structs * X.m;
+    for (uint32_t e = threadIdx.x; e < n; e += 256) {
+        const uint32_t kx = e / X.m, px = e - kx * X.m;
+        uint32_t end = px;
+        for (uint32_t g = 0; g < a.nb_gaps; ++g) end += a.S[(size_t)kx * a.nb_gaps + g] + a.blen[g + 1];
+        const uint32_t d = (self && px + 1 >= a.blen[0] && end < X.m) ? 1u : 0u;
+        const size_t o = (size_t)X.uv_base + e;
+        const unsigned long long w = a.uv[o];
+        a.v[o] += (uint32_t)w + d;
+        a.u[o] += (uint32_t)(w >> 32) + d;
+    }
+}
+
+hipError_t launch_struct_range_fold(const SrFoldArgs &a, uint32_t nx, hipStream_t s)
+{
+    hipLaunchKernelGGL(struct_range_fold, dim3(nx), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_struct_range_sweep(const SrSweepArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+{
+    uint32_t planes = 1;
+    for (uint32_t d = 0; d < a.ndist; ++d) {
+        const uint32_t p = hd_counter_planes(a.dlen[d], a.derr[d]);
+        planes = p > planes ? p : planes;
+    }
+    switch (planes <= 3 ? 3 : planes) {
+    case 3: hipLaunchKernelGGL((struct_range_sweep<3>), dim3(grid), dim3(64), lds, s, a); break;
+    case 4: hipLaunchKernelGGL((struct_range_sweep<4>), dim3(grid), dim3(64), lds, s, a); break;
+    case 5: hipLaunchKernelGGL((struct_range_sweep<5>), dim3(grid), dim3(64), lds, s, a); break;
+    case 6: hipLaunchKernelGGL((struct_range_sweep<6>), dim3(grid), dim3(64), lds, s, a); break;
+    case 7: hipLaunchKernelGGL((struct_range_sweep<7>), dim3(grid), dim3(64), lds, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_struct_range_merge(const SrMergeArgs &a, uint32_t units, size_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL(struct_range_merge, dim3(units), dim3(kSrMergeThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+// LDS bytes of a sweep workgroup and of a merge workgroup (the layouts at the top of each).
+size_t struct_range_sweep_lds(uint32_t sigma, uint32_t wl, uint32_t m_max)
+{
+    return ((size_t)sigma * wl + (m_max + 1) / 2 + 1) * 4;
+}
+
+size_t struct_range_merge_lds(uint32_t hcap, uint32_t ndist, uint32_t nbk)
+{
+    return ((size_t)kMaxRangeSeqs + 1 + kSrMaxBoxes + (1u << kSrKeyBits) + (size_t)ndist * nbk + (hcap + 1) / 2) * 4;
+}
+
+}  // namespace motex

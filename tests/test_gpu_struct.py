@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from motex_amd import MotexParams, synth
+from motex_amd import MotexError, MotexParams, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -12,33 +12,53 @@ def rand_seqs(rng, n, lo, hi, alpha):
     return [a[rng.integers(0, len(a), rng.integers(lo, hi + 1))].tobytes() for _ in range(n)]
 
 
-@pytest.fixture(params=["1", "0"], ids=["symmetric", "ordered"])
+@pytest.fixture(params=["range", "symmetric", "ordered"])
 def sym(request, monkeypatch):
-    """Runs a test with the pair-symmetric structured plan (Hamming boxes: each unordered
-    pair's box DPs serve both orders) and with it switched off."""
-    monkeypatch.setenv("MOTEX_SYM", request.param)
-    return request.param == "1"
+    """Runs a test with each structured plan: range-batched (Hamming boxes: every row of x
+    against packed text ranges, struct_range_kernels.hip), and one pair per work item with
+    the pair-symmetric plan (each unordered pair's box DPs serve both orders) and without."""
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1" if request.param == "range" else "0")
+    monkeypatch.setenv("MOTEX_SYM", "0" if request.param == "ordered" else "1")
+    return request.param
+
+
+def planes(length, err):
+    low = err.bit_length()
+    return ((1 << low) - 1 - err + length).bit_length()
+
+
+def range_eligible(seqs, k, e, boxes, distance):
+    """Whether the range-batched kernel can take the batch: Hamming boxes, at most 4 gaps,
+    counters of at most 7 planes, a whole text and the longest x in one 8192-diagonal window."""
+    maxlen = max(len(s) for s in seqs)
+    shapes = [(k, e)] + [(b[2], b[3]) for b in boxes]
+    return (distance == 0 and len(boxes) <= 4 and 2 * maxlen + 32 <= 8192
+            and all(planes(l, x) <= 7 for l, x in shapes))
 
 
 def check(ctx, oracle, seqs, k, e, boxes, distance=0, sym=None):
     print(f"[struct parity] n={len(seqs)} total={sum(map(len, seqs))} k={k} e={e} d={distance} "
-          f"boxes={boxes} sym={sym}", flush=True)
+          f"boxes={boxes} plan={sym}", flush=True)
     p = MotexParams(k=k, e=e, distance=distance, boxes=boxes)
     b = ctx.batch(seqs, p)
     st = b.run()
     u, v, off = b.download()
     b.close()
-    if sym is not None:
-        assert st["symmetric"] == int(sym and distance == 0)
+    if sym == "symmetric":
+        assert st["symmetric"] == int(distance == 0)
+    elif sym == "ordered":
+        assert st["symmetric"] == 0
+    elif sym == "range" and range_eligible(seqs, k, e, boxes, distance):
+        assert st["kernel_variant"] == 1 and st["symmetric"] == 0
     ou, ov, ooff, _ = oracle.all_pairs(seqs, distance, k, e, boxes=boxes)
     bad = np.nonzero((u != ou) | (v != ov))[0]
     assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: gpu {u[bad[:5]]},{v[bad[:5]]} oracle {ou[bad[:5]]},{ov[bad[:5]]}"
-    return u, v
+    return u, v, st
 
 
 def test_dna1000_structured_subset(ctx, oracle, sym):
     w = synth.dna1000_structured(scale=0.02)  # 20 sequences of 600 nt, one planted structured motif
-    u, v = check(ctx, oracle, w.seqs, w.k, w.e, list(w.boxes), sym=sym)
+    u, v, _ = check(ctx, oracle, w.seqs, w.k, w.e, list(w.boxes), sym=sym)
     assert v.sum() > 0
 
 
@@ -137,3 +157,57 @@ def test_boxes_longer_than_64(ctx, oracle, sym):
     seqs = rand_seqs(rng, 4, 300, 600, b"AC")
     u, v = check_struct(ctx, oracle, seqs, 0, 100, 30, [(5, 9, 80, 24)], "long boxes")
     assert v.sum() > 0
+
+
+def test_range_plan_is_the_default_for_hamming_boxes(ctx, monkeypatch):
+    monkeypatch.delenv("MOTEX_STRUCT_RANGE", raising=False)
+    w = synth.dna1000_structured(scale=0.01)
+    b = ctx.batch(w.seqs, MotexParams(k=w.k, e=w.e, boxes=list(w.boxes)))
+    st = b.run()
+    b.close()
+    assert st["kernel_variant"] == 1 and st["handed_pairs"] == 0
+
+
+@pytest.mark.parametrize("qcap", [64, 2304])
+def test_range_overflow_hands_pairs_to_the_pair_kernel(ctx, oracle, monkeypatch, qcap):
+    # hit lists too small for a range's hits (about 2400 per 12-text range here):
+    # those ranges' pairs go to the per-pair kernel through the overflow list, in the same run
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1")
+    monkeypatch.setenv("MOTEX_SR_HCAP", str(qcap))
+    w = synth.dna1000_structured(scale=0.02)
+    u, v, st = check(ctx, oracle, w.seqs, w.k, w.e, list(w.boxes), sym="range")
+    print(f"[struct] qcap {qcap}: {st['handed_pairs']} pairs handed over", flush=True)
+    if qcap == 64:
+        assert st["handed_pairs"] == len(w.seqs) ** 2
+    assert st["handed_pairs"] <= len(w.seqs) ** 2
+    assert v.sum() > 0
+
+
+def test_range_overflow_list_full_is_an_error(ctx, monkeypatch):
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1")
+    monkeypatch.setenv("MOTEX_SR_HCAP", "64")
+    monkeypatch.setenv("MOTEX_SR_OVF_CAP", "3")
+    w = synth.dna1000_structured(scale=0.02)
+    b = ctx.batch(w.seqs, MotexParams(k=w.k, e=w.e, boxes=list(w.boxes)))
+    with pytest.raises(MotexError, match="overflowed"):
+        b.run()
+    b.close()
+
+
+def test_range_plan_with_short_texts_between(ctx, oracle, monkeypatch):
+    # texts shorter than the whole structure end a range; x shorter than it has no rows
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1")
+    rng = np.random.default_rng(61)
+    seqs = rand_seqs(rng, 30, 5, 300, b"ACG")
+    u, v, st = check(ctx, oracle, seqs, 5, 1, [(3, 9, 5, 1)], sym="range")
+    assert st["kernel_variant"] == 1 and v.sum() > 0
+
+
+def test_range_plan_dense_rows(ctx, oracle, monkeypatch):
+    # 6-nt boxes with one error in DNA: about 38 hits per row of a range, beyond the
+    # one-gap merge's register slots, so those rows take the chain walk
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1")
+    rng = np.random.default_rng(62)
+    seqs = rand_seqs(rng, 40, 100, 250, b"ACGT")
+    u, v, st = check(ctx, oracle, seqs, 6, 1, [(3, 8, 6, 1)], sym="range")
+    assert st["kernel_variant"] == 1 and st["handed_pairs"] == 0 and v.sum() > 0
