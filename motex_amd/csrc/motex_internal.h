@@ -220,6 +220,9 @@ struct SrSweepArgs {
     StructItem *ovf;            // pairs handed to the per-pair kernel (a unit that overflowed)
     uint32_t *ovf_count;        // [0] pairs handed over, [1] set when the list was full
     uint32_t *self_ovf;         // per sequence: 1 if the unit holding its own pair overflowed
+    const uint32_t *uvb;        // per sequence: u/v index of (structure 0, position 0)
+    uint32_t sym;               // pair-symmetric units: texts before x are skipped, the pairs
+                                // (x, j > x) credited both ways
     uint32_t ovf_cap;
     uint32_t W, sigma, pad_bits;
     uint32_t wl;                // NE window words per symbol: 4 blocks of a row's longest span
@@ -235,6 +238,8 @@ struct SrMergeArgs {
     const uint32_t *seq_start;
     const uint32_t *S;          // nb_structs x nb_gaps gap lengths
     unsigned long long *uv;     // per u/v index: v in the low word, u in the high word
+    const uint32_t *uvb;        // per sequence: u/v index of (structure 0, position 0)
+    uint32_t sym;               // pair-symmetric units (one gap)
     uint32_t first;             // unit of slot 0 (one workgroup per slot)
     uint32_t m_max, hcap, ndist, nbk;  // nbk: position buckets of 16, (window + m_max + 15) / 16
     uint32_t nb_gaps, nb_structs;

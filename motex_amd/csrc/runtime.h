@@ -166,8 +166,9 @@ struct motex_batch {
     // per-pair kernel (whose count and full flag are words 2 and 3 of d_queue), the merge's
     // 64-bit u/v accumulators, the x list of the fold and per-sequence self-pair flags
     bool struct_range = false;
-    motex::DevBuf d_sr_items, d_sr_slots, d_sr_queue, d_ovf, d_sr_uv, d_sr_xs, d_sr_self;
+    motex::DevBuf d_sr_items, d_sr_slots, d_sr_queue, d_ovf, d_sr_uv, d_sr_xs, d_sr_self, d_sr_uvb;
     uint32_t sr_nx = 0;
+    bool sr_sym = false;
     uint32_t sr_nitems = 0, sr_chunk = 0, sr_nchunks = 0, sr_grid = 0, sr_hcap = 0, sr_ovf_cap = 0, sr_ovf_grid = 0;
     uint32_t sr_wl = 0, sr_nbk = 0, sr_m_max = 0;
     uint32_t ed_slice_stream_words = 0;

@@ -160,8 +160,8 @@ static int struct_range_prepare(motex_batch *b, uint32_t maxlen, bool *taken)
     for (uint32_t d = 0; d < t.ndist; ++d) planes = std::max(planes, hd_counter_planes(t.dlen[d], t.derr[d]));
     if (planes > 7 || 2ull * maxlen + 32 > kSrWindow) return MOTEX_OK;
     const uint32_t m_max = std::max<uint32_t>(maxlen, 1);
-    // the merge's u keys: (kx * m + px) * 32 + text in 32 bits
-    if ((uint64_t)t.nb_structs * m_max >= (1ull << 27)) return MOTEX_OK;
+    // the merge's u keys: (kx * m + px) * 32 + text below 2^31 (the text side's keys set bit 31)
+    if ((uint64_t)t.nb_structs * m_max >= (1ull << 26)) return MOTEX_OK;
     const uint32_t wl = sr_window_words(maxlen);
     const size_t sweep_lds = struct_range_sweep_lds(ss.sigma, wl, m_max);
     if (sweep_lds > 65536) return MOTEX_OK;
@@ -207,7 +207,12 @@ static int struct_range_prepare(motex_batch *b, uint32_t maxlen, bool *taken)
         text_syms += r.ye - r.ys;
         j = jj;
     }
-    // units: every x against every range, longest x first
+    // units: every x against every range (pair-symmetric, one gap: the ranges from the one
+    // holding x on, each unordered pair once), longest x first
+    const bool sym = G == 1 && struct_symmetric(b);
+    std::vector<uint32_t> range_of(ss.num, 0);
+    for (uint32_t r = 0; r < ranges.size(); ++r)
+        for (uint32_t j = ranges[r].j0; j < ranges[r].j0 + ranges[r].ny; ++j) range_of[j] = r;
     std::vector<uint32_t> xs;
     for (uint32_t i = 0; i < ss.num; ++i)
         if (ss.len[i] >= tot) xs.push_back(i);
@@ -215,7 +220,8 @@ static int struct_range_prepare(motex_batch *b, uint32_t maxlen, bool *taken)
     std::vector<SrItem> items;
     items.reserve(xs.size() * ranges.size());
     for (uint32_t i : xs)
-        for (uint32_t r = 0; r < ranges.size(); ++r) items.push_back({ss.start[i], ss.len[i], (uint32_t)ss.uv_off[i], r, i, 0});
+        for (uint32_t r = sym ? range_of[i] : 0; r < ranges.size(); ++r)
+            items.push_back({ss.start[i], ss.len[i], (uint32_t)ss.uv_off[i], r, i, 0});
     std::vector<SrX> xlist;
     for (uint32_t i : xs) xlist.push_back({i, ss.len[i], (uint32_t)ss.uv_off[i], 0});
     const uint64_t slot_bytes = (uint64_t)(kSrHead + hcap) * 4;
@@ -250,6 +256,16 @@ static int struct_range_prepare(motex_batch *b, uint32_t maxlen, bool *taken)
         (e = hipMemcpyAsync(b->d_sr_xs.p, xlist.data(), xlist.size() * sizeof(SrX), hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail(e, "copy x list");
     b->sr_nx = (uint32_t)xlist.size();
+    {
+        std::vector<uint32_t> uvb(ss.num);
+        for (uint32_t i = 0; i < ss.num; ++i) uvb[i] = (uint32_t)ss.uv_off[i];
+        if ((e = b->d_sr_uvb.alloc(std::max<size_t>(ss.num, 1) * 4)) != hipSuccess) return hip_fail(e, "alloc u/v bases");
+        if (ss.num && (e = hipMemcpyAsync(b->d_sr_uvb.p, uvb.data(), ss.num * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return hip_fail(e, "copy u/v bases");
+        // the host vectors of these copies end with this function
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "range plan upload");
+    }
+    b->sr_sym = sym;
     const uint64_t pairs = (uint64_t)xs.size() * texts;
     b->sr_ovf_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(pairs, kSrOvfCap));
     if (const char *o = getenv("MOTEX_SR_OVF_CAP")) b->sr_ovf_cap = std::max(1u, std::min(b->sr_ovf_cap, (uint32_t)atoi(o)));
@@ -268,7 +284,7 @@ static int struct_range_prepare(motex_batch *b, uint32_t maxlen, bool *taken)
     b->sr_ovf_grid = std::min<uint32_t>(b->ctx->num_cu * 4, b->sr_ovf_cap);
     b->tile.variant = 1;
     b->tile.waves_per_cu = wpc;
-    b->stats.symmetric = 0;
+    b->stats.symmetric = sym ? 1u : 0u;
     // cells: one DP per distinct box shape and ordered pair; swept: every unit's window
     double cells = 0, swept = 0;
     for (uint32_t d = 0; d < t.ndist; ++d) {
@@ -487,6 +503,8 @@ static int struct_range_launch(motex_batch *b)
     ma.S = t.S;
     ma.uv = (unsigned long long *)b->d_sr_uv.p;
     sa.self_ovf = (uint32_t *)b->d_sr_self.p;
+    sa.uvb = ma.uvb = (const uint32_t *)b->d_sr_uvb.p;
+    sa.sym = ma.sym = b->sr_sym ? 1u : 0u;
     ma.m_max = b->sr_m_max;
     ma.hcap = b->sr_hcap;
     ma.ndist = t.ndist;

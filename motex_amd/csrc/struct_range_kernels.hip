@@ -56,6 +56,7 @@ struct SrUnit {
     uint32_t m;             // rows of x
     uint32_t self;          // text of the range that is x itself, or kSrNoSelf
     int32_t diag;           // position - row on x's own diagonal (pt = x_start + px)
+    const uint32_t *tuvb;   // per text of the range: its u/v index of (structure 0, position 0)
 };
 
 // First hit of shape d in row px or later.
@@ -175,9 +176,32 @@ __device__ void sr_walk(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, u
     }
 }
 
+// Whether the chain (px0, r0) -> (px1, r1) is the first, in rows, of the transposed pair's
+// (ky, r0): no occurrence of box 0 at r0 in an earlier row has a box-1 hit at r1 in its row
+// window, and no box-1 hit at r1 lies in an earlier row of px0's window.  (The key set's
+// fallback for the text side.)
+__device__ bool sr_first_chain_t(const SrMergeArgs &a, const SrUnit &U, uint32_t px0, int32_t r0, uint32_t px1,
+                                 int32_t r1)
+{
+    const uint32_t lo_g = a.blen[1] + a.gmin[0], hi_g = a.blen[1] + a.gmax[0];
+    bool first = true;
+    sr_scan(U, a.dist_of[1], r1, r1, [&](uint32_t p, int32_t) {
+        if (p >= px0 + lo_g && p < px1) first = false;
+    });
+    sr_scan(U, a.dist_of[0], r0, r0, [&](uint32_t q, int32_t) {
+        if (!first || q >= px0 || q + 1 < a.blen[0]) return;
+        sr_scan(U, a.dist_of[1], r1, r1, [&](uint32_t p, int32_t) {
+            if (p >= q + lo_g && p <= q + hi_g && p < U.m) first = false;
+        });
+    });
+    return first;
+}
+
 // Occurrence of box 0, one gap: box 1 ends in row px1 = px0 + len1 + gmin + kx at a position
 // in r0 + [gmin + len1, gmax + len1], inside the text.  The buckets of that window give every
-// such hit, and each is one chain of kx (but the one on x's own diagonal).
+// such hit, and each is one chain of kx (but the one on x's own diagonal).  With pair-symmetric
+// units a chain in a text j after x is also one of the transposed pair (j, x): box 0 at j's
+// position r0 - ts, ky = r1 - r0 - len1 - gmin; u of j once per (ky, r0) in the unit.
 __device__ void sr_join(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, uint32_t uv_base, uint32_t px0,
                         int32_t r0, uint32_t ti, int32_t ts, int32_t te)
 {
@@ -187,10 +211,19 @@ __device__ void sr_join(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, u
     const int32_t lo = r0 + (int32_t)(gmn + len1), hi = min(r0 + (int32_t)(a.gmax[0] + len1), te - 1);
     if (lo > hi) return;
     const bool diag = ti == U.self && r0 - (int32_t)px0 == U.diag;
+    const bool both = a.sym && ti != U.self;
     sr_scan(U, a.dist_of[1], lo, hi, [&](uint32_t px1, int32_t r1) {
         const uint32_t kx = px1 - px0 - len1 - gmn;
         if (kx > span || (diag && r1 - (int32_t)px1 == U.diag)) return;
         sr_credit(a, U, keys, uv_base, kx, px0, r0, ti, ts, te, 1, px1, r1);
+        if (both) {
+            const uint32_t ky = (uint32_t)(r1 - lo);
+            const uint32_t n = (uint32_t)(te - ts);
+            const uint32_t k = sr_key_insert(keys, 0x80000000u | (ky << 14) | (uint32_t)r0);
+            const unsigned long long add =
+                1ull + ((k == 1 || (k == 2 && sr_first_chain_t(a, U, px0, r0, px1, r1))) ? 1ull << 32 : 0ull);
+            atomicAdd(a.uv + (size_t)U.tuvb[ti] + (size_t)ky * n + (uint32_t)(r0 - ts), add);
+        }
     });
 }
 
@@ -352,20 +385,24 @@ __global__ __launch_bounds__(64) void struct_range_sweep(SrSweepArgs a)
         if (lane == 0) slot[0] = overflow ? 1u : 0u;
         if (overflow) {
             // hand the unit's pairs to the per-pair kernel
+            // pair-symmetric units hand over the pairs with x and the texts after it
+            const uint32_t j_first = a.sym && I.xi > HR.j0 ? I.xi : HR.j0;
+            const uint32_t nhand = HR.j0 + HR.ny - j_first;
             if (lane == 0) {
-                const uint32_t at = atomicAdd(a.ovf_count, HR.ny);
-                if (at + HR.ny > a.ovf_cap) atomicOr(a.ovf_count + 1, 1u);
+                const uint32_t at = atomicAdd(a.ovf_count, nhand);
+                if (at + nhand > a.ovf_cap) atomicOr(a.ovf_count + 1, 1u);
                 bcast[0] = at;
                 // x against itself goes with them: the fold must not credit its diagonal
                 if (I.xi >= HR.j0 && I.xi < HR.j0 + HR.ny) a.self_ovf[I.xi] = 1;
             }
             __syncthreads();
             const uint32_t at = bcast[0];
-            for (uint32_t y = lane; y < HR.ny; y += 64) {
+            for (uint32_t y = lane; y < nhand; y += 64) {
                 if (at + y >= a.ovf_cap) break;
-                const uint32_t j = HR.j0 + y;
+                const uint32_t j = j_first + y;
                 const uint32_t ts = a.seq_start[j], n = a.seq_start[j + 1] - ts;
-                a.ovf[at + y] = StructItem{I.x_start, m, ts, n, I.uv_base, kNoSplit};
+                // pair-symmetric units: the pair (j, x) goes along
+                a.ovf[at + y] = StructItem{I.x_start, m, ts, n, I.uv_base, a.sym && j != I.xi ? a.uvb[j] : kNoSplit};
             }
         }
     }
@@ -385,16 +422,19 @@ __global__ __launch_bounds__(kSrMergeThreads) void struct_range_merge(SrMergeArg
     const HdRange HR = a.ranges[I.r];
     const uint32_t m = I.m;
     const int32_t q0 = sr_window_q0((int32_t)HR.ys, m);
-    // LDS: text starts (kMaxRangeSeqs + 1, as positions pt - q0) | shape ends (kSrMaxBoxes) |
-    //      u key set (2^kSrKeyBits) | bucket ends (ndist x nbk) | bucket entries (u16, hcap)
+    // LDS: text starts (kMaxRangeSeqs + 1, as positions pt - q0) | texts' u/v bases
+    //      (kMaxRangeSeqs) | shape ends (kSrMaxBoxes) | u key set (2^kSrKeyBits) | bucket ends
+    //      (ndist x nbk) | bucket entries (u16, hcap)
     int32_t *starts = reinterpret_cast<int32_t *>(smem);
-    uint32_t *ends = smem + kMaxRangeSeqs + 1;
+    uint32_t *tuvb = smem + kMaxRangeSeqs + 1;
+    uint32_t *ends = tuvb + kMaxRangeSeqs;
     uint32_t *keys = ends + kSrMaxBoxes;
     uint32_t *C = keys + (1u << kSrKeyBits);
     const uint32_t nbc = a.ndist * a.nbk;
     uint16_t *bq = reinterpret_cast<uint16_t *>(C + nbc);
     const uint32_t *hits = slot + kSrHead;
     if (tid <= HR.ny) starts[tid] = (int32_t)a.seq_start[HR.j0 + tid] - q0;
+    if (tid < HR.ny) tuvb[tid] = a.uvb[HR.j0 + tid];
     if (tid < a.ndist) ends[tid] = slot[1 + tid];
     for (uint32_t b = tid; b < nbc; b += kSrMergeThreads) C[b] = 0;
     for (uint32_t k = tid; k < (1u << kSrKeyBits); k += kSrMergeThreads) keys[k] = kSrNoKey;
@@ -437,7 +477,10 @@ __global__ __launch_bounds__(kSrMergeThreads) void struct_range_merge(SrMergeArg
 
     // occurrences of box 0
     const uint32_t self = I.xi >= HR.j0 && I.xi < HR.j0 + HR.ny ? I.xi - HR.j0 : kSrNoSelf;
-    const SrUnit U{hits, ends, bq, C, a.nbk, m, self, (int32_t)I.x_start - q0};
+    const SrUnit U{hits, ends, bq, C, a.nbk, m, self, (int32_t)I.x_start - q0, tuvb};
+    // pair-symmetric units: the texts before x are x's pairs with earlier sequences, whose
+    // units hold them
+    const uint32_t t_first = a.sym && I.xi > HR.j0 ? I.xi - HR.j0 : 0u;
     const int32_t ys = starts[0], ye = starts[HR.ny];
     const uint32_t len0 = a.blen[0];
     const uint32_t d0 = a.dist_of[0];
@@ -448,7 +491,7 @@ __global__ __launch_bounds__(kSrMergeThreads) void struct_range_merge(SrMergeArg
         if (r0 < ys || r0 >= ye) continue;
         const uint32_t ti = sr_text_of(starts, HR.ny, r0);
         const int32_t ts = starts[ti], te = starts[ti + 1];
-        if (r0 - (int32_t)len0 + 1 < ts) continue;  // box 0's window inside the text
+        if (ti < t_first || r0 - (int32_t)len0 + 1 < ts) continue;  // box 0's window inside the text
         if (a.nb_gaps == 1)
             sr_join(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
         else
@@ -520,7 +563,8 @@ size_t struct_range_sweep_lds(uint32_t sigma, uint32_t wl, uint32_t m_max)
 
 size_t struct_range_merge_lds(uint32_t hcap, uint32_t ndist, uint32_t nbk)
 {
-    return ((size_t)kMaxRangeSeqs + 1 + kSrMaxBoxes + (1u << kSrKeyBits) + (size_t)ndist * nbk + (hcap + 1) / 2) * 4;
+    return ((size_t)2 * kMaxRangeSeqs + 1 + kSrMaxBoxes + (1u << kSrKeyBits) + (size_t)ndist * nbk + (hcap + 1) / 2) *
+           4;
 }
 
 }  // namespace motex

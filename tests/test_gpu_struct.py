@@ -12,13 +12,14 @@ def rand_seqs(rng, n, lo, hi, alpha):
     return [a[rng.integers(0, len(a), rng.integers(lo, hi + 1))].tobytes() for _ in range(n)]
 
 
-@pytest.fixture(params=["range", "symmetric", "ordered"])
+@pytest.fixture(params=["range", "range-ordered", "symmetric", "ordered"])
 def sym(request, monkeypatch):
     """Runs a test with each structured plan: range-batched (Hamming boxes: every row of x
-    against packed text ranges, struct_range_kernels.hip), and one pair per work item with
-    the pair-symmetric plan (each unordered pair's box DPs serve both orders) and without."""
-    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1" if request.param == "range" else "0")
-    monkeypatch.setenv("MOTEX_SYM", "0" if request.param == "ordered" else "1")
+    against packed text ranges, struct_range_kernels.hip; with one gap each unordered pair
+    once, credited both ways) and range-batched over ordered pairs, and one pair per work item
+    with the pair-symmetric plan (each unordered pair's box DPs serve both orders) and without."""
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1" if request.param.startswith("range") else "0")
+    monkeypatch.setenv("MOTEX_SYM", "0" if request.param in ("ordered", "range-ordered") else "1")
     return request.param
 
 
@@ -49,6 +50,8 @@ def check(ctx, oracle, seqs, k, e, boxes, distance=0, sym=None):
     elif sym == "ordered":
         assert st["symmetric"] == 0
     elif sym == "range" and range_eligible(seqs, k, e, boxes, distance):
+        assert st["kernel_variant"] == 1 and st["symmetric"] == int(len(boxes) == 1)
+    elif sym == "range-ordered" and range_eligible(seqs, k, e, boxes, distance):
         assert st["kernel_variant"] == 1 and st["symmetric"] == 0
     ou, ov, ooff, _ = oracle.all_pairs(seqs, distance, k, e, boxes=boxes)
     bad = np.nonzero((u != ou) | (v != ov))[0]
@@ -177,9 +180,10 @@ def test_range_overflow_hands_pairs_to_the_pair_kernel(ctx, oracle, monkeypatch,
     w = synth.dna1000_structured(scale=0.02)
     u, v, st = check(ctx, oracle, w.seqs, w.k, w.e, list(w.boxes), sym="range")
     print(f"[struct] qcap {qcap}: {st['handed_pairs']} pairs handed over", flush=True)
-    if qcap == 64:
-        assert st["handed_pairs"] == len(w.seqs) ** 2
-    assert st["handed_pairs"] <= len(w.seqs) ** 2
+    n = len(w.seqs)
+    if qcap == 64:  # every unit: each unordered pair once (pair-symmetric units, one gap)
+        assert st["handed_pairs"] == n * (n + 1) // 2
+    assert st["handed_pairs"] <= n * (n + 1) // 2
     assert v.sum() > 0
 
 
@@ -201,6 +205,18 @@ def test_range_plan_with_short_texts_between(ctx, oracle, monkeypatch):
     seqs = rand_seqs(rng, 30, 5, 300, b"ACG")
     u, v, st = check(ctx, oracle, seqs, 5, 1, [(3, 9, 5, 1)], sym="range")
     assert st["kernel_variant"] == 1 and v.sum() > 0
+
+
+@pytest.mark.parametrize("symmetric", ["1", "0"])
+def test_range_overflow_ordered_and_symmetric(ctx, oracle, monkeypatch, symmetric):
+    # some units overflow (hit slots of 2304 against about 2400 hits), both unit plans
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1")
+    monkeypatch.setenv("MOTEX_SYM", symmetric)
+    monkeypatch.setenv("MOTEX_SR_HCAP", "2304")
+    w = synth.dna1000_structured(scale=0.03)
+    u, v, st = check(ctx, oracle, w.seqs, w.k, w.e, list(w.boxes), sym="range" if symmetric == "1" else "range-ordered")
+    print(f"[struct] symmetric {symmetric}: {st['handed_pairs']} pairs handed over", flush=True)
+    assert v.sum() > 0
 
 
 def test_range_plan_dense_rows(ctx, oracle, monkeypatch):
