@@ -116,7 +116,7 @@ static const uint64_t kSrSlotBytes = 2ull << 30;      // unit slots of one launc
 // Waves per CU the sweep kernel's registers allow, by counter planes.
 static uint32_t sr_waves_vgpr(uint32_t planes)
 {
-    return planes <= 4 ? 32 : planes == 5 ? 28 : planes == 6 ? 24 : 20;
+    return planes <= 3 ? 32 : planes == 4 ? 28 : planes == 5 ? 24 : 20;
 }
 
 // NE window words per symbol of the sweep: a row reads words (px >> 5) + 64 b + lane and the

@@ -262,6 +262,99 @@ __device__ bool sr_first_chain(const SrMergeArgs &a, const SrUnit &U, uint32_t k
 // Sweep
 // ---------------------------------------------------------------------------------------
 
+// Appends row px's hits (the clear bits of high) to the slot; false when they do not fit.
+__device__ __forceinline__ bool sr_emit(const uint32_t *high, uint32_t px, uint32_t lane, uint32_t *hits,
+                                        uint32_t hcap, uint32_t &nq)
+{
+    const uint32_t w0 = ~high[0], w1 = ~high[1], w2 = ~high[2], w3 = ~high[3];
+    const uint32_t any = w0 | w1 | w2 | w3;
+    const uint64_t mb = __ballot(any != 0);
+    if (!mb) return true;
+    const uint32_t nh = __builtin_popcount(w0) + __builtin_popcount(w1) + __builtin_popcount(w2) + __builtin_popcount(w3);
+    uint64_t bt = __ballot(nh > 1);
+    if (!bt) {
+        // at most one hit a lane (the common case): slot = lanes with a hit below
+        const uint32_t tot = (uint32_t)__builtin_popcountll(mb);
+        if (nq + tot > hcap) return false;
+        if (any) {
+            const uint32_t b = ((w1 | w3) ? 1u : 0u) | ((w2 | w3) ? 2u : 0u);
+            const uint32_t at = nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
+            hits[at] = (px << 16) | (kDiagPerBlock * b + 32 * lane + __builtin_ctz(any));
+        }
+        nq += tot;
+        return true;
+    }
+    // some lane has several: slots from the wave's exclusive prefix of the counts, one ballot
+    // per count level
+    uint32_t pre = 0, tot = 0;
+    bt = mb;
+    for (uint32_t t = 1; bt; ++t) {
+        pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(bt >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bt, pre));
+        tot += (uint32_t)__builtin_popcountll(bt);
+        bt = __ballot(nh > t);
+    }
+    if (nq + tot > hcap) return false;
+    if (nh) {
+        uint32_t at = nq + pre;
+        const uint32_t tag = (px << 16) | (32 * lane);
+#pragma unroll
+        for (int b = 0; b < (int)kSrBlocks; ++b) {
+            uint32_t hv = ~high[b];
+            while (hv) {
+                hits[at++] = tag + kDiagPerBlock * b + __builtin_ctz(hv);
+                hv &= hv - 1;
+            }
+        }
+    }
+    nq += tot;
+    return true;
+}
+
+// OR of the planes at and above the threshold plane PH (a clear bit: the window is within the
+// error bound).  PH >= P only when the bound is at least the box length: every window passes.
+template <int P, int PH>
+__device__ __forceinline__ uint32_t sr_high(const uint32_t *c)
+{
+    if constexpr (PH >= P) return 0u;
+    else return planes_high<P, PH>(c);
+}
+
+// One shape's sweep down the rows of x: P counter planes, threshold plane PH.  The first len
+// rows fill the windows (no leaving symbol); returns true when the hits overflowed the slot.
+template <int P, int PH>
+__device__ bool sr_sweep_shape(const char *ne_b, const uint16_t *xoff, uint32_t m, uint32_t len, uint32_t err,
+                               uint32_t *hits, uint32_t hcap, uint32_t lane, uint32_t &nq)
+{
+    uint32_t cnt[kSrBlocks][P];
+#pragma unroll
+    for (int b = 0; b < (int)kSrBlocks; ++b) planes_set<P>(cnt[b], hd_bias(err));
+    uint32_t high[kSrBlocks];
+    const uint32_t warm = min(len, m);
+    for (uint32_t px = 0; px < warm; ++px) {
+        const uint32_t *wn = reinterpret_cast<const uint32_t *>(ne_b + xoff[px]);
+#pragma unroll
+        for (int b = 0; b < (int)kSrBlocks; ++b) {
+            planes_inc<P>(cnt[b], __builtin_amdgcn_alignbit(wn[64 * b + 1], wn[64 * b], px & 31));
+            high[b] = sr_high<P, PH>(cnt[b]);
+        }
+    }
+    if (len <= m && !sr_emit(high, len - 1, lane, hits, hcap, nq)) return true;
+    for (uint32_t px = len; px < m; ++px) {
+        const uint32_t *wn = reinterpret_cast<const uint32_t *>(ne_b + xoff[px]);
+        const uint32_t *wo = reinterpret_cast<const uint32_t *>(ne_b + xoff[px - len]);
+        const uint32_t sho = (px - len) & 31;
+#pragma unroll
+        for (int b = 0; b < (int)kSrBlocks; ++b) {
+            const uint32_t an = __builtin_amdgcn_alignbit(wn[64 * b + 1], wn[64 * b], px & 31);
+            const uint32_t bo = __builtin_amdgcn_alignbit(wo[64 * b + 1], wo[64 * b], sho);
+            planes_slide<P>(cnt[b], an, bo);
+            high[b] = sr_high<P, PH>(cnt[b]);
+        }
+        if (!sr_emit(high, px, lane, hits, hcap, nq)) return true;
+    }
+    return false;
+}
+
 template <int P>
 __global__ __launch_bounds__(64) void struct_range_sweep(SrSweepArgs a)
 {
@@ -302,83 +395,16 @@ __global__ __launch_bounds__(64) void struct_range_sweep(SrSweepArgs a)
         const char *ne_b = reinterpret_cast<const char *>(ne_l) + 4 * lane;
         for (uint32_t d = 0; d < a.ndist && !overflow; ++d) {
             const uint32_t len = a.dlen[d], err = a.derr[d];
-            uint32_t hm[P];
-            planes_high_masks<P>(hm, hd_low_planes(err));
-            uint32_t cnt[kSrBlocks][P];
-#pragma unroll
-            for (int b = 0; b < (int)kSrBlocks; ++b) planes_set<P>(cnt[b], hd_bias(err));
-            for (uint32_t px = 0; px < m; ++px) {
-                const uint32_t *wn = reinterpret_cast<const uint32_t *>(ne_b + xoff[px]);
-                uint32_t high[kSrBlocks];
-                if (px >= len) {
-                    const uint32_t *wo = reinterpret_cast<const uint32_t *>(ne_b + xoff[px - len]);
-                    const uint32_t sho = (px - len) & 31;
-#pragma unroll
-                    for (int b = 0; b < (int)kSrBlocks; ++b) {
-                        const uint32_t an = __builtin_amdgcn_alignbit(wn[64 * b + 1], wn[64 * b], px & 31);
-                        const uint32_t bo = __builtin_amdgcn_alignbit(wo[64 * b + 1], wo[64 * b], sho);
-                        planes_slide<P>(cnt[b], an, bo);
-                        high[b] = planes_high_masked<P>(cnt[b], hm);
-                    }
-                } else {
-#pragma unroll
-                    for (int b = 0; b < (int)kSrBlocks; ++b) {
-                        const uint32_t an = __builtin_amdgcn_alignbit(wn[64 * b + 1], wn[64 * b], px & 31);
-                        planes_inc<P>(cnt[b], an);
-                        high[b] = planes_high_masked<P>(cnt[b], hm);
-                    }
-                }
-                if (px + 1 < len) continue;  // no full window in x yet
-                // hits: the clear bits of high, in any block
-                const uint32_t w0 = ~high[0], w1 = ~high[1], w2 = ~high[2], w3 = ~high[3];
-                const uint32_t any = w0 | w1 | w2 | w3;
-                const uint64_t mb = __ballot(any != 0);
-                if (!mb) continue;
-                const uint32_t nh = __builtin_popcount(w0) + __builtin_popcount(w1) + __builtin_popcount(w2) +
-                                    __builtin_popcount(w3);
-                uint64_t bt = __ballot(nh > 1);
-                if (!bt) {
-                    // at most one hit a lane (the common case): slot = lanes with a hit below
-                    const uint32_t tot = (uint32_t)__builtin_popcountll(mb);
-                    if (nq + tot > a.hcap) {
-                        overflow = true;
-                        break;
-                    }
-                    if (any) {
-                        const uint32_t b = ((w1 | w3) ? 1u : 0u) | ((w2 | w3) ? 2u : 0u);
-                        const uint32_t at =
-                            nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
-                        hits[at] = (px << 16) | (kDiagPerBlock * b + 32 * lane + __builtin_ctz(any));
-                    }
-                    nq += tot;
-                    continue;
-                }
-                // some lane has several: slots from the wave's exclusive prefix of the counts,
-                // one ballot per count level
-                uint32_t pre = 0, tot = 0;
-                bt = mb;
-                for (uint32_t t = 1; bt; ++t) {
-                    pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(bt >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bt, pre));
-                    tot += (uint32_t)__builtin_popcountll(bt);
-                    bt = __ballot(nh > t);
-                }
-                if (nq + tot > a.hcap) {
-                    overflow = true;
-                    break;
-                }
-                if (nh) {
-                    uint32_t at = nq + pre;
-                    const uint32_t tag = (px << 16) | (32 * lane);
-#pragma unroll
-                    for (int b = 0; b < (int)kSrBlocks; ++b) {
-                        uint32_t hv = ~high[b];
-                        while (hv) {
-                            hits[at++] = tag + kDiagPerBlock * b + __builtin_ctz(hv);
-                            hv &= hv - 1;
-                        }
-                    }
-                }
-                nq += tot;
+            // the threshold plane is fixed per shape: one loop nest per (P, PH)
+            switch (min(hd_low_planes(err), (uint32_t)P)) {
+            case 0: overflow = sr_sweep_shape<P, 0>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
+            case 1: overflow = sr_sweep_shape<P, 1>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
+            case 2: overflow = sr_sweep_shape<P, 2>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
+            case 3: overflow = sr_sweep_shape<P, 3>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
+            case 4: overflow = sr_sweep_shape<P, (P > 4 ? 4 : P)>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
+            case 5: overflow = sr_sweep_shape<P, (P > 5 ? 5 : P)>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
+            case 6: overflow = sr_sweep_shape<P, (P > 6 ? 6 : P)>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
+            default: overflow = sr_sweep_shape<P, P>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
             }
             if (lane == 0) slot[1 + d] = nq;
         }
