@@ -240,7 +240,7 @@ def main():
     kms = float(np.mean(kernel_ms))
     ops = OPS_PER_CELL[w.distance]
     algo = cells * ops / (kms / 1e3) / 1e12
-    kernel = (("struct_range_kernel" if st["kernel_variant"] == 1 else "struct_kernel") if w.boxes else
+    kernel = (("struct_range" if st["kernel_variant"] == 1 else "struct_kernel") if w.boxes else
               "hd_batched_kernel" if w.distance == 0 else
               {2: "ed_slice_kernel", 3: "ed_slice_pipe_kernel"}.get(st["kernel_variant"], "ed_kernel"))
     tr = traffic_from_profile(kernel, a.workload) if a.scale == 1.0 else None
