@@ -176,6 +176,63 @@ def merge_reference_definition(x, y, k, e, boxes, box_ok):
     return u, v
 
 
+def merge_reference_definition_gaps(x, y, boxes, box_ok, S):
+    """u, v of one pair from per-box flag matrices, any number of gaps: for x-side
+    combination kx and y-side combination ky, box 0 ending at (I, J) (1-based) makes a chain
+    when box b ends at I + sum(gy_c + len_c), J + sum(gx_c + len_c) for every b."""
+    m, n = len(x), len(y)
+    I, J = np.nonzero(box_ok[0])
+    u = np.zeros(len(S) * m, np.int64)
+    v = np.zeros(len(S) * m, np.int64)
+    for kx, gx in enumerate(S):
+        tot = np.zeros(m + 1, np.int64)
+        for gy in S:
+            ii, jj = I.copy(), J.copy()
+            good = np.ones(len(I), bool)
+            for b, (_, _, bl, _) in enumerate(boxes):
+                ii = ii + gy[b] + bl
+                jj = jj + gx[b] + bl
+                inb = (ii <= n) & (jj <= m)
+                good &= inb
+                idx = np.nonzero(good)[0]
+                good[idx] = box_ok[b + 1][ii[idx], jj[idx]]
+            np.add.at(tot, J[good], 1)
+        v[kx * m: (kx + 1) * m] += tot[1:]
+        u[kx * m: (kx + 1) * m] += tot[1:] > 0
+    return u, v
+
+
+@pytest.mark.parametrize("distance,boxes", [
+    (0, [(2, 4, 3, 1), (0, 2, 4, 1)]),
+    (0, [(1, 2, 3, 0), (2, 3, 3, 1), (0, 1, 4, 1)]),
+    (1, [(1, 3, 3, 1), (0, 2, 4, 1)]),
+])
+def test_structured_merge_definition_several_gaps(oracle, distance, boxes):
+    # the odometer walk over several gaps against the definition: every x-side and y-side
+    # gap combination tried on the box flag matrices (Hamming flags from brute-force windows)
+    rng = np.random.default_rng(41 + len(boxes) + distance)
+    x, y = rand_seqs(rng, 2, 90, 140, b"AC")
+    k, e = 4, 1
+    mats = []
+    for ell, err in [(k, e)] + [(b[2], b[3]) for b in boxes]:
+        ok = np.zeros((len(y) + 1, len(x) + 1), bool)
+        if distance == 0:
+            ok[ell:, ell:] = (brute_hd_matrix(x, y, ell) <= err).T
+        else:
+            for t, p, _ in oracle.hits(distance, x, y, ell, err):
+                ok[t + 1, p + 1] = True
+        mats.append(ok)
+    S = oracle.recurse([b[0] for b in boxes], [b[1] for b in boxes]).tolist()
+    u, v = oracle.struct_pair(distance, x, y, k, e, boxes)
+    bu, bv = merge_reference_definition_gaps(x, y, boxes, mats, S)
+    assert np.array_equal(u.astype(np.int64), bu)
+    assert np.array_equal(v.astype(np.int64), bv)
+    assert v.sum() > 0
+    # a dropped constraint on the last box changes the counts
+    loose = mats[:-1] + [np.ones_like(mats[-1])]
+    assert not np.array_equal(merge_reference_definition_gaps(x, y, boxes, loose, S)[1], bv)
+
+
 @pytest.mark.parametrize("distance", [0, 1])
 def test_structured_merge_definition(oracle, distance):
     rng = np.random.default_rng(31 + distance)
