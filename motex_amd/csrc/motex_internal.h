@@ -245,6 +245,7 @@ struct SrMergeArgs {
     uint32_t nb_gaps, nb_structs;
     uint32_t blen[kSrMaxBoxes], dist_of[kSrMaxBoxes];
     uint32_t gmin[kSrMaxGaps], gmax[kSrMaxGaps];
+    uint32_t gstride[kSrMaxGaps];  // combinations per step of gap g (last gap fastest)
 };
 
 }  // namespace motex

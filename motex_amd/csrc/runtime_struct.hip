@@ -225,8 +225,10 @@ static int struct_range_prepare(motex_batch *b, uint32_t maxlen, bool *taken)
     std::vector<SrX> xlist;
     for (uint32_t i : xs) xlist.push_back({i, ss.len[i], (uint32_t)ss.uv_off[i], 0});
     const uint64_t slot_bytes = (uint64_t)(kSrHead + hcap) * 4;
+    uint64_t slot_budget = kSrSlotBytes;
+    if (const char *sb = getenv("MOTEX_SR_SLOT_BYTES")) slot_budget = std::max<uint64_t>(1, strtoull(sb, nullptr, 10));
     const uint32_t chunk =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(items.size(), kSrSlotBytes / slot_bytes));
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(items.size(), slot_budget / slot_bytes));
     const uint32_t nchunks = items.empty() ? 0 : (uint32_t)((items.size() + chunk - 1) / chunk);
 
     uint32_t wpc = (uint32_t)std::max<size_t>(1, std::min<size_t>(sr_waves_vgpr(planes), (160 * 1024) / sweep_lds));
@@ -520,6 +522,10 @@ static int struct_range_launch(motex_batch *b)
     for (uint32_t g = 0; g < kSrMaxGaps; ++g) {
         ma.gmin[g] = t.gmin[g];
         ma.gmax[g] = t.gmax[g];
+    }
+    for (uint32_t g = t.nb_gaps, st = 1; g-- > 0;) {
+        ma.gstride[g] = st;
+        st *= t.gmax[g] - t.gmin[g] + 1;
     }
     const size_t sweep_lds = struct_range_sweep_lds(t.sigma, sa.wl, sa.m_max);
     const size_t merge_lds = struct_range_merge_lds(ma.hcap, ma.ndist, ma.nbk);

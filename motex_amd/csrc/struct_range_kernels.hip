@@ -136,9 +136,9 @@ __device__ __forceinline__ uint32_t sr_key_insert(uint32_t *keys, uint32_t key)
     return 2;
 }
 
-// Whether the chain of occurrence (px0, r0) for kx is the least of its (kx, px0, text): no
-// occurrence of row px0 in the text with a smaller position has a chain for kx and, with one
-// gap, no hit of row px1 before r1 completes this one.  (The key set's fallback.)
+// One gap: whether the chain (px0, r0) -> (px1, r1) is the least of its (kx, px0, text): no hit of
+// row px1 before r1 completes the occurrence, and no occurrence of row px0 in the text with a
+// smaller position has a chain for kx.  (The key set's fallback.)
 __device__ bool sr_first_chain(const SrMergeArgs &a, const SrUnit &U, uint32_t kx, uint32_t px0, int32_t r0,
                                int32_t ts, int32_t te, uint32_t px1, int32_t r1);
 
@@ -162,18 +162,73 @@ __device__ __forceinline__ void sr_credit(const SrMergeArgs &a, const SrUnit &U,
     if (add) atomicAdd(a.uv + (size_t)uv_base + (size_t)kx * U.m + px0, add);
 }
 
-// Occurrence of box 0 at row px0, position r0, in text ti = [ts, te) (positions), any number of
-// gaps: its chains for every gap combination (on x's own diagonal, less the all-diagonal one).
+// Every chain of boxes B..nb_gaps from box B-1 ending at (pxp, rp), inside a text ending before
+// rend: the box-B hits of the position window rp + [gmin + len_B, gmax + len_B], from the
+// buckets, whose rows lie in pxp + [gmin + len_B, gmax + len_B]; the row names box B's x-side
+// gap, and the gaps build the combination index kx (last gap fastest).  f(kx) for each complete
+// chain but the one on x's own diagonal (dg: the chain so far lies on it).
+template <int B, class F>
+__device__ void sr_enum(const SrMergeArgs &a, const SrUnit &U, uint32_t pxp, int32_t rp, int32_t rend, uint32_t kxa,
+                        bool dg, F &f)
+{
+    const uint32_t span = a.gmax[B - 1] - a.gmin[B - 1];
+    const uint32_t glo = pxp + a.gmin[B - 1] + a.blen[B];
+    const int32_t lo = rp + (int32_t)(a.gmin[B - 1] + a.blen[B]);
+    const int32_t hi = min(rp + (int32_t)(a.gmax[B - 1] + a.blen[B]), rend - 1);
+    if (lo > hi || glo >= U.m) return;
+    sr_scan(U, a.dist_of[B], lo, hi, [&](uint32_t px, int32_t r) {
+        const uint32_t g = px - glo;
+        if (g > span) return;
+        const uint32_t kx = kxa + g * a.gstride[B - 1];
+        const bool d = dg && r - (int32_t)px == U.diag;
+        if (B == (int)a.nb_gaps) {
+            if (!d) f(kx);
+        } else if constexpr (B < (int)kSrMaxGaps) {
+            sr_enum<B + 1>(a, U, px, r, rend, kx, d, f);
+        }
+    });
+}
+
+// Whether the t-th chain of occurrence (px0, r0) (enumeration order), of combination kx, is the
+// first of its (kx, px0, text): no earlier chain of this occurrence has kx, and no occurrence of
+// row px0 in the text with a smaller position has a chain for kx.  (The key set's fallback,
+// several gaps.)
+__device__ bool sr_first_chain_g(const SrMergeArgs &a, const SrUnit &U, uint32_t kx, uint32_t px0, int32_t r0,
+                                 int32_t ts, int32_t te, uint32_t t)
+{
+    uint32_t seen = 0, before = 0;
+    auto count = [&](uint32_t k) {
+        if (seen++ < t && k == kx) ++before;
+    };
+    sr_enum<1>(a, U, px0, r0, te, 0u, false, count);
+    if (before) return false;
+    const uint32_t d0 = a.dist_of[0];
+    const int32_t from = ts + (int32_t)a.blen[0] - 1;
+    for (uint32_t h2 = sr_row_lower(U, d0, px0); h2 < U.ends[d0] && (U.hits[h2] >> 16) == px0; ++h2) {
+        const int32_t r2 = (int32_t)(U.hits[h2] & 0xFFFFu) + (int32_t)px0;
+        if (r2 < from || r2 >= r0) continue;
+        if (sr_chains<1>(a, U, kx, px0, r2, te)) return false;
+    }
+    return true;
+}
+
+// Occurrence of box 0 at row px0, position r0, in text ti = [ts, te) (positions), several gaps:
+// each chain is one v of its combination; u once per (kx, px0, text) through the key set.
 __device__ void sr_walk(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, uint32_t uv_base, uint32_t px0,
                         int32_t r0, uint32_t ti, int32_t ts, int32_t te)
 {
     const bool diag = ti == U.self && r0 - (int32_t)px0 == U.diag;
-    for (uint32_t kx = 0; kx < a.nb_structs; ++kx) {
-        uint32_t c = sr_chains<1>(a, U, kx, px0, r0, te);
-        if (!c) continue;
-        c -= diag ? 1u : 0u;
-        sr_credit(a, U, keys, uv_base, kx, px0, r0, ti, ts, te, c, 0, INT32_MIN);
-    }
+    uint32_t ord = 0;
+    auto credit = [&](uint32_t kx) {
+        const uint32_t t = ord++;
+        unsigned long long add = 1;
+        if (ti != U.self) {
+            const uint32_t k = sr_key_insert(keys, ((kx * U.m + px0) << 5) | ti);
+            if (k == 1 || (k == 2 && sr_first_chain_g(a, U, kx, px0, r0, ts, te, t))) add += 1ull << 32;
+        }
+        atomicAdd(a.uv + (size_t)uv_base + (size_t)kx * U.m + px0, add);
+    };
+    sr_enum<1>(a, U, px0, r0, te, 0u, diag, credit);
 }
 
 // Whether the chain (px0, r0) -> (px1, r1) is the first, in rows, of the transposed pair's
@@ -230,30 +285,23 @@ __device__ void sr_join(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, u
 __device__ bool sr_first_chain(const SrMergeArgs &a, const SrUnit &U, uint32_t kx, uint32_t px0, int32_t r0,
                                int32_t ts, int32_t te, uint32_t px1, int32_t r1)
 {
+    (void)kx;
     const uint32_t d0 = a.dist_of[0];
     const int32_t from = ts + (int32_t)a.blen[0] - 1;
-    if (a.nb_gaps == 1) {
-        const uint32_t len1 = a.blen[1];
-        const int32_t off_lo = (int32_t)(a.gmin[0] + len1), off_hi = (int32_t)(a.gmax[0] + len1);
-        bool first = true;
-        sr_scan(U, a.dist_of[1], r0 + off_lo, r1 - 1, [&](uint32_t p, int32_t) {
-            if (p == px1) first = false;
-        });
-        for (uint32_t h2 = sr_row_lower(U, d0, px0); first && h2 < U.ends[d0] && (U.hits[h2] >> 16) == px0; ++h2) {
-            const int32_t r2 = (int32_t)(U.hits[h2] & 0xFFFFu) + (int32_t)px0;
-            if (r2 < from || r2 >= r0) continue;
-            sr_scan(U, a.dist_of[1], r2 + off_lo, min(r2 + off_hi, te - 1), [&](uint32_t p, int32_t) {
-                if (p == px1) first = false;
-            });
-        }
-        return first;
-    }
-    for (uint32_t h2 = sr_row_lower(U, d0, px0); h2 < U.ends[d0] && (U.hits[h2] >> 16) == px0; ++h2) {
+    const uint32_t len1 = a.blen[1];
+    const int32_t off_lo = (int32_t)(a.gmin[0] + len1), off_hi = (int32_t)(a.gmax[0] + len1);
+    bool first = true;
+    sr_scan(U, a.dist_of[1], r0 + off_lo, r1 - 1, [&](uint32_t p, int32_t) {
+        if (p == px1) first = false;
+    });
+    for (uint32_t h2 = sr_row_lower(U, d0, px0); first && h2 < U.ends[d0] && (U.hits[h2] >> 16) == px0; ++h2) {
         const int32_t r2 = (int32_t)(U.hits[h2] & 0xFFFFu) + (int32_t)px0;
         if (r2 < from || r2 >= r0) continue;
-        if (sr_chains<1>(a, U, kx, px0, r2, te)) return false;
+        sr_scan(U, a.dist_of[1], r2 + off_lo, min(r2 + off_hi, te - 1), [&](uint32_t p, int32_t) {
+            if (p == px1) first = false;
+        });
     }
-    return true;
+    return first;
 }
 
 }  // namespace
@@ -438,6 +486,7 @@ __global__ __launch_bounds__(64) void struct_range_sweep(SrSweepArgs a)
 // Merge
 // ---------------------------------------------------------------------------------------
 
+template <bool ONE_GAP>
 __global__ __launch_bounds__(kSrMergeThreads) void struct_range_merge(SrMergeArgs a)
 {
     extern __shared__ uint32_t smem[];
@@ -518,7 +567,7 @@ __global__ __launch_bounds__(kSrMergeThreads) void struct_range_merge(SrMergeArg
         const uint32_t ti = sr_text_of(starts, HR.ny, r0);
         const int32_t ts = starts[ti], te = starts[ti + 1];
         if (ti < t_first || r0 - (int32_t)len0 + 1 < ts) continue;  // box 0's window inside the text
-        if (a.nb_gaps == 1)
+        if constexpr (ONE_GAP)
             sr_join(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
         else
             sr_walk(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
@@ -577,7 +626,10 @@ hipError_t launch_struct_range_sweep(const SrSweepArgs &a, uint32_t grid, size_t
 
 hipError_t launch_struct_range_merge(const SrMergeArgs &a, uint32_t units, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL(struct_range_merge, dim3(units), dim3(kSrMergeThreads), lds, s, a);
+    if (a.nb_gaps == 1)
+        hipLaunchKernelGGL((struct_range_merge<true>), dim3(units), dim3(kSrMergeThreads), lds, s, a);
+    else
+        hipLaunchKernelGGL((struct_range_merge<false>), dim3(units), dim3(kSrMergeThreads), lds, s, a);
     return hipGetLastError();
 }
 

@@ -227,3 +227,17 @@ def test_range_plan_dense_rows(ctx, oracle, monkeypatch):
     seqs = rand_seqs(rng, 40, 100, 250, b"ACGT")
     u, v, st = check(ctx, oracle, seqs, 6, 1, [(3, 8, 6, 1)], sym="range")
     assert st["kernel_variant"] == 1 and st["handed_pairs"] == 0 and v.sum() > 0
+
+
+@pytest.mark.parametrize("symmetric", ["1", "0"])
+def test_range_plan_in_several_launches(ctx, oracle, monkeypatch, symmetric):
+    # slot memory for a handful of units (about 46 KB each here) per launch pair: the run is
+    # several sweep/merge launch pairs sharing the accumulators, the self-pair flags and the
+    # overflow list
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1")
+    monkeypatch.setenv("MOTEX_SYM", symmetric)
+    monkeypatch.setenv("MOTEX_SR_SLOT_BYTES", "300000")
+    rng = np.random.default_rng(63)
+    seqs = rand_seqs(rng, 24, 150, 400, b"ACGT")
+    u, v, st = check(ctx, oracle, seqs, 7, 1, [(2, 9, 7, 1)], sym="range" if symmetric == "1" else "range-ordered")
+    assert v.sum() > 0
