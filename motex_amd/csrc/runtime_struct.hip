@@ -207,9 +207,14 @@ static int struct_range_prepare(motex_batch *b, uint32_t maxlen, bool *taken)
         text_syms += r.ye - r.ys;
         j = jj;
     }
-    // units: every x against every range (pair-symmetric, one gap: the ranges from the one
-    // holding x on, each unordered pair once), longest x first
-    const bool sym = G == 1 && struct_symmetric(b);
+    // units: every x against every range (pair-symmetric: the ranges from the one holding x
+    // on, each unordered pair once), longest x first
+    // With several gaps the merge, not the sweep, bounds the run, and the text side's credits
+    // share its key set: 500 x 600 nt, 8/1 boxes, two gaps 4.8 -> 4.5 ms, three gaps 7.6 ->
+    // 9.5 ms.  So those take the plan only when MOTEX_SYM=1 asks for it.  (Text-side keys hold
+    // the combination in 17 bits.)
+    const char *sym_env = getenv("MOTEX_SYM");
+    const bool sym = struct_symmetric(b) && (G == 1 || (sym_env && sym_env[0] == '1')) && t.nb_structs < (1u << 17);
     std::vector<uint32_t> range_of(ss.num, 0);
     for (uint32_t r = 0; r < ranges.size(); ++r)
         for (uint32_t j = ranges[r].j0; j < ranges[r].j0 + ranges[r].ny; ++j) range_of[j] = r;

@@ -165,11 +165,12 @@ __device__ __forceinline__ void sr_credit(const SrMergeArgs &a, const SrUnit &U,
 // Every chain of boxes B..nb_gaps from box B-1 ending at (pxp, rp), inside a text ending before
 // rend: the box-B hits of the position window rp + [gmin + len_B, gmax + len_B], from the
 // buckets, whose rows lie in pxp + [gmin + len_B, gmax + len_B]; the row names box B's x-side
-// gap, and the gaps build the combination index kx (last gap fastest).  f(kx) for each complete
-// chain but the one on x's own diagonal (dg: the chain so far lies on it).
+// gap and the position its text-side gap, which build the combination indexes kx and ky (last
+// gap fastest).  f(kx, ky) for each complete chain but the one on x's own diagonal (dg: the
+// chain so far lies on it).
 template <int B, class F>
 __device__ void sr_enum(const SrMergeArgs &a, const SrUnit &U, uint32_t pxp, int32_t rp, int32_t rend, uint32_t kxa,
-                        bool dg, F &f)
+                        uint32_t kya, bool dg, F &f)
 {
     const uint32_t span = a.gmax[B - 1] - a.gmin[B - 1];
     const uint32_t glo = pxp + a.gmin[B - 1] + a.blen[B];
@@ -180,11 +181,12 @@ __device__ void sr_enum(const SrMergeArgs &a, const SrUnit &U, uint32_t pxp, int
         const uint32_t g = px - glo;
         if (g > span) return;
         const uint32_t kx = kxa + g * a.gstride[B - 1];
+        const uint32_t ky = kya + (uint32_t)(r - lo) * a.gstride[B - 1];
         const bool d = dg && r - (int32_t)px == U.diag;
         if (B == (int)a.nb_gaps) {
-            if (!d) f(kx);
+            if (!d) f(kx, ky);
         } else if constexpr (B < (int)kSrMaxGaps) {
-            sr_enum<B + 1>(a, U, px, r, rend, kx, d, f);
+            sr_enum<B + 1>(a, U, px, r, rend, kx, ky, d, f);
         }
     });
 }
@@ -197,10 +199,10 @@ __device__ bool sr_first_chain_g(const SrMergeArgs &a, const SrUnit &U, uint32_t
                                  int32_t ts, int32_t te, uint32_t t)
 {
     uint32_t seen = 0, before = 0;
-    auto count = [&](uint32_t k) {
+    auto count = [&](uint32_t k, uint32_t) {
         if (seen++ < t && k == kx) ++before;
     };
-    sr_enum<1>(a, U, px0, r0, te, 0u, false, count);
+    sr_enum<1>(a, U, px0, r0, te, 0u, 0u, false, count);
     if (before) return false;
     const uint32_t d0 = a.dist_of[0];
     const int32_t from = ts + (int32_t)a.blen[0] - 1;
@@ -212,14 +214,42 @@ __device__ bool sr_first_chain_g(const SrMergeArgs &a, const SrUnit &U, uint32_t
     return true;
 }
 
+// The transposed pair's side of the same: whether the t-th chain of occurrence (px0, r0), of
+// text-side combination ky, is the first of its (ky, r0): no earlier chain of this occurrence
+// has ky, and no occurrence of box 0 at r0 in an earlier row has a chain with ky.
+__device__ bool sr_first_chain_tg(const SrMergeArgs &a, const SrUnit &U, uint32_t ky, uint32_t px0, int32_t r0,
+                                  int32_t te, uint32_t t)
+{
+    uint32_t seen = 0, before = 0;
+    auto count = [&](uint32_t, uint32_t k) {
+        if (seen++ < t && k == ky) ++before;
+    };
+    sr_enum<1>(a, U, px0, r0, te, 0u, 0u, false, count);
+    if (before) return false;
+    bool first = true;
+    sr_scan(U, a.dist_of[0], r0, r0, [&](uint32_t q, int32_t) {
+        if (!first || q >= px0 || q + 1 < a.blen[0]) return;
+        auto any = [&](uint32_t, uint32_t k) {
+            if (k == ky) first = false;
+        };
+        sr_enum<1>(a, U, q, r0, te, 0u, 0u, false, any);
+    });
+    return first;
+}
+
 // Occurrence of box 0 at row px0, position r0, in text ti = [ts, te) (positions), several gaps:
-// each chain is one v of its combination; u once per (kx, px0, text) through the key set.
+// each chain is one v of its combination; u once per (kx, px0, text) through the key set.  On
+// pair-symmetric units a chain in a text j after x is also one of the transposed pair (j, x):
+// box 0 at j's position r0 - ts, text-side combination ky; u of j once per (ky, r0).
+template <bool SYM>
 __device__ void sr_walk(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, uint32_t uv_base, uint32_t px0,
                         int32_t r0, uint32_t ti, int32_t ts, int32_t te)
 {
     const bool diag = ti == U.self && r0 - (int32_t)px0 == U.diag;
+    const bool both = SYM && ti != U.self;
+    const uint32_t n = (uint32_t)(te - ts);
     uint32_t ord = 0;
-    auto credit = [&](uint32_t kx) {
+    auto credit = [&](uint32_t kx, uint32_t ky) {
         const uint32_t t = ord++;
         unsigned long long add = 1;
         if (ti != U.self) {
@@ -227,8 +257,15 @@ __device__ void sr_walk(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, u
             if (k == 1 || (k == 2 && sr_first_chain_g(a, U, kx, px0, r0, ts, te, t))) add += 1ull << 32;
         }
         atomicAdd(a.uv + (size_t)uv_base + (size_t)kx * U.m + px0, add);
+        if constexpr (SYM) {
+            if (!both) return;
+            const uint32_t k = sr_key_insert(keys, 0x80000000u | (ky << 14) | (uint32_t)r0);
+            const unsigned long long tadd =
+                1ull + ((k == 1 || (k == 2 && sr_first_chain_tg(a, U, ky, px0, r0, te, t))) ? 1ull << 32 : 0ull);
+            atomicAdd(a.uv + (size_t)U.tuvb[ti] + (size_t)ky * n + (uint32_t)(r0 - ts), tadd);
+        }
     };
-    sr_enum<1>(a, U, px0, r0, te, 0u, diag, credit);
+    sr_enum<1>(a, U, px0, r0, te, 0u, 0u, diag, credit);
 }
 
 // Whether the chain (px0, r0) -> (px1, r1) is the first, in rows, of the transposed pair's
@@ -257,6 +294,7 @@ __device__ bool sr_first_chain_t(const SrMergeArgs &a, const SrUnit &U, uint32_t
 // such hit, and each is one chain of kx (but the one on x's own diagonal).  With pair-symmetric
 // units a chain in a text j after x is also one of the transposed pair (j, x): box 0 at j's
 // position r0 - ts, ky = r1 - r0 - len1 - gmin; u of j once per (ky, r0) in the unit.
+template <bool SYM>
 __device__ void sr_join(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, uint32_t uv_base, uint32_t px0,
                         int32_t r0, uint32_t ti, int32_t ts, int32_t te)
 {
@@ -266,12 +304,12 @@ __device__ void sr_join(const SrMergeArgs &a, const SrUnit &U, uint32_t *keys, u
     const int32_t lo = r0 + (int32_t)(gmn + len1), hi = min(r0 + (int32_t)(a.gmax[0] + len1), te - 1);
     if (lo > hi) return;
     const bool diag = ti == U.self && r0 - (int32_t)px0 == U.diag;
-    const bool both = a.sym && ti != U.self;
+    const bool both = SYM && ti != U.self;
     sr_scan(U, a.dist_of[1], lo, hi, [&](uint32_t px1, int32_t r1) {
         const uint32_t kx = px1 - px0 - len1 - gmn;
         if (kx > span || (diag && r1 - (int32_t)px1 == U.diag)) return;
         sr_credit(a, U, keys, uv_base, kx, px0, r0, ti, ts, te, 1, px1, r1);
-        if (both) {
+        if (SYM && both) {
             const uint32_t ky = (uint32_t)(r1 - lo);
             const uint32_t n = (uint32_t)(te - ts);
             const uint32_t k = sr_key_insert(keys, 0x80000000u | (ky << 14) | (uint32_t)r0);
@@ -486,7 +524,7 @@ __global__ __launch_bounds__(64) void struct_range_sweep(SrSweepArgs a)
 // Merge
 // ---------------------------------------------------------------------------------------
 
-template <bool ONE_GAP>
+template <bool ONE_GAP, bool SYM>
 __global__ __launch_bounds__(kSrMergeThreads) void struct_range_merge(SrMergeArgs a)
 {
     extern __shared__ uint32_t smem[];
@@ -568,9 +606,9 @@ __global__ __launch_bounds__(kSrMergeThreads) void struct_range_merge(SrMergeArg
         const int32_t ts = starts[ti], te = starts[ti + 1];
         if (ti < t_first || r0 - (int32_t)len0 + 1 < ts) continue;  // box 0's window inside the text
         if constexpr (ONE_GAP)
-            sr_join(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
+            sr_join<SYM>(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
         else
-            sr_walk(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
+            sr_walk<SYM>(a, U, keys, I.uv_base, px0, r0, ti, ts, te);
     }
 }
 
@@ -626,10 +664,15 @@ hipError_t launch_struct_range_sweep(const SrSweepArgs &a, uint32_t grid, size_t
 
 hipError_t launch_struct_range_merge(const SrMergeArgs &a, uint32_t units, size_t lds, hipStream_t s)
 {
-    if (a.nb_gaps == 1)
-        hipLaunchKernelGGL((struct_range_merge<true>), dim3(units), dim3(kSrMergeThreads), lds, s, a);
+    const dim3 grid(units), block(kSrMergeThreads);
+    if (a.nb_gaps == 1 && a.sym)
+        hipLaunchKernelGGL((struct_range_merge<true, true>), grid, block, lds, s, a);
+    else if (a.nb_gaps == 1)
+        hipLaunchKernelGGL((struct_range_merge<true, false>), grid, block, lds, s, a);
+    else if (a.sym)
+        hipLaunchKernelGGL((struct_range_merge<false, true>), grid, block, lds, s, a);
     else
-        hipLaunchKernelGGL((struct_range_merge<false>), dim3(units), dim3(kSrMergeThreads), lds, s, a);
+        hipLaunchKernelGGL((struct_range_merge<false, false>), grid, block, lds, s, a);
     return hipGetLastError();
 }
 

@@ -50,7 +50,7 @@ def check(ctx, oracle, seqs, k, e, boxes, distance=0, sym=None):
     elif sym == "ordered":
         assert st["symmetric"] == 0
     elif sym == "range" and range_eligible(seqs, k, e, boxes, distance):
-        assert st["kernel_variant"] == 1 and st["symmetric"] == int(len(boxes) == 1)
+        assert st["kernel_variant"] == 1 and st["symmetric"] == 1
     elif sym == "range-ordered" and range_eligible(seqs, k, e, boxes, distance):
         assert st["kernel_variant"] == 1 and st["symmetric"] == 0
     ou, ov, ooff, _ = oracle.all_pairs(seqs, distance, k, e, boxes=boxes)
