@@ -241,3 +241,21 @@ def test_range_plan_in_several_launches(ctx, oracle, monkeypatch, symmetric):
     seqs = rand_seqs(rng, 24, 150, 400, b"ACGT")
     u, v, st = check(ctx, oracle, seqs, 7, 1, [(2, 9, 7, 1)], sym="range" if symmetric == "1" else "range-ordered")
     assert v.sum() > 0
+
+
+def test_range_plan_protein_alphabet(ctx, oracle, monkeypatch):
+    # 20 letters: a larger NE window per unit (20 symbol rows in LDS)
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1")
+    w = synth.prot500_edit(scale=0.03)
+    seqs = [s[:700] for s in w.seqs]
+    u, v, st = check(ctx, oracle, seqs, 5, 1, [(3, 12, 5, 1)], sym="range")
+    assert st["kernel_variant"] == 1 and v.sum() > 0
+
+
+def test_range_plan_on_genes(ctx, oracle, monkeypatch):
+    # the reference's H. sapiens genes (N symbols, repeats), one gap, on the default plan
+    monkeypatch.delenv("MOTEX_STRUCT_RANGE", raising=False)
+    from motex_amd import refdata
+    seqs = refdata.hsapiens(60).seqs
+    u, v, st = check(ctx, oracle, seqs, 8, 1, [(10, 30, 8, 1)])
+    assert st["kernel_variant"] == 1 and v.sum() > 0
