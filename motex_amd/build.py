@@ -23,7 +23,12 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{os.path.join(ROOT, 'include')}", f"
 # Packed FP32 instructions are kept out of every device code object.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["hd_kernels.hip", "ed_kernels.hip", "ed_slice_kernels.hip", "ed_slice_kernels_k15.hip",
+# Per-source device flags.  The Hamming kernels add each row's hit count to a per-row LDS word
+# from the lanes that hit (a handful); the atomic optimizer would turn that uniform-address add
+# into a scalar loop over the active lanes on every row.
+HD_RING_SOURCES = [f"hd_ring_kernels_k{k}.hip" for k in (8, 10, 12, 14, 16)]
+SOURCE_FLAGS = {s: ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"] for s in ["hd_kernels.hip", *HD_RING_SOURCES]}
+LIB_SOURCES = ["hd_kernels.hip", *HD_RING_SOURCES, "ed_kernels.hip", "ed_slice_kernels.hip", "ed_slice_kernels_k15.hip",
                "ed_slice_kernels_k23.hip", "struct_kernels.hip", "struct_range_kernels.hip", "runtime.hip",
                "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip", "runtime_noalloc.hip"]
 CLI_SOURCES = ["cli/main.cpp", "cli/app.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/writer_back.cpp", "cli/stats.cpp",
@@ -58,7 +63,7 @@ def _obj(src):
 
 def _compile_lib(src):
     out = _obj(src)
-    _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", out])
+    _run([HIPCC, *COMMON, *DEVICE, *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", out])
     return out
 
 

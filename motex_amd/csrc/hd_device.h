@@ -14,18 +14,18 @@
 namespace motex {
 
 // Planes below the threshold plane for error bound e.
-__host__ __device__ inline uint32_t hd_low_planes(uint32_t e)
+__host__ __device__ constexpr uint32_t hd_low_planes(uint32_t e)
 {
     return e ? 32u - (uint32_t)__builtin_clz(e) : 0u;
 }
 
-__host__ __device__ inline uint32_t hd_bias(uint32_t e)
+__host__ __device__ constexpr uint32_t hd_bias(uint32_t e)
 {
     return (1u << hd_low_planes(e)) - 1u - e;
 }
 
 // Planes a biased window counter of length k needs.
-__host__ __device__ inline uint32_t hd_counter_planes(uint32_t k, uint32_t e)
+__host__ __device__ constexpr uint32_t hd_counter_planes(uint32_t k, uint32_t e)
 {
     return 32u - (uint32_t)__builtin_clz(hd_bias(e) + k);
 }

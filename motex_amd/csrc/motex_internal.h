@@ -29,10 +29,15 @@ constexpr double kSlicePipeShare = 0.5;
 // Serving jobs (sharing the device): pipelined below this many items per CU.
 constexpr uint32_t kSlicePipeItemsPerCu = 12;
 
-// A text range of the Hamming work queue: whole sequences j0 .. j0+ny-1, or one part of a
-// long sequence j0 (ny = 1), covering text positions [ys, ye).
+// A text range of the Hamming work queue: whole sequences j0 .. j0+ny-1 covering text
+// positions [ys, ye), or one block of diagonals of a long sequence j0 (ny = 1, [ys, ye) the
+// whole sequence).  A work item's window starts at diagonal ((ys - r1 + 1) rounded down to a
+// multiple of 32) + qoff: whole-sequence ranges have qoff = 0; the parts of a long sequence
+// have qoff = 0, D, 2D, ... (D = diagonals per window), so that their windows tile the
+// diagonals of every row tile exactly.
 struct HdRange {
     uint32_t ys, ye, j0, ny;
+    uint32_t qoff;
 };
 
 // One unit of the Hamming work queue: rows [r0, r1) of sequence x against the text ranges

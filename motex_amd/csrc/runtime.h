@@ -231,7 +231,12 @@ int pair_run(motex_batch *b, uint32_t i, uint32_t j);
 int ed_pair_run(motex_batch *b, uint32_t i, uint32_t j);
 int struct_pair_run(motex_batch *b, uint32_t i, uint32_t j);
 HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t persistent);
-hipError_t hd_alloc_hitq(DevBuf &buf, uint64_t grid, uint32_t blocks);
+hipError_t hd_alloc_hitq(DevBuf &buf, uint64_t grid, uint32_t blocks, uint32_t cap);
+// Whether four-block Hamming launches for (k, e) run the register-ring kernels (which hold
+// 4 waves per SIMD).
+bool hd_has_ring(uint32_t k, uint32_t e);
+// Entries per wave's Hamming hit queue for (k, e) and a launch of `blocks` blocks.
+uint32_t hd_hitq_cap(uint32_t k, uint32_t e, uint32_t blocks);
 void serving_shutdown(motex_ctx *c);
 int noalloc_init(int device);  // looks up the no-allocation scratch array's address on `device`
 

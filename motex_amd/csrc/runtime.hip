@@ -224,10 +224,13 @@ void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
             split_of.push_back(kNoSplit);
             return;
         }
+        // blocks of D diagonals: a row tile [px_start, r1) meets diagonals from
+        // (start - r1 + 1) rounded down (>= start - r1 - 30) to end - 1 - px_start
         uint32_t slot = plan.split_slots++;
-        for (uint32_t p = 0; p < n; p += (uint32_t)cap) {
-            uint32_t e = std::min<uint32_t>(n, p + (uint32_t)cap);
-            plan.ranges.push_back({ss.start[j] + p, ss.start[j] + e, j, 1});
+        const uint32_t D = kDiagPerBlock * blocks;
+        const uint64_t nb = ((uint64_t)n + plan.rows_max + 30 + D - 1) / D;
+        for (uint64_t i = 0; i < nb; ++i) {
+            plan.ranges.push_back({ss.start[j], ss.start[j] + n, j, 1, (uint32_t)(i * D)});
             split_of.push_back(slot);
         }
     };
@@ -296,6 +299,7 @@ static int64_t hd_wbase(const HdItem &it, const HdRange &R)
 {
     int64_t q0 = ((int64_t)R.ys - (int64_t)it.r1 + 1);
     q0 = q0 >= 0 ? (q0 & ~31LL) : -(((-q0) + 31) & ~31LL);
+    q0 += R.qoff;
     int64_t qp = q0 + (int64_t)kPadBits;
     int64_t w = qp >= 0 ? qp / 32 : -((-qp + 31) / 32);
     return w + (it.px_start >> 5);
@@ -510,7 +514,8 @@ int hd_prepare(motex_batch *b)
     // hit queues of the persistent grid (one per workgroup)
     const uint64_t grid = std::min<uint64_t>((uint64_t)c->num_cu * ts.waves_per_cu, b->hd.items.size());
     tr.mark("NE tables");
-    if ((e = hd_alloc_hitq(b->d_hitq, grid, ts.blocks)) != hipSuccess) return hip_fail(e, "alloc hit queues");
+    if ((e = hd_alloc_hitq(b->d_hitq, grid, ts.blocks, hd_hitq_cap(b->params.k, b->params.e, 4))) != hipSuccess)
+        return hip_fail(e, "alloc hit queues");
     tr.mark("hit queues");
     b->stats.symmetric = sym ? 1u : 0u;
     b->stats.swept_cells = hd_swept_cells(b->hd);
@@ -535,9 +540,9 @@ double hd_swept_cells(const HdPlan &plan)
     return s;
 }
 
-hipError_t hd_alloc_hitq(DevBuf &buf, uint64_t grid, uint32_t blocks)
+hipError_t hd_alloc_hitq(DevBuf &buf, uint64_t grid, uint32_t blocks, uint32_t cap)
 {
-    const uint64_t need = std::max<uint64_t>(grid, 1) * kHitQueueCap * (blocks + 1) * 4;
+    const uint64_t need = std::max<uint64_t>(grid, 1) * cap * (blocks + 1) * 4;
     if (buf.p && buf.bytes >= need) return hipSuccess;
     return buf.alloc(need);
 }
@@ -555,7 +560,7 @@ HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t
     a.split_flags = (uint32_t *)b->d_split.p;
     a.queue = (uint32_t *)b->d_queue.p;
     a.hitq = (uint32_t *)b->d_hitq.p;
-    a.hitq_cap = kHitQueueCap;
+    a.hitq_cap = hd_hitq_cap(b->params.k, b->params.e, 4);
     a.fbits = (uint32_t *)b->d_fbits.p;
     a.f_words = b->f_words;
     a.W = b->hd.W;
@@ -594,7 +599,7 @@ int hd_launch(motex_batch *b)
         HdArgs a = hd_args(b, b->hd, b->d_items, 1);
         uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->num_cu * b->tile.waves_per_cu, b->hd.items.size());
         if (b->grid_cap) grid = std::min(grid, b->grid_cap);
-        if ((uint64_t)grid * kHitQueueCap * (b->tile.blocks + 1) * 4 > b->d_hitq.bytes)
+        if ((uint64_t)grid * a.hitq_cap * (b->tile.blocks + 1) * 4 > b->d_hitq.bytes)
             return set_error(MOTEX_EINVAL, "internal: Hamming grid of %u beyond its hit queues", grid);
         if ((e = launch_hd(a, b->tile.blocks, grid, s)) != hipSuccess) return hip_fail(e, "launch Hamming kernel");
         if (b->d_fbits.p &&
@@ -745,7 +750,8 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
     if ((e = launch_fill_u32((uint32_t *)flags.p, flags.bytes / 4, 0, s)) != hipSuccess) return hip_fail(e, "clear flags");
     uint32_t max_count = 0;
     for (const Launch &L : launches) max_count = std::max(max_count, L.count);
-    if ((e = hd_alloc_hitq(hitq, max_count, 4)) != hipSuccess) return hip_fail(e, "alloc plain hit queues");
+    if ((e = hd_alloc_hitq(hitq, max_count, 4, hd_hitq_cap(k, b->params.e, 4))) != hipSuccess)
+        return hip_fail(e, "alloc plain hit queues");
     HdPlan shape;
     shape.rows_max = all.rows_max;
     if ((e = hipEventRecord(b->ev0, s)) != hipSuccess) return hip_fail(e, "record plain start");

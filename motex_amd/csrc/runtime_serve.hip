@@ -73,7 +73,7 @@ static size_t footprint(const Job &j, const motex_ctx *c)
     } else if (j.params.distance == 0) {
         bytes += 32 * (T + kPadBits + 16384) / 8;                // NE tables (up to 32 symbols)
         bytes += (T / 256 + 1) * (T / 7000 + 1) * sizeof(HdItem); // work items (upper bound)
-        bytes += grid * 2 * kHitQueueCap * 5 * 4;                 // hit queues (uncapped grid)
+        bytes += grid * 2 * hd_hitq_cap(j.params.k, j.params.e, 4) * 5 * 4;  // hit queues (uncapped grid)
     } else {
         // boundary streams (text-parallel kernel: up to 32 words per row, grid of 12 waves/CU)
         bytes += std::max<uint64_t>(grid * 2 * (maxlen + 128) * 8, (uint64_t)c->num_cu * 12 * 2 * maxlen * 32 * 4);

@@ -40,6 +40,17 @@ static TileShape default_tile(const motex_params &p, const SeqSet &ss)
         t.blocks = ss.sigma <= 8 ? 4 : (ss.sigma <= 24 ? 2 : 1);
         t.rows = 128;
         t.waves_per_cu = 32;
+        if (t.blocks == 4 && hd_has_ring(p.k, p.e)) {
+            // the register-ring kernels hold 4 waves per SIMD; texts longer than a window are
+            // cut into blocks of diagonals that do not depend on the row tile, so when every
+            // text is, longer tiles amortise the k - 1 warm-up rows (one 4-Mnt sequence:
+            // 168 ms at 128 rows, 162.5 at 256; 1000-nt texts lose a text per range above 160)
+            t.waves_per_cu = 16;
+            bool all_long = true;
+            for (uint32_t j = 0; j < ss.num && all_long; ++j)
+                all_long = ss.len[j] < p.k || ss.len[j] > kDiagPerBlock * 4 - 256 - 32;
+            if (all_long) t.rows = 256;
+        }
     } else if (!p.nb_gaps && p.k >= 4 && p.k <= kMaxSliceK && ss.sigma <= 32) {
         // text-parallel edit kernel: 100-200 VGPRs up to k = 14 (2-4 waves per SIMD
         // resident), 2 waves per SIMD beyond
@@ -134,7 +145,7 @@ extern "C" int motex_autotune(motex_ctx *c, const char *const *seqs, const uint3
     if (!p->nb_gaps && p->distance == 0) {
         for (uint32_t blocks : {1u, 2u, 4u})
             for (uint32_t rows : {64u, 128u, 192u, 256u})
-                for (uint32_t waves : {24u, 32u}) {
+                for (uint32_t waves : {16u, 24u, 32u}) {
                     if (!hd_fits(ss, p->k, blocks, rows)) continue;
                     TileShape t;
                     t.blocks = blocks;

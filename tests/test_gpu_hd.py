@@ -88,6 +88,18 @@ def test_long_sequence_split(ctx, oracle):
     check(ctx, oracle, seqs, 10, 2)
 
 
+@pytest.mark.parametrize("k,e", [(12, 2), (10, 3), (7, 1), (13, 2)])
+def test_long_and_short_mixed(ctx, oracle, k, e):
+    # long sequences are cut into blocks of diagonals per row tile: interior blocks count
+    # their hits in place, the blocks at a sequence's ends go through the hit queue; short
+    # sequences in the same batch keep whole-text ranges.  A two-letter alphabet makes
+    # occurrences dense.
+    rng = np.random.default_rng(k * 10 + e)
+    seqs = (rand_seqs(rng, 2, 17000, 26000, b"AC") + rand_seqs(rng, 5, 30, 900, b"ACGT") +
+            rand_seqs(rng, 1, 9000, 9000, b"ACGT"))
+    check(ctx, oracle, seqs, k, e)
+
+
 def test_empty_and_single(ctx, oracle):
     check(ctx, oracle, [b"ACGTACGTAC"], 4, 1)
     check(ctx, oracle, [b"ACG", b"AC"], 4, 1)   # every pair skipped
