@@ -36,61 +36,42 @@ __global__ void ne_build_kernel(const uint8_t *__restrict__ codes, uint32_t tota
     }
 }
 
-// u of the text side of pair-symmetric runs: for every text position p, the number of x
-// sequences whose bit (x, p) is set.  A workgroup takes 64 words of positions; its four
-// waves each count a quarter of the x sequences, keeping the 32 counts of a word as
-// bit-sliced planes (plane q = bit q of each count; adding one x's word is a ripple-carry
-// increment that stops when the carry dies out), and the quarters are summed with a
-// bit-sliced adder through LDS.
-__global__ __launch_bounds__(256) void fold_fbits_kernel(const uint32_t *__restrict__ fbits, uint32_t nx,
-                                                         uint32_t f_words, uint32_t total,
-                                                         uint32_t *__restrict__ u)
+// Text side of pair-symmetric runs: the first occurrences that the sweep listed per range
+// (sym_flush), as a histogram in LDS, then u and v of every position += its count.  Plain
+// adds: every position belongs to one range, and the sweep kernel has finished.  A list
+// starts at a multiple of 8 entries and is read 8 entries (16 bytes) per load, 1024 threads
+// per range, so that many loads are in flight.
+__global__ __launch_bounds__(1024) void sym_fold_kernel(const uint16_t *__restrict__ tlist,
+                                                       const uint32_t *__restrict__ toff,
+                                                       const uint32_t *__restrict__ tcnt,
+                                                       const HdRange *__restrict__ ranges, uint32_t *u,
+                                                       uint32_t *v)
 {
-    __shared__ uint32_t part[3][16][64];
-    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t w = blockIdx.x * 64 + lane;
-    uint32_t plane[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) plane[q] = 0;
-    if (w < f_words) {
-        const uint32_t x0 = (uint32_t)((uint64_t)nx * wv / 4), x1 = (uint32_t)((uint64_t)nx * (wv + 1) / 4);
-        for (uint32_t x = x0; x < x1; ++x) {
-            uint32_t carry = fbits[(size_t)x * f_words + w];
-            for (int q = 0; q < 16; ++q) {  // stops where the carry dies out
-                if (!carry) break;
-                const uint32_t t = plane[q] & carry;
-                plane[q] ^= carry;
-                carry = t;
-            }
-        }
-    }
-    if (wv) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) part[wv - 1][q][lane] = plane[q];
-    }
+    __shared__ uint32_t hist[kSymBitmapBits];
+    const uint32_t ri = blockIdx.x, tid = threadIdx.x;
+    for (uint32_t i = tid; i < kSymBitmapBits; i += 1024) hist[i] = 0;
     __syncthreads();
-    if (wv || w >= f_words) return;
-    for (int o = 0; o < 3; ++o) {
-        uint32_t c = 0;
+    const uint32_t off = toff[ri], n = min(tcnt[ri], toff[ri + 1] - off);
+    const uint4 *l8 = reinterpret_cast<const uint4 *>(tlist + off);
+    for (uint32_t g = tid; g < n / 8; g += 1024) {
+        const uint4 q = l8[g];
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const uint32_t a = plane[q], b = part[o][q][lane];
-            plane[q] = a ^ b ^ c;
-            c = (a & b) | (c & (a ^ b));
+        for (int i = 0; i < 4; ++i) {
+            atomicAdd(&hist[w[i] & 0xFFFFu], 1u);
+            atomicAdd(&hist[w[i] >> 16], 1u);
         }
     }
-    uint32_t any = 0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) any |= plane[q];
-    while (any) {
-        const uint32_t b = __builtin_ctz(any);
-        any &= any - 1;
-        const uint32_t p = w * 32 + b;
-        if (p >= total) break;
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) cnt |= ((plane[q] >> b) & 1u) << q;
-        u[p] += cnt;
+    for (uint32_t i = (n & ~7u) + tid; i < n; i += 1024) atomicAdd(&hist[tlist[off + i]], 1u);
+    __syncthreads();
+    const HdRange R = ranges[ri];
+    for (uint32_t bit = tid + 32; bit < R.ye - R.ys + 32; bit += 1024) {
+        const uint32_t h = hist[bit];
+        if (h) {
+            const uint32_t pos = R.ys + bit - 32;
+            u[pos] += h;
+            v[pos] += h;
+        }
     }
 }
 
@@ -153,11 +134,11 @@ static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream
     }
 }
 
-size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max)
+size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max, uint32_t sym_group)
 {
-    // NE window | v, sequence mask | range starts | row offsets (+2) | u
+    // NE window | v, sequence mask | range starts | row offsets (+2) | u | text-side bitmaps
     return (size_t)sigma * wl * 4 + (size_t)rows_max * 8 + (kMaxRangeSeqs + 1) * 4 +
-           ((size_t)rows_max + 2) * 4 + (size_t)rows_max * 4;
+           ((size_t)rows_max + 2) * 4 + (size_t)rows_max * 4 + (size_t)sym_group * 1024;
 }
 
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)
@@ -173,12 +154,11 @@ hipError_t launch_ne_build(const uint8_t *codes, uint32_t total, uint32_t sigma,
     return hipGetLastError();
 }
 
-hipError_t launch_fold_fbits(const uint32_t *fbits, uint32_t nx, uint32_t f_words, uint32_t total,
-                             uint32_t *u, hipStream_t s)
+hipError_t launch_sym_fold(const uint16_t *tlist, const uint32_t *toff, const uint32_t *tcnt, const HdRange *ranges,
+                           uint32_t nranges, uint32_t *u, uint32_t *v, hipStream_t s)
 {
-    if (nx > 65535) return hipErrorInvalidValue;  // counts are 16-bit planes
-    hipLaunchKernelGGL(fold_fbits_kernel, dim3((f_words + 63) / 64), dim3(256), 0, s, fbits, nx, f_words,
-                       total, u);
+    if (!nranges) return hipSuccess;
+    hipLaunchKernelGGL(sym_fold_kernel, dim3(nranges), dim3(1024), 0, s, tlist, toff, tcnt, ranges, u, v);
     return hipGetLastError();
 }
 
@@ -193,7 +173,7 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t val, hipStream_t s)
 
 hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_t s)
 {
-    size_t lds = hd_lds_bytes(a.sigma, a.wl, a.rows_max);
+    size_t lds = hd_lds_bytes(a.sigma, a.wl, a.rows_max, a.sym_group);
     switch (blocks) {
     case 1: return launch_p<1>(a, grid, lds, s);
     case 2: return launch_p<2>(a, grid, lds, s);

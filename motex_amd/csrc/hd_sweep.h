@@ -44,14 +44,16 @@ namespace motex {
 //
 // Pair-symmetric items (sym): texts j < xi are dropped (their pairs are computed with the
 // roles swapped), and an occurrence in a text j > xi is also credited to the text side:
-// v of (j, text position) += 1 and the (xi, text position) bit of fbits is set, from which
-// hd_fold_fbits adds the distinct-x count to u after the kernel.
+// v of (j, text position) += 1, and u of (j, text position) += 1 for the first occurrence of
+// x_xi there.  The item runs every row of x_xi against this range, so the range's bitmap tb
+// (bit pt - ys + 32 for text position pt, in LDS) tells the first occurrence exactly; first
+// occurrences stay in the bitmap until the item ends (sym_flush), later ones add to v here.
 template <int B>
 __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, int32_t q0,
                                            uint32_t px_start, uint32_t xi, bool sym,
                                            const uint32_t *hq_meta, const uint32_t *hq_res,
                                            uint32_t nq, const uint32_t *starts, uint32_t *acc_v,
-                                           uint32_t *ymask)
+                                           uint32_t *ymask, uint32_t *tb)
 {
     __syncthreads();  // the queue stores of every lane are visible to the loads below
     const int32_t ys = (int32_t)R.ys, ye = (int32_t)R.ye, km1 = (int32_t)a.k - 1;
@@ -99,21 +101,21 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
                         hits += __builtin_popcount(hv);
                         ym |= 1u << y;
                         if (sym && j > xi) {
-                            // text side: one v per occurrence, one bit per (xi, position)
-                            for (uint32_t h2 = hv; h2;) {
+                            // text side: one v per occurrence, one u per position that x_xi
+                            // reaches for the first time (the bitmap's word offset is >= 0:
+                            // base >= ys - 31)
+                            const uint32_t woff = (uint32_t)(base - ys + 32);
+                            const uint32_t w0 = woff >> 5, sh = woff & 31;
+                            const uint32_t lo_bits = hv << sh;
+                            const uint32_t hi_bits = sh ? hv >> (32 - sh) : 0u;
+                            uint32_t seen = 0;  // positions x_xi had reached before
+                            if (lo_bits) seen |= (lo_bits & atomicOr(tb + w0, lo_bits)) >> sh;
+                            if (hi_bits) seen |= (hi_bits & atomicOr(tb + w0 + 1, hi_bits)) << (32 - sh);
+                            for (uint32_t h2 = seen; h2;) {
                                 const uint32_t t = __builtin_ctz(h2);
                                 h2 &= h2 - 1;
                                 atomicAdd(a.v + (uint32_t)(base + (int32_t)t), 1u);
                             }
-                            // the flags' positions (all >= 0) span words w0 and w0 + 1 of
-                            // fbits; w0 is -1 when base < 0, and then holds none of them
-                            const int32_t w0 = base >> 5;
-                            const uint32_t sh = (uint32_t)base & 31;
-                            const uint32_t lo_bits = hv << sh;
-                            const uint32_t hi_bits = sh ? hv >> (32 - sh) : 0u;
-                            uint32_t *frow = a.fbits + (size_t)xi * a.f_words;
-                            if (lo_bits) atomicOr(frow + w0, lo_bits);
-                            if (hi_bits) atomicOr(frow + (w0 + 1), hi_bits);
                         }
                     }
                 }
@@ -125,6 +127,46 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
             atomicOr(ymask + r, ym);
         }
     }
+}
+
+// End of a pair-symmetric item: appends the first occurrences of x_xi in range ri (the set
+// bits of its bitmap tb, offsets pt - ys + 32) to the range's list, whose fold after the
+// kernel credits each one u += 1 and v += 1 (sym_fold_kernel).  One wave-wide prefix sum and
+// one counter atomic per range; a list that is full takes the rest as direct atomics.
+__device__ __forceinline__ void sym_flush(const HdArgs &a, uint32_t ri, uint32_t ys, const uint32_t *tb)
+{
+    const uint32_t lane = threadIdx.x;
+    uint32_t wd[4], n = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        wd[i] = tb[4 * lane + i];
+        n += __builtin_popcount(wd[i]);
+    }
+    uint32_t incl = n;  // inclusive prefix sum over the wave's lanes
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= (uint32_t)d) incl += o;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (!total) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(a.tcnt + ri, total);
+    base = __builtin_amdgcn_readfirstlane(base);
+    const uint32_t off = a.toff[ri], cap = a.toff[ri + 1] - off;
+    uint32_t idx = base + incl - n;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        for (uint32_t h = wd[i]; h; h &= h - 1, ++idx) {
+            const uint32_t bit = 32 * (4 * lane + i) + __builtin_ctz(h);
+            if (idx < cap) {
+                a.tlist[off + idx] = (uint16_t)bit;
+            } else {
+                const uint32_t pos = ys + bit - 32;
+                atomicAdd(a.u + pos, 1u);
+                atomicAdd(a.v + pos, 1u);
+            }
+        }
 }
 
 // Persistent batched kernel: one wave per workgroup pulls items from the queue head.
@@ -162,7 +204,8 @@ void hd_batched_kernel(HdArgs a)
     const uint32_t wl = a.wl;
     // LDS: NE window (sigma rows; at offset 0 so that the block offsets fit the read
     //      instructions' immediates) | per-row v, sequence mask | range sequence starts |
-    //      per-row NE byte offsets (2 spare entries for the prefetch) | per-row u
+    //      per-row NE byte offsets (2 spare entries for the prefetch) | per-row u |
+    //      text-side bitmaps
     uint32_t *ne_l = smem;
     uint32_t *acc_v = ne_l + a.sigma * wl;
     uint32_t *ymask = acc_v + a.rows_max;
@@ -171,6 +214,8 @@ void hd_batched_kernel(HdArgs a)
     // u counts distinct texts per row over all of the item's ranges: 32 bits, since an item
     // may span more texts than 16 bits can count (short sequences in large inputs)
     uint32_t *acc_u = xb + a.rows_max + 2;
+    // pair-symmetric launches: 256 words per range of an item (a.sym_group ranges at most)
+    uint32_t *tbits = acc_u + a.rows_max;
     const uint32_t lane4 = lane * 4;
     // this wave's hit queue: tags, then B words per entry (16-byte aligned)
     uint32_t *hq_meta = a.hitq + (size_t)blockIdx.x * a.hitq_cap * (B + 1);
@@ -187,264 +232,284 @@ void hd_batched_kernel(HdArgs a)
         }
         if (it >= a.nitems) break;
         const HdItem I = a.items[it];
-        const uint32_t nrows = I.r1 - I.px_start;
-        const uint32_t pxs5 = I.px_start >> 5;
-        __syncthreads();  // previous item finished with LDS
-        // row r reads its window of NE_{x[px]} at byte offset xb[r] (+ 4 lane + 256 block)
-        for (uint32_t r = lane; r < nrows + 2; r += 64) {
-            if (r < nrows) {
-                const uint32_t px = I.px_start + r;
-                xb[r] = (a.codes[I.x_start + px] * wl + (px >> 5) - pxs5) * 4;
-                acc_v[r] = 0;
-                acc_u[r] = 0;
-                ymask[r] = 0;
-            } else {
-                xb[r] = 0;
-            }
-        }
-        const uint32_t rep0 = I.r0 > a.k - 1 ? I.r0 : a.k - 1;
         const uint32_t bias = hd_bias(a.e);
         // PH < 0 (factors longer than 64): the threshold plane is chosen at run time
         uint32_t hmask[P];
 #pragma unroll
         for (int p = 0; p < P; ++p)
             hmask[p] = __builtin_amdgcn_readfirstlane((uint32_t)p >= hd_low_planes(a.e) ? ~0u : 0u);
-        for (uint32_t ri = I.rb; ri < I.re; ++ri) {
-            const HdRange R = a.ranges[ri];
-            // first diagonal (pt - px) of this range, a multiple of 32
-            const int32_t q0 = (((int32_t)R.ys - (int32_t)I.r1 + 1) & ~31) + (int32_t)R.qoff;
-            const uint32_t wbase = (uint32_t)(((q0 + (int32_t)kPadBits) >> 5) + (int32_t)pxs5);
-            // A block of a long sequence whose every cell of a reported row is a valid pair
-            // (text window inside the sequence: start + k - 1 <= pt < end): its rows count their
-            // hits in place (popcount, one LDS add) instead of queueing them.
-            const bool interior = I.split != kNoSplit &&
-                                  (int64_t)rep0 + q0 >= (int64_t)R.ys + a.k - 1 &&
-                                  (int64_t)I.r1 - 1 + q0 + (int64_t)kDiagPerBlock * B <= (int64_t)R.ye;
-            __syncthreads();
-            for (uint32_t c = 0; c < a.sigma; ++c) {
-                const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
-                for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
+        __syncthreads();  // previous item finished with LDS
+        // pair-symmetric items: the text-side first-occurrence bitmaps of their ranges
+        if (I.sym)
+            for (uint32_t w = lane; w < (I.re - I.rb) * 256; w += 64) tbits[w] = 0;
+        // row tiles of tile_rows reported rows (one, except on pair-symmetric items, which take
+        // every row of x)
+        for (uint32_t tr0 = I.r0; tr0 < I.r1; tr0 += a.tile_rows) {
+            const uint32_t tr1 = min(I.r1, tr0 + a.tile_rows);
+            const uint32_t px_start = tr0 - (a.k - 1);  // first computed row (k - 1 warm-up rows)
+            const uint32_t nrows = tr1 - px_start;
+            const uint32_t pxs5 = px_start >> 5;
+            __syncthreads();  // previous tile finished with the row arrays
+            // row r reads its window of NE_{x[px]} at byte offset xb[r] (+ 4 lane + 256 block)
+            for (uint32_t r = lane; r < nrows + 2; r += 64) {
+                if (r < nrows) {
+                    const uint32_t px = px_start + r;
+                    xb[r] = (a.codes[I.x_start + px] * wl + (px >> 5) - pxs5) * 4;
+                    acc_v[r] = 0;
+                    acc_u[r] = 0;
+                    ymask[r] = 0;
+                } else {
+                    xb[r] = 0;
+                }
             }
-            if (lane <= R.ny) starts[lane] = a.seq_start[R.j0 + lane];
-            __syncthreads();
+            const uint32_t rep0 = tr0;  // first reported row (tiles start at k - 1 or later)
+            for (uint32_t ri = I.rb; ri < I.re; ++ri) {
+                uint32_t *tb = tbits + (ri - I.rb) * 256;
+                const HdRange R = a.ranges[ri];
+                // first diagonal (pt - px) of this range, a multiple of 32
+                const int32_t q0 = (((int32_t)R.ys - (int32_t)tr1 + 1) & ~31) + (int32_t)R.qoff;
+                const uint32_t wbase = (uint32_t)(((q0 + (int32_t)kPadBits) >> 5) + (int32_t)pxs5);
+                // A block of a long sequence whose every cell of a reported row is a valid pair
+                // (text window inside the sequence: start + k - 1 <= pt < end): its rows count
+                // their hits in place (popcount, one LDS add) instead of queueing them.
+                const bool interior = I.split != kNoSplit &&
+                                      (int64_t)rep0 + q0 >= (int64_t)R.ys + a.k - 1 &&
+                                      (int64_t)tr1 - 1 + q0 + (int64_t)kDiagPerBlock * B <= (int64_t)R.ye;
+                __syncthreads();
+                for (uint32_t c = 0; c < a.sigma; ++c) {
+                    const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
+                    for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
+                }
+                if (lane <= R.ny) starts[lane] = a.seq_start[R.j0 + lane];
+                __syncthreads();
 
-            uint32_t nq = 0;  // queued (row, lane) entries, wave-uniform
-            uint32_t cnt[B][P];
+                uint32_t nq = 0;  // queued (row, lane) entries, wave-uniform
+                uint32_t cnt[B][P];
 #pragma unroll
-            for (int b = 0; b < B; ++b) planes_set<P>(cnt[b], bias);
+                for (int b = 0; b < B; ++b) planes_set<P>(cnt[b], bias);
 
-            // The NE words a row reads: for each block, the two words under its window for the
-            // entering column (en) and, in the generic form, for the column leaving at r - k (lv).
-            struct Win {
-                uint32_t en[B][2], lv[B][2];
-            };
-            // (xn, xo: xb of rows r and r - k)
-            auto load_win = [&](uint32_t xn, uint32_t xo, Win &w, auto leave) {
-                const char *pn = reinterpret_cast<const char *>(ne_l) + (xn + lane4);
-                const char *po = reinterpret_cast<const char *>(ne_l) + (xo + lane4);
+                // The NE words a row reads: for each block, the two words under its window for the
+                // entering column (en) and, in the generic form, for the column leaving at r - k (lv).
+                struct Win {
+                    uint32_t en[B][2], lv[B][2];
+                };
+                // (xn, xo: xb of rows r and r - k)
+                auto load_win = [&](uint32_t xn, uint32_t xo, Win &w, auto leave) {
+                    const char *pn = reinterpret_cast<const char *>(ne_l) + (xn + lane4);
+                    const char *po = reinterpret_cast<const char *>(ne_l) + (xo + lane4);
 #pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    const uint32_t *wn = reinterpret_cast<const uint32_t *>(pn + 256 * b);
-                    w.en[b][0] = wn[0];
-                    w.en[b][1] = wn[1];
-                    if constexpr (decltype(leave)::value) {
-                        const uint32_t *wo = reinterpret_cast<const uint32_t *>(po + 256 * b);
-                        w.lv[b][0] = wo[0];
-                        w.lv[b][1] = wo[1];
-                    }
-                }
-            };
-            // entering mismatch bits of row r (px = px_start + r) from its loaded words
-            auto enter_bits = [&](uint32_t r, const Win &w, uint32_t *an) {
-                const uint32_t shn = (I.px_start + r) & 31;
-#pragma unroll
-                for (int b = 0; b < B; ++b) an[b] = __builtin_amdgcn_alignbit(w.en[b][1], w.en[b][0], shn);
-            };
-            // one row: window counters of row r from its entering bits (and the bits of the
-            // column leaving at r - k when LEAVE), then the row's hit flags into the queue
-            // (the ring form appends a whole trip of K rows before it drains: the queue then
-            // holds hitq_cap >= 64 (K + 2) entries, and it is drained once more than
-            // hitq_cap - 64 K are queued)
-            auto step = [&](uint32_t r, const uint32_t *an, const uint32_t *bo, auto leave) {
-                const uint32_t px = I.px_start + r;
-                uint32_t high[B];
-                uint32_t all = ~0u;  // lanes' words with no hit in any block
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    if constexpr (decltype(leave)::value) planes_slide<P>(cnt[b], an[b], bo[b]);
-                    else planes_inc<P>(cnt[b], an[b]);
-                    if constexpr (PH >= 0) {
-                        high[b] = planes_high<P, PH>(cnt[b]);
-                    } else {
-                        uint32_t h = 0;
-#pragma unroll
-                        for (int p = 0; p < P; ++p) h |= cnt[b][p] & hmask[p];
-                        high[b] = h;
-                    }
-                    all &= high[b];
-                }
-                if (px >= rep0 && interior) {
-                    if (all != ~0u) {
-                        uint32_t c = 0;  // set high-plane bits: the lane's cells with no hit
-#pragma unroll
-                        for (int b = 0; b < B; ++b) c += __builtin_popcount(high[b]);
-                        atomicAdd(acc_v + r, 32u * B - c);
-                    }
-                } else if (px >= rep0) {
-                    const uint64_t m = __ballot(all != ~0u);
-                    if (m) {
-                        if (all != ~0u) {
-                            // the entry holds the high-plane words (clear bit = hit); 32-bit
-                            // byte offsets from the wave's (scalar) queue bases
-                            const uint32_t slot =
-                                nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                            char *res_b = reinterpret_cast<char *>(hq_res) + slot * (4u * B);
-                            if constexpr (B == 4) {
-                                *reinterpret_cast<uint4 *>(res_b) = make_uint4(high[0], high[1], high[2], high[3]);
-                            } else if constexpr (B == 2) {
-                                *reinterpret_cast<uint2 *>(res_b) = make_uint2(high[0], high[1]);
-                            } else {
-                                *reinterpret_cast<uint32_t *>(res_b) = high[0];
-                            }
-                            *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hq_meta) + slot * 4u) = r | (lane << 16);
+                    for (int b = 0; b < B; ++b) {
+                        const uint32_t *wn = reinterpret_cast<const uint32_t *>(pn + 256 * b);
+                        w.en[b][0] = wn[0];
+                        w.en[b][1] = wn[1];
+                        if constexpr (decltype(leave)::value) {
+                            const uint32_t *wo = reinterpret_cast<const uint32_t *>(po + 256 * b);
+                            w.lv[b][0] = wo[0];
+                            w.lv[b][1] = wo[1];
                         }
-                        nq += (uint32_t)__builtin_popcountll(m);
-                        if constexpr (K == 0) {
-                            if (nq > a.hitq_cap - 64) {
-                                drain_hits<B>(a, R, q0, I.px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask);
-                                nq = 0;
-                            }
-                        }
-                    }
-                }
-            };
-            if constexpr (K > 0) {
-                // The column leaving at row r is the one that entered at row r - K, whose bits
-                // the lane computed then: a ring of the last K rows' entering words (slot
-                // r mod K; every item has at least K rows, and the loop runs in trips of K
-                // rows from row K on, so every slot index is a constant).
-                uint32_t ring[K][B];
-                // rows load their words as they go: the ring already holds 4 K words, and a
-                // prefetched row would cost 2 B more; the scheduling barrier after each row
-                // keeps the compiler from hoisting later rows' loads
-#pragma unroll
-                for (int s = 0; s < K; ++s) {
-                    Win w;
-                    load_win(xb[s], 0u, w, std::false_type{});
-                    enter_bits(s, w, ring[s]);
-                    step(s, ring[s], ring[s], std::false_type{});
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                auto trip_drain = [&] {
-                    if (nq > a.hitq_cap - 64 * K) {
-                        drain_hits<B>(a, R, q0, I.px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask);
-                        nq = 0;
                     }
                 };
-                trip_drain();
-                uint32_t r = K;
-                for (; r + K <= nrows; r += K) {
+                // entering mismatch bits of row r (px = px_start + r) from its loaded words
+                auto enter_bits = [&](uint32_t r, const Win &w, uint32_t *an) {
+                    const uint32_t shn = (px_start + r) & 31;
+#pragma unroll
+                    for (int b = 0; b < B; ++b) an[b] = __builtin_amdgcn_alignbit(w.en[b][1], w.en[b][0], shn);
+                };
+                // one row: window counters of row r from its entering bits (and the bits of the
+                // column leaving at r - k when LEAVE), then the row's hit flags into the queue
+                // (the ring form appends a whole trip of K rows before it drains: the queue then
+                // holds hitq_cap >= 64 (K + 2) entries, and it is drained once more than
+                // hitq_cap - 64 K are queued)
+                auto step = [&](uint32_t r, const uint32_t *an, const uint32_t *bo, auto leave) {
+                    const uint32_t px = px_start + r;
+                    uint32_t high[B];
+                    uint32_t all = ~0u;  // lanes' words with no hit in any block
+#pragma unroll
+                    for (int b = 0; b < B; ++b) {
+                        if constexpr (decltype(leave)::value) planes_slide<P>(cnt[b], an[b], bo[b]);
+                        else planes_inc<P>(cnt[b], an[b]);
+                        if constexpr (PH >= 0) {
+                            high[b] = planes_high<P, PH>(cnt[b]);
+                        } else {
+                            uint32_t h = 0;
+#pragma unroll
+                            for (int p = 0; p < P; ++p) h |= cnt[b][p] & hmask[p];
+                            high[b] = h;
+                        }
+                        all &= high[b];
+                    }
+                    if (px >= rep0 && interior) {
+                        if (all != ~0u) {
+                            uint32_t c = 0;  // set high-plane bits: the lane's cells with no hit
+#pragma unroll
+                            for (int b = 0; b < B; ++b) c += __builtin_popcount(high[b]);
+                            atomicAdd(acc_v + r, 32u * B - c);
+                        }
+                    } else if (px >= rep0) {
+                        const uint64_t m = __ballot(all != ~0u);
+                        if (m) {
+                            if (all != ~0u) {
+                                // the entry holds the high-plane words (clear bit = hit); 32-bit
+                                // byte offsets from the wave's (scalar) queue bases
+                                const uint32_t slot =
+                                    nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                                char *res_b = reinterpret_cast<char *>(hq_res) + slot * (4u * B);
+                                if constexpr (B == 4) {
+                                    *reinterpret_cast<uint4 *>(res_b) = make_uint4(high[0], high[1], high[2], high[3]);
+                                } else if constexpr (B == 2) {
+                                    *reinterpret_cast<uint2 *>(res_b) = make_uint2(high[0], high[1]);
+                                } else {
+                                    *reinterpret_cast<uint32_t *>(res_b) = high[0];
+                                }
+                                *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hq_meta) + slot * 4u) =
+                                    r | (lane << 16);
+                            }
+                            nq += (uint32_t)__builtin_popcountll(m);
+                            if constexpr (K == 0) {
+                                if (nq > a.hitq_cap - 64) {
+                                    drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts,
+                                                  acc_v, ymask, tb);
+                                    nq = 0;
+                                }
+                            }
+                        }
+                    }
+                };
+                if constexpr (K > 0) {
+                    // The column leaving at row r is the one that entered at row r - K, whose bits
+                    // the lane computed then: a ring of the last K rows' entering words (slot
+                    // r mod K; every item has at least K rows, and the loop runs in trips of K
+                    // rows from row K on, so every slot index is a constant).
+                    uint32_t ring[K][B];
+                    // rows load their words as they go: the ring already holds 4 K words, and a
+                    // prefetched row would cost 2 B more; the scheduling barrier after each row
+                    // keeps the compiler from hoisting later rows' loads
 #pragma unroll
                     for (int s = 0; s < K; ++s) {
                         Win w;
-                        load_win(xb[r + s], 0u, w, std::false_type{});
-                        uint32_t an[B];
-                        enter_bits(r + s, w, an);
-                        step(r + s, an, ring[s], std::true_type{});
-#pragma unroll
-                        for (int b = 0; b < B; ++b) ring[s][b] = an[b];
+                        load_win(xb[s], 0u, w, std::false_type{});
+                        enter_bits(s, w, ring[s]);
+                        step(s, ring[s], ring[s], std::false_type{});
                         __builtin_amdgcn_sched_barrier(0);
                     }
+                    auto trip_drain = [&] {
+                        if (nq > a.hitq_cap - 64 * K) {
+                            drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v,
+                                          ymask, tb);
+                            nq = 0;
+                        }
+                    };
                     trip_drain();
-                }
+                    uint32_t r = K;
+                    for (; r + K <= nrows; r += K) {
 #pragma unroll
-                for (int s = 0; s < K - 1; ++s) {
-                    if (r + s < nrows) {
-                        Win w;
-                        load_win(xb[r + s], 0u, w, std::false_type{});
-                        uint32_t an[B];
-                        enter_bits(r + s, w, an);
-                        step(r + s, an, ring[s], std::true_type{});
+                        for (int s = 0; s < K; ++s) {
+                            Win w;
+                            load_win(xb[r + s], 0u, w, std::false_type{});
+                            uint32_t an[B];
+                            enter_bits(r + s, w, an);
+                            step(r + s, an, ring[s], std::true_type{});
 #pragma unroll
-                        for (int b = 0; b < B; ++b) ring[s][b] = an[b];
+                            for (int b = 0; b < B; ++b) ring[s][b] = an[b];
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                        trip_drain();
                     }
-                }
-            } else {
-                // generic k: the leaving column is read again from the NE window
-                auto row = [&](uint32_t r, const Win &w, auto leave) {
-                    uint32_t an[B], bo[B];
-                    enter_bits(r, w, an);
-                    if constexpr (decltype(leave)::value) {
-                        const uint32_t sho = (I.px_start + r - a.k) & 31;
 #pragma unroll
-                        for (int b = 0; b < B; ++b) bo[b] = __builtin_amdgcn_alignbit(w.lv[b][1], w.lv[b][0], sho);
+                    for (int s = 0; s < K - 1; ++s) {
+                        if (r + s < nrows) {
+                            Win w;
+                            load_win(xb[r + s], 0u, w, std::false_type{});
+                            uint32_t an[B];
+                            enter_bits(r + s, w, an);
+                            step(r + s, an, ring[s], std::true_type{});
+#pragma unroll
+                            for (int b = 0; b < B; ++b) ring[s][b] = an[b];
+                        }
                     }
-                    step(r, an, bo, leave);
-                };
-                const uint32_t warm = nrows < a.k ? nrows : a.k;
-                // two rows per trip: the counter planes then stay in place across the back edge
-                // (one row per trip leaves register copies of the planes in the loop).  The row
-                // offsets are loaded a trip ahead, so a row waits for one LDS round trip.
-                uint32_t r = 0;
-                Win wa, wb;
-                for (; r + 1 < warm; r += 2) {
-                    load_win(xb[r], 0u, wa, std::false_type{});
-                    load_win(xb[r + 1], 0u, wb, std::false_type{});
-                    row(r, wa, std::false_type{});
-                    row(r + 1, wb, std::false_type{});
-                }
-                if (r < warm) {
-                    load_win(xb[r], 0u, wa, std::false_type{});
-                    row(r, wa, std::false_type{});
-                    ++r;
-                }
-                if (r < nrows) {
-                    uint32_t n0 = xb[r], n1 = xb[r + 1], o0 = xb[r - a.k], o1 = xb[r + 1 - a.k];
-                    for (; r + 1 < nrows; r += 2) {
-                        const uint32_t m0 = xb[r + 2], m1 = xb[r + 3];
-                        const uint32_t p0 = xb[r + 2 - a.k], p1 = xb[r + 3 - a.k];
+                } else {
+                    // generic k: the leaving column is read again from the NE window
+                    auto row = [&](uint32_t r, const Win &w, auto leave) {
+                        uint32_t an[B], bo[B];
+                        enter_bits(r, w, an);
+                        if constexpr (decltype(leave)::value) {
+                            const uint32_t sho = (px_start + r - a.k) & 31;
+#pragma unroll
+                            for (int b = 0; b < B; ++b) bo[b] = __builtin_amdgcn_alignbit(w.lv[b][1], w.lv[b][0], sho);
+                        }
+                        step(r, an, bo, leave);
+                    };
+                    const uint32_t warm = nrows < a.k ? nrows : a.k;
+                    // two rows per trip: the counter planes then stay in place across the back edge
+                    // (one row per trip leaves register copies of the planes in the loop).  The row
+                    // offsets are loaded a trip ahead, so a row waits for one LDS round trip.
+                    uint32_t r = 0;
+                    Win wa, wb;
+                    for (; r + 1 < warm; r += 2) {
+                        load_win(xb[r], 0u, wa, std::false_type{});
+                        load_win(xb[r + 1], 0u, wb, std::false_type{});
+                        row(r, wa, std::false_type{});
+                        row(r + 1, wb, std::false_type{});
+                    }
+                    if (r < warm) {
+                        load_win(xb[r], 0u, wa, std::false_type{});
+                        row(r, wa, std::false_type{});
+                        ++r;
+                    }
+                    if (r < nrows) {
+                        uint32_t n0 = xb[r], n1 = xb[r + 1], o0 = xb[r - a.k], o1 = xb[r + 1 - a.k];
+                        for (; r + 1 < nrows; r += 2) {
+                            const uint32_t m0 = xb[r + 2], m1 = xb[r + 3];
+                            const uint32_t p0 = xb[r + 2 - a.k], p1 = xb[r + 3 - a.k];
+                            load_win(n0, o0, wa, std::true_type{});
+                            row(r, wa, std::true_type{});
+                            load_win(n1, o1, wb, std::true_type{});
+                            row(r + 1, wb, std::true_type{});
+                            n0 = m0;
+                            n1 = m1;
+                            o0 = p0;
+                            o1 = p1;
+                        }
                         load_win(n0, o0, wa, std::true_type{});
-                        row(r, wa, std::true_type{});
-                        load_win(n1, o1, wb, std::true_type{});
-                        row(r + 1, wb, std::true_type{});
-                        n0 = m0;
-                        n1 = m1;
-                        o0 = p0;
-                        o1 = p1;
+                        if (r < nrows) row(r, wa, std::true_type{});
                     }
-                    load_win(n0, o0, wa, std::true_type{});
-                    if (r < nrows) row(r, wa, std::true_type{});
+                }
+                if (nq)
+                    drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask, tb);
+                __syncthreads();
+                // distinct sequences of this range per row
+                for (uint32_t r = lane; r < nrows; r += 64) {
+                    const uint32_t m = ymask[r];
+                    if (m) {
+                        if (I.split != kNoSplit) acc_u[r] = 1;
+                        else acc_u[r] += __builtin_popcount(m);
+                        ymask[r] = 0;
+                    }
                 }
             }
-            if (nq) drain_hits<B>(a, R, q0, I.px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask);
             __syncthreads();
-            // distinct sequences of this range per row
+            // flush the tile's rows; a long text split over several items counts once per row
+            // (every count of an item over a long sequence is that sequence's, so u is dv != 0)
             for (uint32_t r = lane; r < nrows; r += 64) {
-                const uint32_t m = ymask[r];
-                if (m) {
-                    if (I.split != kNoSplit) acc_u[r] = 1;
-                    else acc_u[r] += __builtin_popcount(m);
-                    ymask[r] = 0;
+                const uint32_t dv = acc_v[r];
+                if (!dv) continue;
+                uint32_t du = I.split != kNoSplit ? 1u : acc_u[r];
+                const uint32_t px = px_start + r;
+                if (I.split != kNoSplit && du) {
+                    const uint32_t xf = I.xf_base + px;
+                    const uint32_t bit = 1u << (xf & 31);
+                    const uint32_t old = atomicOr(a.split_flags + (size_t)I.split * a.split_words + (xf >> 5), bit);
+                    du = (old & bit) ? 0u : 1u;
                 }
+                if (du) atomicAdd(a.u + I.uv_base + px, du);
+                atomicAdd(a.v + I.uv_base + px, dv);
             }
-        }
-        __syncthreads();
-        // flush the item's rows; a long text split over several items counts once per row
-        // (every count of an item over a long sequence is that sequence's, so u is dv != 0)
-        for (uint32_t r = lane; r < nrows; r += 64) {
-            const uint32_t dv = acc_v[r];
-            if (!dv) continue;
-            uint32_t du = I.split != kNoSplit ? 1u : acc_u[r];
-            const uint32_t px = I.px_start + r;
-            if (I.split != kNoSplit && du) {
-                const uint32_t xf = I.xf_base + px;
-                const uint32_t bit = 1u << (xf & 31);
-                const uint32_t old = atomicOr(a.split_flags + (size_t)I.split * a.split_words + (xf >> 5), bit);
-                du = (old & bit) ? 0u : 1u;
-            }
-            if (du) atomicAdd(a.u + I.uv_base + px, du);
-            atomicAdd(a.v + I.uv_base + px, dv);
+        }  // row tiles
+        if (I.sym) {
+            __syncthreads();  // the drains' bitmap updates are visible
+            for (uint32_t ri = I.rb; ri < I.re; ++ri)
+                sym_flush(a, ri, a.ranges[ri].ys, tbits + (ri - I.rb) * 256);
         }
     }
 }

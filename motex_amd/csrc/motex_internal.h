@@ -16,6 +16,10 @@ constexpr uint32_t kMaxSliceK = 30;
 constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
 constexpr uint32_t kMaxRangeSeqs = 32;       // sequences per Hamming text range (one mask word)
 constexpr uint32_t kHitQueueCap = 512;       // Hamming hit queue entries per wave (global)
+// Text ranges per pair-symmetric Hamming item at most: each holds a 1 KB first-occurrence
+// bitmap in LDS.
+constexpr uint32_t kMaxSymGroup = 3;
+constexpr uint32_t kSymBitmapBits = 8192;    // a range's first-occurrence bitmap (256 words)
 constexpr uint32_t kSlicePipe = 4;           // waves per item of the pipelined text-parallel edit kernel
 // Edit kernel choice, from sweeps of 40-500 protein and DNA sequences of U[100,2000], k 8-14:
 // below kSliceItemsPerCu (x, group of 32 texts) items per CU the one-text-per-lane kernel
@@ -40,14 +44,15 @@ struct HdRange {
     uint32_t qoff;
 };
 
-// One unit of the Hamming work queue: rows [r0, r1) of sequence x against the text ranges
-// [rb, re), processed in order by one wave that keeps per-row u/v in LDS.
+// One unit of the Hamming work queue: rows [r0, r1) of sequence x, in row tiles of tile_rows,
+// against the text ranges [rb, re), processed in order by one wave that keeps per-row u/v in
+// LDS.  Items of the ordered plan hold one row tile; pair-symmetric items every row of x.
 struct HdItem {
     uint32_t x_start;   // text position of x's first symbol
     uint32_t uv_base;   // u/v index of x's pos_x = 0
     uint32_t xf_base;   // global factor index of (x, pos_x = 0): row flags for split texts
     uint32_t r0, r1;    // reported rows (positions in x)
-    uint32_t px_start;  // first computed row = max(0, r0 - k + 1)
+    uint32_t px_start;  // first computed row of the first tile = r0 - k + 1
     uint32_t rb, re;    // text ranges
     uint32_t split;     // kNoSplit (whole texts only) or the split slot of the one long text
                         // every range of the item is a part of
@@ -67,9 +72,6 @@ struct HdArgs {
     uint32_t *v;
     uint32_t *split_flags;     // split slots x split_words
     uint32_t *queue;           // work-queue head (one counter)
-    uint32_t *fbits;           // pair-symmetric runs: bit (xi, text position) = text position
-                               // has an occurrence of some factor of x_xi (f_words per x)
-    uint32_t f_words;
     uint32_t *hitq;            // per-workgroup hit queues: hitq_cap tags + hitq_cap x B words
     uint32_t hitq_cap;         // entries per hit queue (a multiple of 4, at least 128)
     uint32_t W;                // words per NE row
@@ -78,7 +80,12 @@ struct HdArgs {
     uint32_t k, e;
     uint32_t wl;               // LDS words per symbol row
     uint32_t split_words;
-    uint32_t rows_max;         // largest r1 - px_start of any item (LDS row arrays)
+    uint16_t *tlist;           // pair-symmetric runs: per range, first text-side occurrences
+    const uint32_t *toff;      //   (bitmap offsets pt - ys + 32) at tlist[toff[r] ..
+    uint32_t *tcnt;            //   toff[r + 1]), tcnt[r] of them appended
+    uint32_t rows_max;         // largest row tile + k - 1 (LDS row arrays)
+    uint32_t tile_rows;        // reported rows per row tile
+    uint32_t sym_group;        // ranges per pair-symmetric item at most (LDS text-side bitmaps)
     uint32_t persistent;       // 1: pull items from *queue; 0: item = blockIdx.x
 };
 

@@ -104,6 +104,7 @@ struct HdPlan {
     std::vector<HdItem> items;
     std::vector<HdRange> ranges;
     uint32_t blocks = 4, rows = 512, rows_max = 0, wl = 0, W = 0, split_slots = 0;
+    uint32_t sym_group = 0;  // pair-symmetric plans: ranges per item at most
 };
 
 struct Job;
@@ -156,8 +157,9 @@ struct motex_batch {
         if (ev2) (void)hipEventDestroy(ev2);
     }
     motex::DevBuf d_codes, d_seq_start, d_u, d_v, d_queue, d_ne, d_items, d_split, d_aux, d_aux2, d_ranges, d_hitq;
-    motex::DevBuf d_fbits;           // pair-symmetric Hamming runs: x-by-text occurrence bits
-    uint32_t f_words = 0;
+    // pair-symmetric Hamming runs: per range, the text side's first-occurrence list
+    // (d_tlist from d_toff[r]) and its fill count (d_tcnt)
+    motex::DevBuf d_tlist, d_toff, d_tcnt;
     bool ed_slice = false;           // edit runs on the text-parallel kernel
     bool ed_pipe = false;            // ... in its pipelined form (kSlicePipe waves per item)
     motex::DevBuf d_gplanes, d_goff; // its text groups
@@ -202,9 +204,9 @@ hipError_t launch_ed(const EdArgs &a, int width, uint32_t grid, hipStream_t s);
 uint32_t ed_strip_cols(int width);          // columns per strip of the one-text-per-lane kernel
 uint32_t ed_stream_row_words(int width);    // boundary-stream words per row
 hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_t s);
-hipError_t launch_fold_fbits(const uint32_t *fbits, uint32_t nx, uint32_t f_words, uint32_t total,
-                             uint32_t *u, hipStream_t s);
-size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max);
+size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max, uint32_t sym_group);
+hipError_t launch_sym_fold(const uint16_t *tlist, const uint32_t *toff, const uint32_t *tcnt, const HdRange *ranges,
+                           uint32_t nranges, uint32_t *u, uint32_t *v, hipStream_t s);
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max);
 
 // per-mode entry points (runtime.hip, ed_runtime.hip, struct_runtime.hip)

@@ -214,6 +214,9 @@ void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
     plan.rows = rows;
     plan.rows_max = rows + k - 1;
     plan.split_slots = 0;
+    // pair-symmetric items take every row of x against at most kMaxSymGroup ranges
+    if (sym) group_max = std::min(group_max, kMaxSymGroup);
+    plan.sym_group = sym ? group_max : 0;
     // text ranges: whole sequences packed up to cap positions, long ones split in parts
     std::vector<uint32_t> split_of;  // per range: kNoSplit or slot
     auto add_seq_ranges = [&](uint32_t j) {
@@ -272,37 +275,52 @@ void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
             while (r_first < plan.ranges.size() &&
                    plan.ranges[r_first].j0 + plan.ranges[r_first].ny - 1 < i)
                 ++r_first;
+        auto add = [&](uint32_t r0, uint32_t r1, uint32_t rb, uint32_t re, uint32_t split) {
+            HdItem it{};
+            it.x_start = ss.start[i];
+            it.uv_base = (uint32_t)ss.uv_off[i];
+            it.xf_base = (uint32_t)ss.uv_off[i];
+            it.r0 = r0;
+            it.r1 = r1;
+            it.px_start = r0 - (k - 1);
+            it.rb = rb;
+            it.re = re;
+            it.split = split;
+            it.xi = i;
+            it.sym = sym ? 1u : 0u;
+            plan.items.push_back(it);
+        };
+        if (sym) {
+            // every row of x against group_max ranges at a time, except the last sixth of its
+            // ranges, one at a time: sorted by cost, the queue then ends on small items (the
+            // rows are flushed once per item, so the large ones keep the flushes few; 1200 x
+            // 1000 nt: 8.36 ms and 0.44 GiB written, against 8.55 ms and 0.73 GiB with every
+            // item one range and 8.85 ms and 0.39 GiB with every item three)
+            const uint32_t nr = (uint32_t)plan.ranges.size();
+            const uint32_t tail = (nr - r_first + 5) / 6;
+            uint32_t rb = r_first;
+            for (; rb + group_max <= nr - tail; rb += group_max) add(k - 1, m, rb, rb + group_max, kNoSplit);
+            for (; rb < nr; ++rb) add(k - 1, m, rb, rb + 1, kNoSplit);
+            continue;
+        }
         for (uint32_t r0 = k - 1; r0 < m; r0 += rows) {
-            uint32_t r1 = std::min(m, r0 + rows);
-            for (const Group &g : groups) {
-                if (g.re <= r_first) continue;
-                HdItem it{};
-                it.x_start = ss.start[i];
-                it.uv_base = (uint32_t)ss.uv_off[i];
-                it.xf_base = (uint32_t)ss.uv_off[i];
-                it.r0 = r0;
-                it.r1 = r1;
-                it.px_start = r0 - (k - 1);
-                it.rb = std::max(g.rb, r_first);
-                it.re = g.re;
-                it.split = g.split;
-                it.xi = i;
-                it.sym = sym ? 1u : 0u;
-                plan.items.push_back(it);
-            }
+            const uint32_t r1 = std::min(m, r0 + rows);
+            for (const Group &g : groups)
+                if (g.re > r_first) add(r0, r1, std::max(g.rb, r_first), g.re, g.split);
         }
     }
 }
 
-// First NE word an (item, range) reads; mirrors the kernel.
-static int64_t hd_wbase(const HdItem &it, const HdRange &R)
+// First NE word a row tile [r0, r1) of an item reads for a range; mirrors the kernel.
+static int64_t hd_wbase(const HdItem &it, uint32_t r0, uint32_t r1, const HdRange &R)
 {
-    int64_t q0 = ((int64_t)R.ys - (int64_t)it.r1 + 1);
+    (void)it;
+    int64_t q0 = ((int64_t)R.ys - (int64_t)r1 + 1);
     q0 = q0 >= 0 ? (q0 & ~31LL) : -(((-q0) + 31) & ~31LL);
     q0 += R.qoff;
     int64_t qp = q0 + (int64_t)kPadBits;
     int64_t w = qp >= 0 ? qp / 32 : -((-qp + 31) / 32);
-    return w + (it.px_start >> 5);
+    return w + ((r0 - (it.r0 - it.px_start)) >> 5);
 }
 
 // Words per NE table row that every window the plan's items read fits in.
@@ -310,11 +328,12 @@ int hd_table_words(const SeqSet &ss, const HdPlan &plan, uint64_t *Wout)
 {
     uint64_t W = (ss.total + kPadBits) / 32 + 2;
     for (const HdItem &it : plan.items)
-        for (uint32_t r = it.rb; r < it.re; ++r) {
-            int64_t wb = hd_wbase(it, plan.ranges[r]);
-            if (wb < 0) return set_error(MOTEX_EINVAL, "internal: negative table window");
-            W = std::max<uint64_t>(W, (uint64_t)wb + plan.wl + 1);
-        }
+        for (uint32_t t0 = it.r0; t0 < it.r1; t0 += plan.rows)
+            for (uint32_t r = it.rb; r < it.re; ++r) {
+                int64_t wb = hd_wbase(it, t0, std::min(it.r1, t0 + plan.rows), plan.ranges[r]);
+                if (wb < 0) return set_error(MOTEX_EINVAL, "internal: negative table window");
+                W = std::max<uint64_t>(W, (uint64_t)wb + plan.wl + 1);
+            }
     *Wout = W;
     return MOTEX_OK;
 }
@@ -412,29 +431,52 @@ int batch_upload_common(motex_batch *b)
 // Pair symmetry of the Hamming path: the occurrences of ordered pair (j, i) are those of
 // (i, j) with the roles of factor and text position swapped, so each unordered pair can be
 // computed once and credited to both sides.  The text side's u needs, per (x sequence,
-// text position), whether any factor of x occurs there: one bit each (d_fbits), folded into
-// u by hd_fold_fbits after the kernel.  Worth it when occurrences are rare (each costs two
-// global atomics on the text side), the bitmap is small and no text is split across work
-// items.  MOTEX_SYM=0 / 1 forces it off / on (where allowed).
-static bool hd_symmetric(motex_batch *b, const HdPlan &plan)
+// text position), whether any factor of x occurs there: the pair-symmetric items take every
+// row of x, so a first-occurrence bitmap per range in LDS answers it.  Worth it when
+// occurrences are rare (each costs global atomics on the text side) and no text is split
+// across work items.  MOTEX_SYM=0 / 1 forces it off / on (where allowed).
+// Expected occurrences per cell, from the symbol frequencies of the text (letters drawn
+// independently).
+static double hd_cell_hit_rate(const SeqSet &ss, uint32_t k, uint32_t e)
 {
-    const SeqSet &ss = b->ss;
-    if (plan.split_slots || b->nb_structs != 1 || ss.num < 2 || ss.num > 65535 || ss.nx != ss.num) return false;
-    const uint64_t f_words = (ss.total + 31) / 32 + 1;
-    if ((uint64_t)ss.num * f_words * 4 > (4ull << 30)) return false;
-    if (const char *env = getenv("MOTEX_SYM")) return env[0] == '1';
-    // expected occurrences per cell from the symbol frequencies of the text
     std::vector<uint64_t> freq(ss.sigma, 0);
     for (uint8_t c : ss.codes) freq[c]++;
     double pm = 0;
     for (uint64_t f : freq) pm += ((double)f / ss.total) * ((double)f / ss.total);
-    const uint32_t k = b->params.k, e = b->params.e;
     double p = 0, binom = 1;
-    for (uint32_t d = 0; d <= e; ++d) {
+    for (uint32_t d = 0; d <= e && d <= k; ++d) {
         p += binom * std::pow(1 - pm, d) * std::pow(pm, k - d);
         binom = binom * (k - d) / (d + 1);
     }
-    return p < 1e-4;
+    return p;
+}
+
+static bool hd_symmetric(motex_batch *b, const HdPlan &plan)
+{
+    const SeqSet &ss = b->ss;
+    if (plan.split_slots || b->nb_structs != 1 || ss.num < 2 || ss.nx != ss.num) return false;
+    if (const char *env = getenv("MOTEX_SYM")) return env[0] == '1';
+    return hd_cell_hit_rate(ss, b->params.k, b->params.e) < 1e-4;
+}
+
+// Capacity of every range's first-occurrence list (pair-symmetric runs): three times the
+// expected first occurrences of the x sequences before its last text (rows x span x the
+// cell rate), plus one full bitmap; a list that fills up takes the rest as atomics.
+static std::vector<uint32_t> hd_sym_list_offsets(const SeqSet &ss, const HdPlan &plan, uint32_t k, uint32_t e)
+{
+    const double p = hd_cell_hit_rate(ss, k, e);
+    std::vector<uint64_t> rows_upto(ss.num + 1, 0);  // rows of sequences 0 .. j-1
+    for (uint32_t j = 0; j < ss.num; ++j) rows_upto[j + 1] = rows_upto[j] + (ss.len[j] >= k ? ss.len[j] - k + 1 : 0);
+    std::vector<uint32_t> off(plan.ranges.size() + 1, 0);
+    uint64_t total = 0;
+    for (size_t r = 0; r < plan.ranges.size(); ++r) {
+        const HdRange &R = plan.ranges[r];
+        const double rows = (double)rows_upto[R.j0 + R.ny - 1];
+        const double cap = 3.0 * p * rows * (R.ye - R.ys) + kSymBitmapBits;
+        total += ((uint64_t)std::min(cap, (double)(1u << 28)) + 7) & ~7ull;  // lists start at multiples of 8
+        off[r + 1] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFF8u);
+    }
+    return off;
 }
 
 int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const TileShape &ts,
@@ -448,12 +490,12 @@ int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const Ti
     const bool sym = want_sym(plan);
     tr.mark("symmetry estimate");
     // group size: aim at ~16 items per resident wave so the tail of the queue is short
-    uint64_t pairs_units = 0;  // tiles x ranges
+    uint64_t pairs_units = 0;  // items' row tiles x ranges (pair-symmetric items hold all tiles)
     {
         size_t r_first = 0;
         for (uint32_t i = 0; i < ss.nx; ++i) {
             if (ss.len[i] < p.k) continue;
-            uint64_t ntiles = (ss.len[i] - p.k + 1 + ts.rows - 1) / ts.rows;
+            uint64_t ntiles = sym ? 1 : (ss.len[i] - p.k + 1 + ts.rows - 1) / ts.rows;
             if (sym)
                 while (r_first < plan.ranges.size() &&
                        plan.ranges[r_first].j0 + plan.ranges[r_first].ny - 1 < i)
@@ -461,7 +503,9 @@ int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const Ti
             pairs_units += ntiles * (plan.ranges.size() - r_first);
         }
     }
-    uint64_t target_items = (uint64_t)c->num_cu * ts.waves_per_cu * 16;
+    // (a pair-symmetric item holds every row tile of x, several of the ordered plan's items:
+    // ~3 per wave keep the schedule's tail short, and its rows are flushed once per item)
+    uint64_t target_items = (uint64_t)c->num_cu * ts.waves_per_cu * (sym ? 3 : 16);
     uint32_t gmax = (uint32_t)std::max<uint64_t>(1, pairs_units / std::max<uint64_t>(target_items, 1));
     plan_hd_items(ss, p.k, ts.blocks, ts.rows, xs, nullptr, plan, gmax, sym);
     tr.mark("items");
@@ -472,7 +516,9 @@ int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const Ti
         std::vector<uint64_t> cost(plan.items.size());
         for (size_t t = 0; t < cost.size(); ++t) {
             const HdItem &x = plan.items[t];
-            cost[t] = (uint64_t)(x.r1 - x.px_start) * (span[x.re] - span[x.rb] + (x.re - x.rb) * (x.r1 - x.r0));
+            const uint64_t tiles = (x.r1 - x.r0 + plan.rows - 1) / plan.rows;
+            const uint64_t rows = (x.r1 - x.r0) + tiles * (x.r0 - x.px_start);
+            cost[t] = rows * (span[x.re] - span[x.rb] + (x.re - x.rb) * std::min<uint64_t>(x.r1 - x.r0, plan.rows));
         }
         const std::vector<uint32_t> order = order_by_cost_desc(cost);
         std::vector<HdItem> sorted(order.size());
@@ -480,7 +526,7 @@ int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const Ti
         plan.items.swap(sorted);
     }
     plan.wl = hd_lds_words(ts.blocks, plan.rows_max);
-    if (hd_lds_bytes(ss.sigma, plan.wl, plan.rows_max) > 65536 || (uint64_t)ss.sigma * plan.wl >= 65536)
+    if (hd_lds_bytes(ss.sigma, plan.wl, plan.rows_max, plan.sym_group) > 65536 || (uint64_t)ss.sigma * plan.wl >= 65536)
         return set_error(MOTEX_EUNSUPPORTED, "alphabet of %u symbols does not fit the kernel's LDS window", ss.sigma);
     tr.mark("sort");
     *sym_out = sym;
@@ -520,8 +566,14 @@ int hd_prepare(motex_batch *b)
     b->stats.symmetric = sym ? 1u : 0u;
     b->stats.swept_cells = hd_swept_cells(b->hd);
     if (sym) {
-        b->f_words = (uint32_t)((ss.total + 31) / 32 + 1);
-        if ((e = b->d_fbits.alloc((uint64_t)ss.num * b->f_words * 4)) != hipSuccess) return hip_fail(e, "alloc text bits");
+        const std::vector<uint32_t> off = hd_sym_list_offsets(ss, b->hd, b->params.k, b->params.e);
+        if ((e = b->d_toff.alloc(off.size() * 4)) != hipSuccess) return hip_fail(e, "alloc text-side list offsets");
+        if ((e = hipMemcpyAsync(b->d_toff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, bstream(b))) != hipSuccess)
+            return hip_fail(e, "copy text-side list offsets");
+        if ((e = b->d_tlist.alloc(std::max<uint64_t>(off.back(), 1) * 2)) != hipSuccess)
+            return hip_fail(e, "alloc text-side lists");
+        if ((e = b->d_tcnt.alloc(std::max<size_t>(b->hd.ranges.size(), 1) * 4)) != hipSuccess)
+            return hip_fail(e, "alloc text-side counts");
     }
     b->split_words = (uint32_t)((ss.uv_off[ss.num] + 31) / 32 + 1);
     uint64_t sf = (uint64_t)std::max<uint32_t>(b->hd.split_slots, 1) * b->split_words;
@@ -535,8 +587,11 @@ int hd_prepare(motex_batch *b)
 double hd_swept_cells(const HdPlan &plan)
 {
     double s = 0;
-    for (const HdItem &it : plan.items)
-        s += (double)(it.r1 - it.px_start) * (it.re - it.rb) * kDiagPerBlock * plan.blocks;
+    for (const HdItem &it : plan.items) {
+        const uint64_t tiles = (it.r1 - it.r0 + plan.rows - 1) / plan.rows;
+        const uint64_t rows = (it.r1 - it.r0) + tiles * (it.r0 - it.px_start);
+        s += (double)rows * (it.re - it.rb) * kDiagPerBlock * plan.blocks;
+    }
     return s;
 }
 
@@ -561,8 +616,6 @@ HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t
     a.queue = (uint32_t *)b->d_queue.p;
     a.hitq = (uint32_t *)b->d_hitq.p;
     a.hitq_cap = hd_hitq_cap(b->params.k, b->params.e, 4);
-    a.fbits = (uint32_t *)b->d_fbits.p;
-    a.f_words = b->f_words;
     a.W = b->hd.W;
     a.nitems = (uint32_t)plan.items.size();
     a.sigma = b->ss.sigma;
@@ -571,6 +624,11 @@ HdArgs hd_args(motex_batch *b, const HdPlan &plan, const DevBuf &items, uint32_t
     a.wl = plan.wl;
     a.split_words = b->split_words;
     a.rows_max = plan.rows_max;
+    a.tile_rows = plan.rows;
+    a.sym_group = plan.sym_group;
+    a.tlist = (uint16_t *)b->d_tlist.p;
+    a.toff = (const uint32_t *)b->d_toff.p;
+    a.tcnt = (uint32_t *)b->d_tcnt.p;
     a.persistent = persistent;
     return a;
 }
@@ -585,8 +643,8 @@ int batch_clear(motex_batch *b)
     if (b->d_split.p && (e = launch_fill_u32((uint32_t *)b->d_split.p, b->d_split.bytes / 4, 0, s)) != hipSuccess)
         return hip_fail(e, "clear flags");
     if ((e = launch_fill_u32((uint32_t *)b->d_queue.p, 64, 0, s)) != hipSuccess) return hip_fail(e, "clear queue");
-    if (b->d_fbits.p && (e = launch_fill_u32((uint32_t *)b->d_fbits.p, b->d_fbits.bytes / 4, 0, s)) != hipSuccess)
-        return hip_fail(e, "clear text bits");
+    if (b->d_tcnt.p && (e = launch_fill_u32((uint32_t *)b->d_tcnt.p, b->d_tcnt.bytes / 4, 0, s)) != hipSuccess)
+        return hip_fail(e, "clear text-side counts");
     return MOTEX_OK;
 }
 
@@ -602,10 +660,9 @@ int hd_launch(motex_batch *b)
         if ((uint64_t)grid * a.hitq_cap * (b->tile.blocks + 1) * 4 > b->d_hitq.bytes)
             return set_error(MOTEX_EINVAL, "internal: Hamming grid of %u beyond its hit queues", grid);
         if ((e = launch_hd(a, b->tile.blocks, grid, s)) != hipSuccess) return hip_fail(e, "launch Hamming kernel");
-        if (b->d_fbits.p &&
-            (e = launch_fold_fbits((const uint32_t *)b->d_fbits.p, b->ss.num, b->f_words, (uint32_t)b->ss.total,
-                                   (uint32_t *)b->d_u.p, s)) != hipSuccess)
-            return hip_fail(e, "fold text bits");
+        if (b->hd.sym_group &&
+            (e = launch_sym_fold(a.tlist, a.toff, a.tcnt, a.ranges, (uint32_t)b->hd.ranges.size(), a.u, a.v, s)) != hipSuccess)
+            return hip_fail(e, "fold text side");
     }
     return MOTEX_OK;
 }
@@ -753,6 +810,7 @@ int hd_launch_pairs(motex_batch *b, const std::vector<std::pair<uint32_t, uint32
     if ((e = hd_alloc_hitq(hitq, max_count, 4, hd_hitq_cap(k, b->params.e, 4))) != hipSuccess)
         return hip_fail(e, "alloc plain hit queues");
     HdPlan shape;
+    shape.rows = kPlainRows;
     shape.rows_max = all.rows_max;
     if ((e = hipEventRecord(b->ev0, s)) != hipSuccess) return hip_fail(e, "record plain start");
     for (const Launch &L : launches) {

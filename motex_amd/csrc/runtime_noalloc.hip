@@ -142,8 +142,6 @@ static int noalloc_run(motex_ctx *c, uint8_t *base, const char *const *seqs, con
     a.v = (uint32_t *)(base + L.v);
     a.split_flags = (uint32_t *)(base + L.split);
     a.queue = (uint32_t *)(base + L.queue);
-    a.fbits = nullptr;
-    a.f_words = 0;
     a.hitq = (uint32_t *)(base + L.hitq);
     a.hitq_cap = kNoallocHitqCap;
     a.W = (uint32_t)W;
@@ -154,6 +152,8 @@ static int noalloc_run(motex_ctx *c, uint8_t *base, const char *const *seqs, con
     a.wl = plan.wl;
     a.split_words = split_words;
     a.rows_max = plan.rows_max;
+    a.tile_rows = plan.rows;
+    a.sym_group = 0;  // ordered plan
     a.persistent = 1;
     if ((e = launch_hd(a, plan.blocks, (uint32_t)grid, s)) != hipSuccess) return hip_fail(e, "launch Hamming kernel");
     std::vector<uint32_t> hu(uvn), hv(uvn);

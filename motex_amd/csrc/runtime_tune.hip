@@ -72,7 +72,7 @@ static bool hd_fits(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows
 {
     const uint32_t rows_max = rows + k;
     const uint32_t wl = hd_lds_words(blocks, rows_max);
-    return hd_lds_bytes(ss.sigma, wl, rows_max) <= 65536 && (uint64_t)ss.sigma * wl < 65536;
+    return hd_lds_bytes(ss.sigma, wl, rows_max, kMaxSymGroup) <= 65536 && (uint64_t)ss.sigma * wl < 65536;
 }
 
 TileShape choose_tile(motex_ctx *c, const motex_params &p, const SeqSet &ss)
