@@ -55,15 +55,22 @@ __device__ __forceinline__ void planes_inc(uint32_t *c, uint32_t t)
 // the entering bit itself, so one carry chain serves both: an increment carries out of a
 // plane that was 1, a decrement borrows out of a plane that was 0, i.e. the chain continues
 // where the old plane equals the direction bit.
+//
+// Written as 3-input logic ops (v_bitop3_b32; the table's bit a*4 + b*2 + c is f(a, b, c)),
+// two per plane: plane 0 takes enter and leave directly (its new bit is c ^ enter ^ leave,
+// table 0x96, and its carry (enter ^ leave) & ~(c ^ enter), table 0x42), every later plane
+// XORs the carry in and passes on carry & ~(c ^ enter) (table 0x90).
 template <int P>
 __device__ __forceinline__ void planes_slide(uint32_t *c, uint32_t enter, uint32_t leave)
 {
-    uint32_t t = enter ^ leave;
+    const uint32_t c0 = c[0];
+    c[0] = __builtin_amdgcn_bitop3_b32(c0, enter, leave, 0x96);
+    uint32_t t = __builtin_amdgcn_bitop3_b32(c0, enter, leave, 0x42);
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
+    for (int p = 1; p < P; ++p) {
         const uint32_t cp = c[p];
         c[p] = cp ^ t;
-        if (p + 1 < P) t &= ~(cp ^ enter);
+        if (p + 1 < P) t = __builtin_amdgcn_bitop3_b32(t, cp, enter, 0x90);
     }
 }
 
@@ -76,6 +83,22 @@ __device__ __forceinline__ uint32_t planes_high(const uint32_t *c)
 #pragma unroll
     for (int p = PH + 1; p < P; ++p) h |= c[p];
     return h;
+}
+
+// acc & planes_high(c): the last two high planes and acc in one 3-input op ((a | b) & c,
+// table 0xA8).
+template <int P, int PH>
+__device__ __forceinline__ uint32_t planes_high_and(const uint32_t *c, uint32_t acc)
+{
+    static_assert(PH < P, "the counter needs a plane above the threshold");
+    if constexpr (P - PH == 1) {
+        return acc & c[PH];
+    } else {
+        uint32_t h = c[PH];
+#pragma unroll
+        for (int p = PH + 1; p < P - 1; ++p) h |= c[p];
+        return __builtin_amdgcn_bitop3_b32(h, c[P - 1], acc, 0xA8);
+    }
 }
 
 // Masks selecting planes ph..P-1 (wave-uniform), for planes_high_masked: the threshold

@@ -322,24 +322,37 @@ void hd_batched_kernel(HdArgs a)
                 // hitq_cap - 64 K are queued)
                 auto step = [&](uint32_t r, const uint32_t *an, const uint32_t *bo, auto leave) {
                     const uint32_t px = px_start + r;
-                    uint32_t high[B];
                     uint32_t all = ~0u;  // lanes' words with no hit in any block
 #pragma unroll
                     for (int b = 0; b < B; ++b) {
                         if constexpr (decltype(leave)::value) planes_slide<P>(cnt[b], an[b], bo[b]);
                         else planes_inc<P>(cnt[b], an[b]);
-                        if constexpr (PH >= 0) {
-                            high[b] = planes_high<P, PH>(cnt[b]);
-                        } else {
-                            uint32_t h = 0;
+                        if constexpr (PH >= 0) all = planes_high_and<P, PH>(cnt[b], all);
+                    }
+                    // the blocks' high-plane words (clear bit = hit), needed on rows with hits
+                    auto highs = [&](uint32_t *high) {
 #pragma unroll
-                            for (int p = 0; p < P; ++p) h |= cnt[b][p] & hmask[p];
-                            high[b] = h;
+                        for (int b = 0; b < B; ++b) {
+                            if constexpr (PH >= 0) {
+                                high[b] = planes_high<P, PH>(cnt[b]);
+                            } else {
+                                uint32_t h = 0;
+#pragma unroll
+                                for (int p = 0; p < P; ++p) h |= cnt[b][p] & hmask[p];
+                                high[b] = h;
+                            }
                         }
-                        all &= high[b];
+                    };
+                    if constexpr (PH < 0) {
+                        uint32_t high[B];
+                        highs(high);
+#pragma unroll
+                        for (int b = 0; b < B; ++b) all &= high[b];
                     }
                     if (px >= rep0 && interior) {
                         if (all != ~0u) {
+                            uint32_t high[B];
+                            highs(high);
                             uint32_t c = 0;  // set high-plane bits: the lane's cells with no hit
 #pragma unroll
                             for (int b = 0; b < B; ++b) c += __builtin_popcount(high[b]);
@@ -349,6 +362,8 @@ void hd_batched_kernel(HdArgs a)
                         const uint64_t m = __ballot(all != ~0u);
                         if (m) {
                             if (all != ~0u) {
+                                uint32_t high[B];
+                                highs(high);
                                 // the entry holds the high-plane words (clear bit = hit); 32-bit
                                 // byte offsets from the wave's (scalar) queue bases
                                 const uint32_t slot =
