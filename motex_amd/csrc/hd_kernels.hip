@@ -98,7 +98,7 @@ bool hd_has_ring(uint32_t k, uint32_t e)
 
 uint32_t hd_hitq_cap(uint32_t k, uint32_t e, uint32_t blocks)
 {
-    return blocks == 4 && hd_has_ring(k, e) ? std::max<uint32_t>(kHitQueueCap, 64 * (k + 2)) : kHitQueueCap;
+    return blocks == 4 && hd_has_ring(k, e) ? std::max<uint32_t>(kHitQueueCap, 64 * (2 * k + 2)) : kHitQueueCap;
 }
 
 template <int B, int P>
@@ -179,7 +179,7 @@ hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_
     case 2: return launch_p<2>(a, grid, lds, s);
     default:
         // every work item has at least k rows (plan_hd_items), as the ring kernels assume
-        if (hd_has_ring(a.k, a.e) && a.hitq_cap >= 64 * (a.k + 2)) {
+        if (hd_has_ring(a.k, a.e) && a.hitq_cap >= 64 * (2 * a.k + 2)) {
             switch (a.k) {
             case 8: case 9: return launch_hd_ring_k8(a, grid, lds, s);
             case 10: case 11: return launch_hd_ring_k10(a, grid, lds, s);

@@ -317,9 +317,9 @@ void hd_batched_kernel(HdArgs a)
                 };
                 // one row: window counters of row r from its entering bits (and the bits of the
                 // column leaving at r - k when LEAVE), then the row's hit flags into the queue
-                // (the ring form appends a whole trip of K rows before it drains: the queue then
-                // holds hitq_cap >= 64 (K + 2) entries, and it is drained once more than
-                // hitq_cap - 64 K are queued)
+                // (the ring form appends a whole trip of up to 2 K rows before it drains: the
+                // queue then holds hitq_cap >= 64 (2 K + 2) entries, and it is drained once more
+                // than hitq_cap - 64 x trip rows are queued)
                 auto step = [&](uint32_t r, const uint32_t *an, const uint32_t *bo, auto leave) {
                     const uint32_t px = px_start + r;
                     uint32_t all = ~0u;  // lanes' words with no hit in any block
@@ -397,52 +397,54 @@ void hd_batched_kernel(HdArgs a)
                     // r mod K; every item has at least K rows, and the loop runs in trips of K
                     // rows from row K on, so every slot index is a constant).
                     uint32_t ring[K][B];
-                    // rows load their words as they go: the ring already holds 4 K words, and a
-                    // prefetched row would cost 2 B more; the scheduling barrier after each row
-                    // keeps the compiler from hoisting later rows' loads
+                    // Rows are software-pipelined: while row r computes, the words of row r + 1
+                    // load into the other buffer (row r uses buffer (r + K) & 1, so that trips of
+                    // TRIP rows -- K, or 2 K for odd K -- start on buffer 0) and the offset of
+                    // row r + 2 loads from LDS.  The scheduling barrier after each row keeps the
+                    // compiler from hoisting later rows' loads (a register per word and row).
+                    // xb has two spare entries past the last row.
+                    constexpr int TRIP = (K % 2 == 0) ? K : 2 * K;
+                    Win wbuf[2];
+                    load_win(xb[0], 0u, wbuf[K & 1], std::false_type{});
+                    uint32_t x1 = xb[1];
 #pragma unroll
                     for (int s = 0; s < K; ++s) {
-                        Win w;
-                        load_win(xb[s], 0u, w, std::false_type{});
-                        enter_bits(s, w, ring[s]);
+                        load_win(x1, 0u, wbuf[(s + 1 + K) & 1], std::false_type{});
+                        x1 = xb[s + 2];
+                        enter_bits(s, wbuf[(s + K) & 1], ring[s]);
                         step(s, ring[s], ring[s], std::false_type{});
                         __builtin_amdgcn_sched_barrier(0);
                     }
                     auto trip_drain = [&] {
-                        if (nq > a.hitq_cap - 64 * K) {
+                        if (nq > a.hitq_cap - 64 * TRIP) {
                             drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v,
                                           ymask, tb);
                             nq = 0;
                         }
                     };
                     trip_drain();
+                    // (the ring's slot for row r is r mod K: trips start at multiples of K)
+                    auto ring_row = [&](uint32_t r, int s) {
+                        load_win(x1, 0u, wbuf[(s + 1) & 1], std::false_type{});
+                        x1 = xb[r + 2];
+                        uint32_t an[B];
+                        enter_bits(r, wbuf[s & 1], an);
+                        step(r, an, ring[s % K], std::true_type{});
+#pragma unroll
+                        for (int b = 0; b < B; ++b) ring[s % K][b] = an[b];
+                    };
                     uint32_t r = K;
-                    for (; r + K <= nrows; r += K) {
+                    for (; r + TRIP <= nrows; r += TRIP) {
 #pragma unroll
-                        for (int s = 0; s < K; ++s) {
-                            Win w;
-                            load_win(xb[r + s], 0u, w, std::false_type{});
-                            uint32_t an[B];
-                            enter_bits(r + s, w, an);
-                            step(r + s, an, ring[s], std::true_type{});
-#pragma unroll
-                            for (int b = 0; b < B; ++b) ring[s][b] = an[b];
+                        for (int s = 0; s < TRIP; ++s) {
+                            ring_row(r + s, s);
                             __builtin_amdgcn_sched_barrier(0);
                         }
                         trip_drain();
                     }
 #pragma unroll
-                    for (int s = 0; s < K - 1; ++s) {
-                        if (r + s < nrows) {
-                            Win w;
-                            load_win(xb[r + s], 0u, w, std::false_type{});
-                            uint32_t an[B];
-                            enter_bits(r + s, w, an);
-                            step(r + s, an, ring[s], std::true_type{});
-#pragma unroll
-                            for (int b = 0; b < B; ++b) ring[s][b] = an[b];
-                        }
-                    }
+                    for (int s = 0; s < TRIP - 1; ++s)
+                        if (r + s < nrows) ring_row(r + s, s);
                 } else {
                     // generic k: the leaving column is read again from the NE window
                     auto row = [&](uint32_t r, const Win &w, auto leave) {
