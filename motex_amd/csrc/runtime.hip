@@ -292,15 +292,18 @@ void plan_hd_items(const SeqSet &ss, uint32_t k, uint32_t blocks, uint32_t rows,
         };
         if (sym) {
             // every row of x against group_max ranges at a time, except the last sixth of its
-            // ranges, one at a time: sorted by cost, the queue then ends on small items (the
+            // ranges, two at a time: sorted by cost, the queue then ends on small items (the
             // rows are flushed once per item, so the large ones keep the flushes few; 1200 x
-            // 1000 nt: 8.36 ms and 0.44 GiB written, against 8.55 ms and 0.73 GiB with every
-            // item one range and 8.85 ms and 0.39 GiB with every item three)
+            // 1000 nt, at an earlier kernel: 8.36 ms and 0.44 GiB written with a tail of single
+            // ranges, against 8.55 ms and 0.73 GiB with every item one range and 8.85 ms and
+            // 0.39 GiB with every item three; pairs in the tail write 10% less at the same time)
             const uint32_t nr = (uint32_t)plan.ranges.size();
             const uint32_t tail = (nr - r_first + 5) / 6;
             uint32_t rb = r_first;
             for (; rb + group_max <= nr - tail; rb += group_max) add(k - 1, m, rb, rb + group_max, kNoSplit);
-            for (; rb < nr; ++rb) add(k - 1, m, rb, rb + 1, kNoSplit);
+            // (at most group_max ranges per item: the kernel holds that many bitmaps)
+            const uint32_t g_tail = std::min<uint32_t>(2, group_max);
+            for (; rb < nr; rb += g_tail) add(k - 1, m, rb, std::min(nr, rb + g_tail), kNoSplit);
             continue;
         }
         for (uint32_t r0 = k - 1; r0 < m; r0 += rows) {
@@ -508,6 +511,10 @@ int hd_make_plan(motex_ctx *c, const SeqSet &ss, const motex_params &p, const Ti
     uint64_t target_items = (uint64_t)c->num_cu * ts.waves_per_cu * (sym ? 3 : 16);
     uint32_t gmax = (uint32_t)std::max<uint64_t>(1, pairs_units / std::max<uint64_t>(target_items, 1));
     plan_hd_items(ss, p.k, ts.blocks, ts.rows, xs, nullptr, plan, gmax, sym);
+    for (const HdItem &it : plan.items)  // the kernel holds sym_group first-occurrence bitmaps
+        if (it.sym && it.re - it.rb > plan.sym_group)
+            return set_error(MOTEX_EINVAL, "internal: pair-symmetric item of %u ranges beyond %u bitmaps", it.re - it.rb,
+                             plan.sym_group);
     tr.mark("items");
     // longest items first: they set the tail of the persistent schedule
     std::vector<uint64_t> span(plan.ranges.size() + 1, 0);
