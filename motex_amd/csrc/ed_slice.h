@@ -29,7 +29,17 @@
 
 namespace motex {
 
-constexpr int kSlicePc = 5;  // popcount planes (counts up to 31)
+constexpr int kSlicePc = 5;  // popcount planes held (counts up to 31)
+
+// Popcount planes a factor length K needs: a selected word's popcount is at most K (the left
+// candidate holds it there, and the host check of this file asserts it), a candidate's at
+// most K + 1, so bits(K + 1) planes carry them; the planes above stay zero and are not
+// computed.
+template <int K>
+constexpr int slice_pcq()
+{
+    return K + 1 < 16 ? 4 : 5;
+}
 
 // Truth tables of the 3-input logic op (v_bitop3_b32: result bit = table bit number
 // 4 src0 + 2 src1 + src2) used below; the host check of this file evaluates them against the
@@ -111,13 +121,15 @@ MOTEX_HD uint32_t slice_andnot_unless(uint32_t a, uint32_t h, uint32_t s)
 #endif
 }
 
-// out = in + carry (carry: one 0/1 per text)
+// out = in + carry (carry: one 0/1 per text) over the low NQ planes
+template <int NQ = kSlicePc>
 MOTEX_HD void slice_inc(const uint32_t in[kSlicePc], uint32_t carry, uint32_t out[kSlicePc])
 {
-    for (int q = 0; q < kSlicePc; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         out[q] = in[q] ^ carry;
         carry = in[q] & carry;
     }
+    for (int q = NQ; q < kSlicePc; ++q) out[q] = 0;
 }
 
 // Bit-sliced popcount of N planes into kSlicePc planes: full adders (3 planes -> sum and
@@ -138,7 +150,7 @@ MOTEX_HD void slice_ripple(uint32_t *h, int n, uint32_t c)
     }
 }
 
-template <int N>
+template <int N, int NQ = kSlicePc>
 MOTEX_HD void slice_popcount(const uint32_t *x, uint32_t out[kSlicePc])
 {
     uint32_t ones = 0, twos = 0, hi[kSlicePc - 2] = {0, 0, 0};
@@ -164,7 +176,7 @@ MOTEX_HD void slice_popcount(const uint32_t *x, uint32_t out[kSlicePc])
     }
     out[0] = ones;
     out[1] = twos;
-    for (int q = 2; q < kSlicePc; ++q) out[q] = hi[q - 2];
+    for (int q = 2; q < kSlicePc; ++q) out[q] = q < NQ ? hi[q - 2] : 0u;
 }
 
 // Running comparison bits.  Scanning planes from the least significant up, "a > b so far"
@@ -193,13 +205,13 @@ MOTEX_HD uint32_t slice_lt_step(uint32_t a, uint32_t b, uint32_t r)
 // equal popcount and larger word.  va / vb have NV planes.  The key (popcount inverted, then
 // word) is compared in one ripple from the word's lowest plane up through the popcount
 // planes.
-template <int NV>
+template <int NV, int NQ = kSlicePc>
 MOTEX_HD uint32_t slice_better(const uint32_t pa[kSlicePc], const uint32_t *va,
                                const uint32_t pb[kSlicePc], const uint32_t *vb)
 {
     uint32_t r = 0;
     for (int p = 0; p < NV; ++p) r = slice_gt_step(va[p], vb[p], r);
-    for (int q = 0; q < kSlicePc; ++q) r = slice_lt_step(pa[q], pb[q], r);
+    for (int q = 0; q < NQ; ++q) r = slice_lt_step(pa[q], pb[q], r);
     return r;
 }
 
@@ -209,12 +221,13 @@ MOTEX_HD void slice_e_planes(uint32_t e, uint32_t eb[kSlicePc])
     for (int q = 0; q < kSlicePc; ++q) eb[q] = ((e >> q) & 1) ? ~0u : 0u;
 }
 
-// "count <= e" over kSlicePc planes, e given as planes: deciding from the low bits up,
-// "c <= e so far" is majority(~c, e, r).
+// "count <= e" over the low NQ planes (the counts and e fit them), e given as planes:
+// deciding from the low bits up, "c <= e so far" is majority(~c, e, r).
+template <int NQ = kSlicePc>
 MOTEX_HD uint32_t slice_le_planes(const uint32_t c[kSlicePc], const uint32_t eb[kSlicePc])
 {
     uint32_t res = ~0u;
-    for (int q = 0; q < kSlicePc; ++q) res = slice_lt_step(c[q], eb[q], res);
+    for (int q = 0; q < NQ; ++q) res = slice_lt_step(c[q], eb[q], res);
     return res;
 }
 
@@ -234,6 +247,7 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
                          uint32_t full, Slice<K> &D, uint32_t *dropped = nullptr)
 {
     constexpr int W = SliceW<K>::W;
+    constexpr int NQ = slice_pcq<K>();
     // candidates, W+1 planes each (plane W only for the comparison)
     uint32_t a[W + 1], b[W + 1], c[W + 1];
     a[0] = ~0u;
@@ -247,23 +261,23 @@ MOTEX_HD void slice_cell(const Slice<K> &U, const Slice<K> &L, const Slice<K> &G
     }
     uint32_t lsrc[kSlicePc], gsrc[kSlicePc], pcc[kSlicePc];
     for (int q = 0; q < kSlicePc; ++q) {
-        lsrc[q] = slice_mux(full, L.pc[q], L.pm[q]);
-        gsrc[q] = slice_mux(full, G.pc[q], G.pm[q]);
+        lsrc[q] = q < NQ ? slice_mux(full, L.pc[q], L.pm[q]) : 0u;
+        gsrc[q] = q < NQ ? slice_mux(full, G.pc[q], G.pm[q]) : 0u;
     }
-    slice_inc(gsrc, delta, pcc);
+    slice_inc<NQ>(gsrc, delta, pcc);
     // min*: start from a, replace where b then c is strictly better.  a and b both set plane
     // 0 and add one to their source's popcount, so they compare as their sources do.
-    uint32_t sel = slice_better<W>(lsrc, b + 1, U.pc, a + 1);
+    uint32_t sel = slice_better<W, NQ>(lsrc, b + 1, U.pc, a + 1);
     uint32_t d[W + 1], pd[kSlicePc], ps[kSlicePc];
     d[0] = ~0u;
     for (int p = 1; p <= W; ++p) d[p] = slice_mux(sel, b[p], a[p]);
-    for (int q = 0; q < kSlicePc; ++q) ps[q] = slice_mux(sel, lsrc[q], U.pc[q]);
-    slice_inc(ps, ~0u, pd);
-    sel = slice_better<W + 1>(pcc, c, pd, d);
+    for (int q = 0; q < kSlicePc; ++q) ps[q] = q < NQ ? slice_mux(sel, lsrc[q], U.pc[q]) : 0u;
+    slice_inc<NQ>(ps, ~0u, pd);
+    sel = slice_better<W + 1, NQ>(pcc, c, pd, d);
     for (int p = 0; p < W; ++p) D.d[p] = slice_mux(sel, c[p], d[p]);
     if (dropped) *dropped |= slice_mux(sel, c[W], d[W]);  // host check of the width bound
-    for (int q = 0; q < kSlicePc; ++q) D.pc[q] = slice_mux(sel, pcc[q], pd[q]);
-    slice_popcount<K - 1>(D.d, D.pm);
+    for (int q = 0; q < kSlicePc; ++q) D.pc[q] = q < NQ ? slice_mux(sel, pcc[q], pd[q]) : 0u;
+    slice_popcount<K - 1, NQ>(D.d, D.pm);
     D.hi = 0;  // kept by the masked cell only
 }
 
@@ -285,6 +299,7 @@ MOTEX_HD void slice_cell_masked(const uint32_t *ud, uint32_t uhi, const uint32_t
                                 const uint32_t *lpm, const uint32_t *gd, const uint32_t *gpm,
                                 uint32_t delta, uint32_t *dd, uint32_t &dhi, uint32_t *dpc)
 {
+    constexpr int NQ = slice_pcq<K>();
     // candidate planes 0..K-1: a = shift(up) | 1, b = shiftc(left) | 1, c = shiftc(diag) | delta
     uint32_t a[K], b[K], c[K];
     a[0] = ~0u;
@@ -296,35 +311,35 @@ MOTEX_HD void slice_cell_masked(const uint32_t *ud, uint32_t uhi, const uint32_t
         c[p] = gd[p - 1];
     }
     uint32_t pcc[kSlicePc];
-    slice_inc(gpm, delta, pcc);
+    slice_inc<NQ>(gpm, delta, pcc);
     // min*: start from a, replace where b then c is strictly better.  a and b both set plane
     // 0 and add one to their source's popcount, so they compare as their sources do.
     uint32_t r = 0;
     for (int p = 1; p < K; ++p) r = slice_gt_step(b[p], a[p], r);
     r = slice_andnot(r, uhi);
-    for (int q = 0; q < kSlicePc; ++q) r = slice_lt_step(lpm[q], upc[q], r);
+    for (int q = 0; q < NQ; ++q) r = slice_lt_step(lpm[q], upc[q], r);
     const uint32_t sel = r;
     uint32_t d[K], pd[kSlicePc], ps[kSlicePc];
     d[0] = ~0u;
     for (int p = 1; p < K; ++p) d[p] = slice_mux(sel, b[p], a[p]);
-    for (int q = 0; q < kSlicePc; ++q) ps[q] = slice_mux(sel, lpm[q], upc[q]);
-    slice_inc(ps, ~0u, pd);
+    for (int q = 0; q < kSlicePc; ++q) ps[q] = q < NQ ? slice_mux(sel, lpm[q], upc[q]) : 0u;
+    slice_inc<NQ>(ps, ~0u, pd);
     // c against d: d's planes K..W are a's where b was not chosen
     r = 0;
     for (int p = 0; p < K; ++p) r = slice_gt_step(c[p], d[p], r);
     r = slice_andnot_unless(r, uhi, sel);
-    for (int q = 0; q < kSlicePc; ++q) r = slice_lt_step(pcc[q], pd[q], r);
+    for (int q = 0; q < NQ; ++q) r = slice_lt_step(pcc[q], pd[q], r);
     const uint32_t sel2 = r;
     for (int p = 0; p < K; ++p) dd[p] = slice_mux(sel2, c[p], d[p]);
     dhi = dd[K - 1] | slice_andnot2(uhi, sel, sel2);
-    for (int q = 0; q < kSlicePc; ++q) dpc[q] = slice_mux(sel2, pcc[q], pd[q]);
+    for (int q = 0; q < kSlicePc; ++q) dpc[q] = q < NQ ? slice_mux(sel2, pcc[q], pd[q]) : 0u;
 }
 
 // pm of a word: the popcount of its low K-1 planes.
 template <int K>
 MOTEX_HD void slice_pm(const uint32_t *d, uint32_t pm[kSlicePc])
 {
-    slice_popcount<K - 1>(d, pm);
+    slice_popcount<K - 1, slice_pcq<K>()>(d, pm);
 }
 
 // Row 0 of column j (1-based): min(j, K) ones in every text.

@@ -114,7 +114,7 @@ __device__ __forceinline__ void slice_left(Slice<K> &L, const uint32_t *ud, cons
     if constexpr (FIRST) {
 #pragma unroll
         for (int p = 0; p < W; ++p) L.d[p] = shr1_from(0u, ud[p]);
-        slice_popcount<W>(L.d, L.pc);
+        slice_popcount<W, slice_pcq<K>()>(L.d, L.pc);
     } else {
 #pragma unroll
         for (int p = 0; p < SP; ++p) L.d[p] = shr1_from(cr[p], ud[p]);
@@ -176,7 +176,7 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
                 slice_cell_masked<K>(U.d, U.hi, U.pc, L.d, L.pm, G.d, G.pm, ne, D.d, D.hi, D.pc);
             }
             if (report) {
-                const uint32_t hit = slice_le_planes(D.pc, eb) & t[5];
+                const uint32_t hit = slice_le_planes<slice_pcq<K>()>(D.pc, eb) & t[5];
                 vcnt += __builtin_popcount(hit);
                 anyh |= hit;
             }
@@ -324,7 +324,7 @@ __device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uin
             slice_cell_masked<K>(c.U.d, c.U.hi, c.U.pc, L.d, L.pm, c.G.d, c.G.pm, ne, D.d, D.hi, D.pc);
         }
         if (c.report) {
-            const uint32_t hit = slice_le_planes(D.pc, c.eb) & t[5];
+            const uint32_t hit = slice_le_planes<slice_pcq<K>()>(D.pc, c.eb) & t[5];
             c.vcnt += __builtin_popcount(hit);
             c.anyh |= hit;
         }
