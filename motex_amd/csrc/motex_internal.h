@@ -20,6 +20,7 @@ constexpr uint32_t kHitQueueCap = 512;       // Hamming hit queue entries per wa
 // bitmap in LDS.
 constexpr uint32_t kMaxSymGroup = 3;
 constexpr uint32_t kSymBitmapBits = 8192;    // a range's first-occurrence bitmap (256 words)
+constexpr uint32_t kMaxSymItems = 1u << 21;  // pair-symmetric Hamming plans: work items at most
 constexpr uint32_t kSlicePipe = 4;           // waves per item of the pipelined text-parallel edit kernel
 // Edit kernel choice, from sweeps of 40-500 protein and DNA sequences of U[100,2000], k 8-14:
 // below kSliceItemsPerCu (x, group of 32 texts) items per CU the one-text-per-lane kernel

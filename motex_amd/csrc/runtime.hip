@@ -458,6 +458,9 @@ static bool hd_symmetric(motex_batch *b, const HdPlan &plan)
 {
     const SeqSet &ss = b->ss;
     if (plan.split_slots || b->nb_structs != 1 || ss.num < 2 || ss.nx != ss.num) return false;
+    // its items hold at most kMaxSymGroup ranges, about N x ranges / (2 kMaxSymGroup) of them:
+    // many short texts (hundreds of ranges per x) would make the work queue huge
+    if ((double)ss.num * plan.ranges.size() / (2.0 * kMaxSymGroup) > (double)kMaxSymItems) return false;
     if (const char *env = getenv("MOTEX_SYM")) return env[0] == '1';
     return hd_cell_hit_rate(ss, b->params.k, b->params.e) < 1e-4;
 }

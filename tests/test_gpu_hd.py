@@ -100,6 +100,25 @@ def test_long_and_short_mixed(ctx, oracle, k, e):
     check(ctx, oracle, seqs, k, e)
 
 
+def test_many_short_texts_keep_the_work_queue_small(ctx, monkeypatch):
+    # pair-symmetric items hold at most three text ranges; with tens of thousands of short
+    # texts that plan would need millions of items, so the ordered plan runs instead (even
+    # when the symmetric one is asked for), and every factor still occurs in its own text
+    monkeypatch.setenv("MOTEX_SYM", "1")
+    rng = np.random.default_rng(7)
+    seqs = rand_seqs(rng, 30000, 40, 40, b"ACGT")
+    print(f"[hd] {len(seqs)} texts of 40 nt, k=12 e=2", flush=True)
+    b = ctx.batch(seqs, MotexParams(k=12, e=2))
+    st = b.run()
+    u, v, off = b.download()
+    b.close()
+    assert st["symmetric"] == 0 and st["work_items"] < 2_000_000
+    u, v = u.reshape(len(seqs), 40), v.reshape(len(seqs), 40)
+    assert np.all(u[:, :11] == 0) and np.all(v[:, :11] == 0)  # no full window before k - 1
+    full_u, full_v = u[:, 11:], v[:, 11:]
+    assert np.all(full_u >= 1) and np.all(full_v >= full_u) and np.all(full_u <= len(seqs))
+
+
 def test_empty_and_single(ctx, oracle):
     check(ctx, oracle, [b"ACGTACGTAC"], 4, 1)
     check(ctx, oracle, [b"ACG", b"AC"], 4, 1)   # every pair skipped
@@ -138,7 +157,7 @@ def test_invalid_params_raise(ctx):
 
 def test_u_counts_beyond_16_bits(ctx):
     """100,000 copies of AAAA, k=4, e=0: the one factor of every sequence occurs once in every
-    sequence, so u = v = N exactly.  Above 65,535 sequences the ordered plan runs, and one
+    sequence, so u = v = N exactly.  The ordered plan runs (occurrences are dense), and one
     work item then spans more than 65,535 texts: the per-row distinct-text count must not
     wrap (the reference keeps u as unsigned int, motex.cc:495-529)."""
     N = 100_000
