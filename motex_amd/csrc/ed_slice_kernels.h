@@ -7,9 +7,10 @@
 //
 // Layout.  A wave owns a strip of 64 columns of x, one per lane, and streams the group's
 // rows through it as a skewed wavefront: at step s lane L works on row s-L.  The left
-// neighbour's cell and the row's text planes arrive from lane L-1 by one DPP wave shift per
-// word; lane 0 takes them from an LDS chunk of 64 rows holding the previous strip's last
-// column (the boundary stream, in the wave's scratch) and the group's text planes.  Lane 63
+// neighbour's cell arrives from lane L-1 by one DPP wave shift per word; lane 0 takes it from
+// an LDS chunk of 64 rows holding the previous strip's last column (the boundary stream, in
+// the wave's scratch).  Every lane reads its row's text planes from a ring of the group's
+// last 128 rows in LDS, filled with the chunk (a read, not six DPP shifts, per step).  Lane 63
 // writes its cells to the next strip's boundary stream.  Past the first strip every column
 // has j > K, so only the low K-1 planes of a word travel (all a truncated left or diagonal
 // candidate keeps) and the receiver recounts their popcount (slice_cell_masked).
@@ -40,23 +41,42 @@ struct SliceLayout {
     static constexpr int W = SliceW<K>::W;
     static constexpr int SP = K - 1;               // planes a later strip's left neighbour sends
     static constexpr int SWS = (SP + 3) & ~3;      // boundary-stream row
-    static constexpr int LW = SWS + 8;             // LDS chunk row: stream row, 6 text words, 2 pad
+    static constexpr int LW = SWS;                 // LDS chunk row: the stream row
 };
+
+// The group's text planes in LDS: rows r of the last 128 at (r mod 128) x 6 words.  A step's
+// lanes read rows s-64 .. s-1 and a chunk fill writes rows s-1 .. s+62: 127 distinct slots.
+constexpr uint32_t kTextRing = 128;
+constexpr uint32_t kTextWords = 6;
+
+// Text planes of row r (5 symbol-code planes and the valid-text mask), 8-byte reads.
+__device__ __forceinline__ void slice_text(const uint32_t *tring, uint32_t r, uint32_t *t)
+{
+    const uint2 *src = reinterpret_cast<const uint2 *>(tring + (r & (kTextRing - 1)) * kTextWords);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const uint2 v = src[q];
+        t[2 * q] = v.x;
+        t[2 * q + 1] = v.y;
+    }
+}
 
 // Lane rr fills row rr of the LDS chunk of 64 rows from r0: the previous strip's boundary row
 // (when from_stream; zeros for the first strip, whose left edge is column 0) with 16-byte
-// loads and stores, then the group's 6 text words with 8-byte ones.  The stream rows
-// (SWS words, a multiple of 4, from a 16-byte aligned base), the text rows (6 words) and the
-// chunk rows (LW words, a multiple of 4) keep those accesses aligned.
+// loads and stores, and the row's slot of the text ring with the group's 6 text words in
+// 8-byte ones.  The stream rows (SWS words, a multiple of 4, from a 16-byte aligned base), the
+// text rows (6 words) and the chunk rows (LW words, a multiple of 4) keep those accesses
+// aligned.
 template <int K>
-__device__ __forceinline__ void slice_fill_row(uint32_t *chunk, uint32_t r0, uint32_t nrows, bool from_stream,
-                                               const uint32_t *sin, const uint32_t *gp, uint32_t rr)
+__device__ __forceinline__ void slice_fill_row(uint32_t *chunk, uint32_t *tring, uint32_t r0, uint32_t nrows,
+                                               bool from_stream, const uint32_t *sin, const uint32_t *gp,
+                                               uint32_t rr)
 {
     typedef SliceLayout<K> Lay;
     constexpr int SWS = Lay::SWS, LW = Lay::LW;
     const uint32_t row = r0 + rr;
     uint4 *dst = reinterpret_cast<uint4 *>(chunk + rr * LW);
-    uint2 *dtext = reinterpret_cast<uint2 *>(chunk + rr * LW + SWS);
+    uint2 *dtext = reinterpret_cast<uint2 *>(tring + (row & (kTextRing - 1)) * kTextWords);
     if (row < nrows) {
         if (from_stream) {
             const uint4 *src = reinterpret_cast<const uint4 *>(sin + (size_t)row * SWS);
@@ -80,11 +100,11 @@ __device__ __forceinline__ void slice_fill_row(uint32_t *chunk, uint32_t r0, uin
 // Loads the LDS chunk of 64 rows from r0: the previous strip's boundary cells (strip > 0)
 // and the group's text planes.
 template <int K>
-__device__ __forceinline__ void slice_chunk(uint32_t *chunk, uint32_t r0, uint32_t nrows, bool from_stream,
-                                            const uint32_t *sin, const uint32_t *gp)
+__device__ __forceinline__ void slice_chunk(uint32_t *chunk, uint32_t *tring, uint32_t r0, uint32_t nrows,
+                                            bool from_stream, const uint32_t *sin, const uint32_t *gp)
 {
     __syncthreads();  // the previous chunk is no longer read
-    slice_fill_row<K>(chunk, r0, nrows, from_stream, sin, gp, threadIdx.x);
+    slice_fill_row<K>(chunk, tring, r0, nrows, from_stream, sin, gp, threadIdx.x);
     __syncthreads();
 }
 
@@ -131,7 +151,7 @@ __device__ __forceinline__ void slice_left(Slice<K> &L, const uint32_t *ud, cons
 template <int K, bool FIRST>
 __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceItem &I, uint32_t strip,
                                             uint32_t nrows, const uint32_t *gp, const uint32_t *sin,
-                                            uint32_t *sout, bool last, uint32_t *chunk)
+                                            uint32_t *sout, bool last, uint32_t *chunk, uint32_t *tring)
 {
     typedef SliceLayout<K> Lay;
     constexpr int LW = Lay::LW;
@@ -151,18 +171,16 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
     Slice<K> U, G;
     slice_row0<K>(j, U);
     slice_row0<K>(j - 1, G);
-    uint32_t tpub[6] = {0, 0, 0, 0, 0, 0};
     uint32_t vcnt = 0, anyh = 0;
     const uint32_t steps = nrows + 63;
     auto step = [&](uint32_t s, auto check) {
         const uint32_t r0 = s - 1;  // 0-based row lane 0 starts this step
-        if ((r0 & 63) == 0) slice_chunk<K>(chunk, r0, nrows, !FIRST, sin, gp);
+        if ((r0 & 63) == 0) slice_chunk<K>(chunk, tring, r0, nrows, !FIRST, sin, gp);
         const uint32_t *cr = chunk + (r0 & 63) * LW;
         Slice<K> L;  // D[i][j-1]; only its planes travel, the receiver recounts pc and pm
         slice_left<K, FIRST>(L, U.d, cr);
-        uint32_t t[6];
-#pragma unroll
-        for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[Lay::SWS + b], tpub[b]);
+        uint32_t t[kTextWords];  // row r0 - lane's text planes (garbage on lanes without a row)
+        slice_text(tring, r0 - lane, t);
         bool active = true;
         if constexpr (decltype(check)::value) active = s > lane && s - lane <= nrows;
         if (active) {
@@ -182,8 +200,6 @@ __device__ __forceinline__ void slice_strip(const EdSliceArgs &a, const EdSliceI
             }
             G = L;
             U = D;
-#pragma unroll
-            for (int b = 0; b < 6; ++b) tpub[b] = t[b];
             if (!last && lane == 63) slice_emit<K>(sout, s, D.d);
         }
     };
@@ -223,6 +239,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(slice_waves_
 void ed_slice_kernel(EdSliceArgs a)
 {
     __shared__ __align__(16) uint32_t chunk[64 * SliceLayout<K>::LW];
+    __shared__ __align__(16) uint32_t tring[kTextRing * kTextWords];
     const uint32_t lane = threadIdx.x;
     uint32_t *st0 = a.scratch + (size_t)blockIdx.x * 2 * a.stream_words;
     uint32_t *st1 = st0 + a.stream_words;
@@ -242,11 +259,11 @@ void ed_slice_kernel(EdSliceArgs a)
         const uint32_t nrows = a.goff[I.group + 1] - g0;
         const uint32_t *gp = a.gplanes + (size_t)g0 * 6;
         const uint32_t nstrips = (I.m + 63) / 64;
-        slice_strip<K, true>(a, I, 0, nrows, gp, st1, st0, nstrips == 1, chunk);
+        slice_strip<K, true>(a, I, 0, nrows, gp, st1, st0, nstrips == 1, chunk, tring);
         for (uint32_t strip = 1; strip < nstrips; ++strip) {
             const uint32_t *sin = (strip & 1) ? st0 : st1;
             uint32_t *sout = (strip & 1) ? st1 : st0;
-            slice_strip<K, false>(a, I, strip, nrows, gp, sin, sout, strip + 1 == nstrips, chunk);
+            slice_strip<K, false>(a, I, strip, nrows, gp, sin, sout, strip + 1 == nstrips, chunk, tring);
         }
     }
 }
@@ -256,18 +273,18 @@ void ed_slice_kernel(EdSliceArgs a)
 // from_stream; zeros for the first strip, whose left edge is column 0) and the group's text
 // planes.  Done by the 64 lanes of the wave that reads the chunk.
 template <int K>
-__device__ __forceinline__ void slice_chunk_fill(uint32_t *chunk, uint32_t r0, uint32_t nrows, bool from_stream,
-                                                 const uint32_t *sin, const uint32_t *gp, uint32_t lane)
+__device__ __forceinline__ void slice_chunk_fill(uint32_t *chunk, uint32_t *tring, uint32_t r0, uint32_t nrows,
+                                                 bool from_stream, const uint32_t *sin, const uint32_t *gp,
+                                                 uint32_t lane)
 {
-    slice_fill_row<K>(chunk, r0, nrows, from_stream, sin, gp, lane);
+    slice_fill_row<K>(chunk, tring, r0, nrows, from_stream, sin, gp, lane);
 }
 
 // One lane's column j of a strip, carried from row to row: D[i-1][j] (what lane L+1 receives
-// next), D[i-1][j-1], the text planes lane L+1 receives next, and the column's hit counts.
+// next), D[i-1][j-1], and the column's hit counts.
 template <int K>
 struct SliceCol {
     Slice<K> U, G;
-    uint32_t tpub[6];
     uint32_t xb[5];          // planes of x[j]'s code
     uint32_t eb[kSlicePc];   // planes of e, wave-uniform
     uint32_t vcnt, anyh, j, full;
@@ -288,8 +305,6 @@ __device__ __forceinline__ void slice_col_begin(SliceCol<K> &c, const EdSliceArg
     c.full = c.j <= (uint32_t)K ? ~0u : 0u;
     slice_row0<K>(c.j, c.U);
     slice_row0<K>(c.j - 1, c.G);
-#pragma unroll
-    for (int b = 0; b < 6; ++b) c.tpub[b] = 0;
     c.vcnt = 0;
     c.anyh = 0;
 }
@@ -300,17 +315,16 @@ __device__ __forceinline__ void slice_col_begin(SliceCol<K> &c, const EdSliceArg
 // (slice_cell, needing pc and pm of left).  Later strips have only columns j > K
 // (slice_cell_masked, needing pm).  CHECK: some lane has no row at this step.
 template <int K, bool FIRST, bool CHECK>
-__device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uint32_t *chunk, uint32_t nrows,
-                                           uint32_t lane, uint32_t *sout, bool emit)
+__device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uint32_t *chunk, const uint32_t *tring,
+                                           uint32_t nrows, uint32_t lane, uint32_t *sout, bool emit)
 {
     typedef SliceLayout<K> Lay;
     constexpr int LW = Lay::LW;
     const uint32_t *cr = chunk + ((s - 1) & 63) * LW;
     Slice<K> L;  // D[i][j-1]
     slice_left<K, FIRST>(L, c.U.d, cr);
-    uint32_t t[6];
-#pragma unroll
-    for (int b = 0; b < 6; ++b) t[b] = shr1_from(cr[Lay::SWS + b], c.tpub[b]);
+    uint32_t t[kTextWords];  // row s-1-lane's text planes (garbage on lanes without a row)
+    slice_text(tring, s - 1 - lane, t);
     bool active = true;
     if constexpr (CHECK) active = s > lane && s - lane <= nrows;
     if (active) {
@@ -330,8 +344,6 @@ __device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uin
         }
         c.G = L;
         c.U = D;
-#pragma unroll
-        for (int b = 0; b < 6; ++b) c.tpub[b] = t[b];
         if (emit && lane == 63) slice_emit<K>(sout, s, D.d);
     }
 }
@@ -342,18 +354,20 @@ __device__ __forceinline__ void slice_step(SliceCol<K> &c, uint32_t s, const uin
 // (fewer registers: the full-cell form of the first strip).
 template <int K, bool FIRST, bool SPLIT = true>
 __device__ __forceinline__ void slice_run_chunk(SliceCol<K> &c, uint32_t ch, const uint32_t *chunk,
-                                                uint32_t nrows, uint32_t lane, uint32_t *sout, bool emit)
+                                                const uint32_t *tring, uint32_t nrows, uint32_t lane, uint32_t *sout,
+                                                bool emit)
 {
     const uint32_t s0 = 64 * ch + 1, s1 = min(64 * ch + 64, nrows + 63);
     if (SPLIT && s0 >= 64 && s1 <= nrows) {
         uint32_t s = s0;
         for (; s + 1 <= s1; s += 2) {
-            slice_step<K, FIRST, false>(c, s, chunk, nrows, lane, sout, emit);
-            slice_step<K, FIRST, false>(c, s + 1, chunk, nrows, lane, sout, emit);
+            slice_step<K, FIRST, false>(c, s, chunk, tring, nrows, lane, sout, emit);
+            slice_step<K, FIRST, false>(c, s + 1, chunk, tring, nrows, lane, sout, emit);
         }
-        if (s <= s1) slice_step<K, FIRST, false>(c, s, chunk, nrows, lane, sout, emit);
+        if (s <= s1) slice_step<K, FIRST, false>(c, s, chunk, tring, nrows, lane, sout, emit);
     } else {
-        for (uint32_t s = s0; s <= s1; ++s) slice_step<K, FIRST, true>(c, s, chunk, nrows, lane, sout, emit);
+        for (uint32_t s = s0; s <= s1; ++s)
+            slice_step<K, FIRST, true>(c, s, chunk, tring, nrows, lane, sout, emit);
     }
 }
 
@@ -386,9 +400,11 @@ void ed_slice_pipe_kernel(EdSliceArgs a)
     typedef SliceLayout<K> Lay;
     constexpr uint32_t P = kSlicePipe;
     __shared__ __align__(16) uint32_t chunks[P][64 * Lay::LW];
+    __shared__ __align__(16) uint32_t trings[P][kTextRing * kTextWords];
     __shared__ uint32_t item_slot;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t *chunk = chunks[wave];
+    uint32_t *tring = trings[wave];
     uint32_t *bufs = a.scratch + (size_t)blockIdx.x * (P + 1) * a.stream_words;
 
     for (;;) {
@@ -413,9 +429,9 @@ void ed_slice_pipe_kernel(EdSliceArgs a)
             slice_col_begin<K>(col, a, I, 0, lane);
             for (; T < nch; ++T) {
                 __syncthreads();
-                slice_chunk_fill<K>(chunk, 64 * T, nrows, false, nullptr, gp, lane);
+                slice_chunk_fill<K>(chunk, tring, 64 * T, nrows, false, nullptr, gp, lane);
                 __syncthreads();
-                slice_run_chunk<K, true, false>(col, T, chunk, nrows, lane, bufs, nstrips > 1);
+                slice_run_chunk<K, true, false>(col, T, chunk, tring, nrows, lane, bufs, nstrips > 1);
             }
             slice_col_end<K>(col, a, I);
         }
@@ -432,13 +448,13 @@ void ed_slice_pipe_kernel(EdSliceArgs a)
             __syncthreads();  // boundary cells of the earlier ticks are written
             if (act) {
                 const uint32_t *sin = bufs + (size_t)((t - 1) % (P + 1)) * a.stream_words;
-                slice_chunk_fill<K>(chunk, 64 * ch, nrows, true, sin, gp, lane);
+                slice_chunk_fill<K>(chunk, tring, 64 * ch, nrows, true, sin, gp, lane);
             }
             __syncthreads();  // the chunk is in LDS
             if (act) {
                 uint32_t *sout = bufs + (size_t)(t % (P + 1)) * a.stream_words;
                 if (ch == 0) slice_col_begin<K>(col, a, I, t, lane);
-                slice_run_chunk<K, false>(col, ch, chunk, nrows, lane, sout, t + 1 < nstrips);
+                slice_run_chunk<K, false>(col, ch, chunk, tring, nrows, lane, sout, t + 1 < nstrips);
                 if (ch + 1 == nch) slice_col_end<K>(col, a, I);
             }
         }
