@@ -471,6 +471,9 @@ static bool hd_symmetric(motex_batch *b, const HdPlan &plan)
 static std::vector<uint32_t> hd_sym_list_offsets(const SeqSet &ss, const HdPlan &plan, uint32_t k, uint32_t e)
 {
     const double p = hd_cell_hit_rate(ss, k, e);
+    // MOTEX_SYM_LIST_CAP=n: n entries per range (tests of the lists' atomic fallback)
+    uint64_t forced = 0;
+    if (const char *env = getenv("MOTEX_SYM_LIST_CAP")) forced = strtoull(env, nullptr, 10);
     std::vector<uint64_t> rows_upto(ss.num + 1, 0);  // rows of sequences 0 .. j-1
     for (uint32_t j = 0; j < ss.num; ++j) rows_upto[j + 1] = rows_upto[j] + (ss.len[j] >= k ? ss.len[j] - k + 1 : 0);
     std::vector<uint32_t> off(plan.ranges.size() + 1, 0);
@@ -478,7 +481,7 @@ static std::vector<uint32_t> hd_sym_list_offsets(const SeqSet &ss, const HdPlan 
     for (size_t r = 0; r < plan.ranges.size(); ++r) {
         const HdRange &R = plan.ranges[r];
         const double rows = (double)rows_upto[R.j0 + R.ny - 1];
-        const double cap = 3.0 * p * rows * (R.ye - R.ys) + kSymBitmapBits;
+        const double cap = forced ? (double)forced : 3.0 * p * rows * (R.ye - R.ys) + kSymBitmapBits;
         total += ((uint64_t)std::min(cap, (double)(1u << 28)) + 7) & ~7ull;  // lists start at multiples of 8
         off[r + 1] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFF8u);
     }

@@ -48,6 +48,16 @@ def test_dna1200_subset(ctx, oracle, sym):
     check(ctx, oracle, w.seqs, w.k, w.e, sym=sym)
 
 
+@pytest.mark.parametrize("cap", ["8", "1"])
+def test_symmetric_text_side_lists_full(ctx, oracle, monkeypatch, cap):
+    # every range's list of first text-side occurrences holds only `cap` entries (rounded up
+    # to 8): the rest take the direct atomics, with the same result
+    monkeypatch.setenv("MOTEX_SYM", "1")
+    monkeypatch.setenv("MOTEX_SYM_LIST_CAP", cap)
+    rng = np.random.default_rng(11)
+    check(ctx, oracle, rand_seqs(rng, 40, 200, 900, b"ACGT"), 8, 2, sym=True)
+
+
 def test_symmetric_is_the_default_for_rare_occurrences(ctx):
     w = synth.dna1200_hamming(scale=0.05)
     b = ctx.batch(w.seqs, MotexParams(k=w.k, e=w.e))
