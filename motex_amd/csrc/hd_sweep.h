@@ -73,10 +73,19 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
         const uint32_t r = meta & 0xFFFFu;
         const int32_t base0 = q0 + (int32_t)(32 * (meta >> 16) + px_start + r);
         uint32_t hits = 0, ym = 0;
+        // the entry's blocks with hits, one per trip (almost always one): a loop over all B
+        // blocks would run the body B times for the lanes of different blocks in turn
+        uint32_t bmask = 0;
 #pragma unroll
-        for (int b = 0; b < B; ++b) {
-            if (!w[b]) continue;
-            const int32_t base = base0 + (int32_t)kDiagPerBlock * b;
+        for (int b = 0; b < B; ++b) bmask |= (w[b] ? 1u : 0u) << b;
+        while (bmask) {
+            const uint32_t b = __builtin_ctz(bmask);
+            bmask &= bmask - 1;
+            uint32_t wb = w[0];
+#pragma unroll
+            for (int bb = 1; bb < B; ++bb)
+                if (b == (uint32_t)bb) wb = w[bb];
+            const int32_t base = base0 + (int32_t)(kDiagPerBlock * b);
             const int32_t first = base > ys ? base : ys;
             uint32_t y = 0;
             if (R.ny > 1) {
@@ -96,7 +105,7 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
                 const uint32_t j = R.j0 + y;
                 if (lo < hi && (!sym || j >= xi)) {
                     const uint64_t span = ((1ull << (hi - base)) - 1) & ~((1ull << (lo - base)) - 1);
-                    const uint32_t hv = w[b] & (uint32_t)span;
+                    const uint32_t hv = wb & (uint32_t)span;
                     if (hv) {
                         hits += __builtin_popcount(hv);
                         ym |= 1u << y;
