@@ -329,8 +329,10 @@ void hd_batched_kernel(HdArgs a)
                 // (the ring form appends a whole trip of up to 2 K rows before it drains: the
                 // queue then holds hitq_cap >= 64 (2 K + 2) entries, and it is drained once more
                 // than hitq_cap - 64 x trip rows are queued)
-                auto step = [&](uint32_t r, const uint32_t *an, const uint32_t *bo, auto leave) {
-                    const uint32_t px = px_start + r;
+                // (rep: row r is reported, px >= rep0 -- a bool, or std::true_type where that is
+                // known at compile time; inner: the range is interior)
+                auto step = [&](uint32_t r, const uint32_t *an, const uint32_t *bo, auto leave, auto rep,
+                                auto inner) {
                     uint32_t all = ~0u;  // lanes' words with no hit in any block
 #pragma unroll
                     for (int b = 0; b < B; ++b) {
@@ -358,43 +360,47 @@ void hd_batched_kernel(HdArgs a)
 #pragma unroll
                         for (int b = 0; b < B; ++b) all &= high[b];
                     }
-                    if (px >= rep0 && interior) {
-                        if (all != ~0u) {
-                            uint32_t high[B];
-                            highs(high);
-                            uint32_t c = 0;  // set high-plane bits: the lane's cells with no hit
-#pragma unroll
-                            for (int b = 0; b < B; ++b) c += __builtin_popcount(high[b]);
-                            atomicAdd(acc_v + r, 32u * B - c);
-                        }
-                    } else if (px >= rep0) {
+                    if (rep) {
+                        // (the range's kind is tested on rows with a hit only)
                         const uint64_t m = __ballot(all != ~0u);
                         if (m) {
-                            if (all != ~0u) {
-                                uint32_t high[B];
-                                highs(high);
-                                // the entry holds the high-plane words (clear bit = hit); 32-bit
-                                // byte offsets from the wave's (scalar) queue bases
-                                const uint32_t slot =
-                                    nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                                char *res_b = reinterpret_cast<char *>(hq_res) + slot * (4u * B);
-                                if constexpr (B == 4) {
-                                    *reinterpret_cast<uint4 *>(res_b) = make_uint4(high[0], high[1], high[2], high[3]);
-                                } else if constexpr (B == 2) {
-                                    *reinterpret_cast<uint2 *>(res_b) = make_uint2(high[0], high[1]);
-                                } else {
-                                    *reinterpret_cast<uint32_t *>(res_b) = high[0];
+                            if (inner) {
+                                if (all != ~0u) {
+                                    uint32_t high[B];
+                                    highs(high);
+                                    uint32_t c = 0;  // set high-plane bits: the lane's cells with no hit
+#pragma unroll
+                                    for (int b = 0; b < B; ++b) c += __builtin_popcount(high[b]);
+                                    atomicAdd(acc_v + r, 32u * B - c);
                                 }
-                                *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hq_meta) + slot * 4u) =
-                                    r | (lane << 16);
-                            }
-                            nq += (uint32_t)__builtin_popcountll(m);
-                            if constexpr (K == 0) {
-                                if (nq > a.hitq_cap - 64) {
-                                    drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts,
-                                                  acc_v, ymask, tb);
-                                    nq = 0;
+                            } else {
+                                if (all != ~0u) {
+                                    uint32_t high[B];
+                                    highs(high);
+                                    // the entry holds the high-plane words (clear bit = hit); 32-bit
+                                    // byte offsets from the wave's (scalar) queue bases
+                                    const uint32_t slot =
+                                        nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                                    char *res_b = reinterpret_cast<char *>(hq_res) + slot * (4u * B);
+                                    if constexpr (B == 4) {
+                                        *reinterpret_cast<uint4 *>(res_b) =
+                                            make_uint4(high[0], high[1], high[2], high[3]);
+                                    } else if constexpr (B == 2) {
+                                        *reinterpret_cast<uint2 *>(res_b) = make_uint2(high[0], high[1]);
+                                    } else {
+                                        *reinterpret_cast<uint32_t *>(res_b) = high[0];
+                                    }
+                                    *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hq_meta) + slot * 4u) =
+                                        r | (lane << 16);
+                                }
+                                nq += (uint32_t)__builtin_popcountll(m);
+                                if constexpr (K == 0) {
+                                    if (nq > a.hitq_cap - 64) {
+                                        drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq,
+                                                      starts, acc_v, ymask, tb);
+                                        nq = 0;
+                                    }
                                 }
                             }
                         }
@@ -404,7 +410,8 @@ void hd_batched_kernel(HdArgs a)
                     // The column leaving at row r is the one that entered at row r - K, whose bits
                     // the lane computed then: a ring of the last K rows' entering words (slot
                     // r mod K; every item has at least K rows, and the loop runs in trips of K
-                    // rows from row K on, so every slot index is a constant).
+                    // rows from row K on, so every slot index is a constant; rows from k - 1 on are
+                    // reported, and the trips start past them).
                     uint32_t ring[K][B];
                     // Rows are software-pipelined: while row r computes, the words of row r + 1
                     // load into the other buffer (row r uses buffer (r + K) & 1, so that trips of
@@ -421,7 +428,7 @@ void hd_batched_kernel(HdArgs a)
                         load_win(x1, 0u, wbuf[(s + 1 + K) & 1], std::false_type{});
                         x1 = xb[s + 2];
                         enter_bits(s, wbuf[(s + K) & 1], ring[s]);
-                        step(s, ring[s], ring[s], std::false_type{});
+                        step(s, ring[s], ring[s], std::false_type{}, s >= K - 1, interior);
                         __builtin_amdgcn_sched_barrier(0);
                     }
                     auto trip_drain = [&] {
@@ -438,7 +445,7 @@ void hd_batched_kernel(HdArgs a)
                         x1 = xb[r + 2];
                         uint32_t an[B];
                         enter_bits(r, wbuf[s & 1], an);
-                        step(r, an, ring[s % K], std::true_type{});
+                        step(r, an, ring[s % K], std::true_type{}, std::true_type{}, interior);
 #pragma unroll
                         for (int b = 0; b < B; ++b) ring[s % K][b] = an[b];
                     };
@@ -464,7 +471,7 @@ void hd_batched_kernel(HdArgs a)
 #pragma unroll
                             for (int b = 0; b < B; ++b) bo[b] = __builtin_amdgcn_alignbit(w.lv[b][1], w.lv[b][0], sho);
                         }
-                        step(r, an, bo, leave);
+                        step(r, an, bo, leave, px_start + r >= rep0, interior);
                     };
                     const uint32_t warm = nrows < a.k ? nrows : a.k;
                     // two rows per trip: the counter planes then stay in place across the back edge
