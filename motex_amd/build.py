@@ -29,8 +29,9 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 HD_RING_SOURCES = [f"hd_ring_kernels_k{k}.hip" for k in (8, 10, 12, 14, 16)]
 SOURCE_FLAGS = {s: ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"] for s in ["hd_kernels.hip", *HD_RING_SOURCES]}
 LIB_SOURCES = ["hd_kernels.hip", *HD_RING_SOURCES, "ed_kernels.hip", "ed_slice_kernels.hip", "ed_slice_kernels_k15.hip",
-               "ed_slice_kernels_k23.hip", "struct_kernels.hip", "struct_range_kernels.hip", "runtime.hip",
-               "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip", "runtime_tune.hip", "runtime_noalloc.hip"]
+               "ed_slice_kernels_k23.hip", "struct_kernels.hip", "struct_range_kernels.hip", "struct_range_ring_short.hip",
+               "struct_range_ring_long.hip", "runtime.hip", "runtime_ed.hip", "runtime_struct.hip", "runtime_serve.hip",
+               "runtime_tune.hip", "runtime_noalloc.hip"]
 CLI_SOURCES = ["cli/main.cpp", "cli/app.cpp", "cli/fasta.cpp", "cli/writer.cpp", "cli/writer_back.cpp", "cli/stats.cpp",
                "cli/switches.cpp"]
 # MPFR (with GMP) for the command line's statistics modes (-b/-u/-U, -I), whose p-values the

@@ -30,19 +30,14 @@
 
 #include "hd_device.h"
 #include "motex_internal.h"
+#include "struct_range_sweep.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace motex {
 
 namespace {
-
-// First diagonal (pt - px) of a unit's window: the range's first position at the last row,
-// rounded down to a multiple of 32.
-__device__ __forceinline__ int32_t sr_window_q0(int32_t ys, uint32_t m)
-{
-    return (ys - (int32_t)m + 1) & ~31;
-}
 
 // One unit as the merge reads it: the slot's hits (HBM) and every shape's hits in position
 // buckets (LDS): entries px << 4 | (position & 15), bucket b of the running sequence over
@@ -345,182 +340,6 @@ __device__ bool sr_first_chain(const SrMergeArgs &a, const SrUnit &U, uint32_t k
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
-// Sweep
-// ---------------------------------------------------------------------------------------
-
-// Appends row px's hits (the clear bits of high) to the slot; false when they do not fit.
-__device__ __forceinline__ bool sr_emit(const uint32_t *high, uint32_t px, uint32_t lane, uint32_t *hits,
-                                        uint32_t hcap, uint32_t &nq)
-{
-    const uint32_t w0 = ~high[0], w1 = ~high[1], w2 = ~high[2], w3 = ~high[3];
-    const uint32_t any = w0 | w1 | w2 | w3;
-    const uint64_t mb = __ballot(any != 0);
-    if (!mb) return true;
-    const uint32_t nh = __builtin_popcount(w0) + __builtin_popcount(w1) + __builtin_popcount(w2) + __builtin_popcount(w3);
-    uint64_t bt = __ballot(nh > 1);
-    if (!bt) {
-        // at most one hit a lane (the common case): slot = lanes with a hit below
-        const uint32_t tot = (uint32_t)__builtin_popcountll(mb);
-        if (nq + tot > hcap) return false;
-        if (any) {
-            const uint32_t b = ((w1 | w3) ? 1u : 0u) | ((w2 | w3) ? 2u : 0u);
-            const uint32_t at = nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
-            hits[at] = (px << 16) | (kDiagPerBlock * b + 32 * lane + __builtin_ctz(any));
-        }
-        nq += tot;
-        return true;
-    }
-    // some lane has several: slots from the wave's exclusive prefix of the counts, one ballot
-    // per count level
-    uint32_t pre = 0, tot = 0;
-    bt = mb;
-    for (uint32_t t = 1; bt; ++t) {
-        pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(bt >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bt, pre));
-        tot += (uint32_t)__builtin_popcountll(bt);
-        bt = __ballot(nh > t);
-    }
-    if (nq + tot > hcap) return false;
-    if (nh) {
-        uint32_t at = nq + pre;
-        const uint32_t tag = (px << 16) | (32 * lane);
-#pragma unroll
-        for (int b = 0; b < (int)kSrBlocks; ++b) {
-            uint32_t hv = ~high[b];
-            while (hv) {
-                hits[at++] = tag + kDiagPerBlock * b + __builtin_ctz(hv);
-                hv &= hv - 1;
-            }
-        }
-    }
-    nq += tot;
-    return true;
-}
-
-// OR of the planes at and above the threshold plane PH (a clear bit: the window is within the
-// error bound).  PH >= P only when the bound is at least the box length: every window passes.
-template <int P, int PH>
-__device__ __forceinline__ uint32_t sr_high(const uint32_t *c)
-{
-    if constexpr (PH >= P) return 0u;
-    else return planes_high<P, PH>(c);
-}
-
-// One shape's sweep down the rows of x: P counter planes, threshold plane PH.  The first len
-// rows fill the windows (no leaving symbol); returns true when the hits overflowed the slot.
-template <int P, int PH>
-__device__ bool sr_sweep_shape(const char *ne_b, const uint16_t *xoff, uint32_t m, uint32_t len, uint32_t err,
-                               uint32_t *hits, uint32_t hcap, uint32_t lane, uint32_t &nq)
-{
-    uint32_t cnt[kSrBlocks][P];
-#pragma unroll
-    for (int b = 0; b < (int)kSrBlocks; ++b) planes_set<P>(cnt[b], hd_bias(err));
-    uint32_t high[kSrBlocks];
-    const uint32_t warm = min(len, m);
-    for (uint32_t px = 0; px < warm; ++px) {
-        const uint32_t *wn = reinterpret_cast<const uint32_t *>(ne_b + xoff[px]);
-#pragma unroll
-        for (int b = 0; b < (int)kSrBlocks; ++b) {
-            planes_inc<P>(cnt[b], __builtin_amdgcn_alignbit(wn[64 * b + 1], wn[64 * b], px & 31));
-            high[b] = sr_high<P, PH>(cnt[b]);
-        }
-    }
-    if (len <= m && !sr_emit(high, len - 1, lane, hits, hcap, nq)) return true;
-    for (uint32_t px = len; px < m; ++px) {
-        const uint32_t *wn = reinterpret_cast<const uint32_t *>(ne_b + xoff[px]);
-        const uint32_t *wo = reinterpret_cast<const uint32_t *>(ne_b + xoff[px - len]);
-        const uint32_t sho = (px - len) & 31;
-#pragma unroll
-        for (int b = 0; b < (int)kSrBlocks; ++b) {
-            const uint32_t an = __builtin_amdgcn_alignbit(wn[64 * b + 1], wn[64 * b], px & 31);
-            const uint32_t bo = __builtin_amdgcn_alignbit(wo[64 * b + 1], wo[64 * b], sho);
-            planes_slide<P>(cnt[b], an, bo);
-            high[b] = sr_high<P, PH>(cnt[b]);
-        }
-        if (!sr_emit(high, px, lane, hits, hcap, nq)) return true;
-    }
-    return false;
-}
-
-template <int P>
-__global__ __launch_bounds__(64) void struct_range_sweep(SrSweepArgs a)
-{
-    static_assert(kSrBlocks == 4, "the hit emission names four blocks");
-    extern __shared__ uint32_t smem[];
-    const uint32_t lane = threadIdx.x;
-    const uint32_t wl = a.wl;
-    // LDS: NE window (sigma rows of wl words) | x table (u16 byte offsets of a row's window
-    //      words) | one word to broadcast through
-    uint32_t *ne_l = smem;
-    uint16_t *xoff = reinterpret_cast<uint16_t *>(smem + a.sigma * wl);
-    uint32_t *bcast = smem + a.sigma * wl + (a.m_max + 1) / 2;
-    const uint32_t slot_words = kSrHead + a.hcap;
-
-    for (;;) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(a.queue, 1u);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= a.nitems) break;
-        const SrItem I = a.items[a.first + it];
-        const HdRange HR = a.ranges[I.r];
-        const uint32_t m = I.m;
-        uint32_t *slot = a.slots + (size_t)it * slot_words;
-        uint32_t *hits = slot + kSrHead;
-        const int32_t q0 = sr_window_q0((int32_t)HR.ys, m);
-        const uint32_t wbase = (uint32_t)((q0 + (int32_t)a.pad_bits) >> 5);
-        __syncthreads();  // the previous unit is done with LDS
-        for (uint32_t r = lane; r < m; r += 64)
-            xoff[r] = (uint16_t)(4 * (a.codes[I.x_start + r] * wl + (r >> 5)));
-        for (uint32_t c = 0; c < a.sigma; ++c) {
-            const uint32_t *src = a.ne + (size_t)c * a.W + wbase;
-            for (uint32_t w = lane; w < wl; w += 64) ne_l[c * wl + w] = src[w];
-        }
-        __syncthreads();
-
-        uint32_t nq = 0;  // hits so far (wave-uniform)
-        bool overflow = false;
-        const char *ne_b = reinterpret_cast<const char *>(ne_l) + 4 * lane;
-        for (uint32_t d = 0; d < a.ndist && !overflow; ++d) {
-            const uint32_t len = a.dlen[d], err = a.derr[d];
-            // the threshold plane is fixed per shape: one loop nest per (P, PH)
-            switch (min(hd_low_planes(err), (uint32_t)P)) {
-            case 0: overflow = sr_sweep_shape<P, 0>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
-            case 1: overflow = sr_sweep_shape<P, 1>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
-            case 2: overflow = sr_sweep_shape<P, 2>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
-            case 3: overflow = sr_sweep_shape<P, 3>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
-            case 4: overflow = sr_sweep_shape<P, (P > 4 ? 4 : P)>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
-            case 5: overflow = sr_sweep_shape<P, (P > 5 ? 5 : P)>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
-            case 6: overflow = sr_sweep_shape<P, (P > 6 ? 6 : P)>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
-            default: overflow = sr_sweep_shape<P, P>(ne_b, xoff, m, len, err, hits, a.hcap, lane, nq); break;
-            }
-            if (lane == 0) slot[1 + d] = nq;
-        }
-        if (lane == 0) slot[0] = overflow ? 1u : 0u;
-        if (overflow) {
-            // hand the unit's pairs to the per-pair kernel
-            // pair-symmetric units hand over the pairs with x and the texts after it
-            const uint32_t j_first = a.sym && I.xi > HR.j0 ? I.xi : HR.j0;
-            const uint32_t nhand = HR.j0 + HR.ny - j_first;
-            if (lane == 0) {
-                const uint32_t at = atomicAdd(a.ovf_count, nhand);
-                if (at + nhand > a.ovf_cap) atomicOr(a.ovf_count + 1, 1u);
-                bcast[0] = at;
-                // x against itself goes with them: the fold must not credit its diagonal
-                if (I.xi >= HR.j0 && I.xi < HR.j0 + HR.ny) a.self_ovf[I.xi] = 1;
-            }
-            __syncthreads();
-            const uint32_t at = bcast[0];
-            for (uint32_t y = lane; y < nhand; y += 64) {
-                if (at + y >= a.ovf_cap) break;
-                const uint32_t j = j_first + y;
-                const uint32_t ts = a.seq_start[j], n = a.seq_start[j + 1] - ts;
-                // pair-symmetric units: the pair (j, x) goes along
-                a.ovf[at + y] = StructItem{I.x_start, m, ts, n, I.uv_base, a.sym && j != I.xi ? a.uvb[j] : kNoSplit};
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------------------
 // Merge
 // ---------------------------------------------------------------------------------------
 
@@ -651,6 +470,16 @@ hipError_t launch_struct_range_sweep(const SrSweepArgs &a, uint32_t grid, size_t
         const uint32_t p = hd_counter_planes(a.dlen[d], a.derr[d]);
         planes = p > planes ? p : planes;
     }
+    const char *env = getenv("MOTEX_SR_RING");  // "0": the generic sweep for every shape
+    const bool ring_on = !(env && env[0] == '0');
+    // the first shape with a length the ring kernels take sweeps in a register ring
+    const bool ring_fits = ((lds + 15) & ~(size_t)15) + (size_t)kSrQueue * 20 <= 65536;  // with the row queue
+    for (uint32_t d = 0; ring_on && ring_fits && planes <= 4 && d < a.ndist; ++d) {
+        const uint32_t len = a.dlen[d], ph = std::min(hd_low_planes(a.derr[d]), 4u);
+        if (len < kSrRingLenMin || len > kSrRingLenMax || ph > 3) continue;
+        return len <= 7 ? launch_struct_range_sweep_ring_short(a, len, ph, grid, lds, s)
+                        : launch_struct_range_sweep_ring_long(a, len, ph, grid, lds, s);
+    }
     switch (planes <= 3 ? 3 : planes) {
     case 3: hipLaunchKernelGGL((struct_range_sweep<3>), dim3(grid), dim3(64), lds, s, a); break;
     case 4: hipLaunchKernelGGL((struct_range_sweep<4>), dim3(grid), dim3(64), lds, s, a); break;
@@ -679,7 +508,7 @@ hipError_t launch_struct_range_merge(const SrMergeArgs &a, uint32_t units, size_
 // LDS bytes of a sweep workgroup and of a merge workgroup (the layouts at the top of each).
 size_t struct_range_sweep_lds(uint32_t sigma, uint32_t wl, uint32_t m_max)
 {
-    return ((size_t)sigma * wl + (m_max + 1) / 2 + 1) * 4;
+    return ((size_t)sigma * wl + (m_max + 3) / 2 + 1) * 4;
 }
 
 size_t struct_range_merge_lds(uint32_t hcap, uint32_t ndist, uint32_t nbk)

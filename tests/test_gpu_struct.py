@@ -259,3 +259,26 @@ def test_range_plan_on_genes(ctx, oracle, monkeypatch):
     seqs = refdata.hsapiens(60).seqs
     u, v, st = check(ctx, oracle, seqs, 8, 1, [(10, 30, 8, 1)])
     assert st["kernel_variant"] == 1 and v.sum() > 0
+
+
+@pytest.mark.parametrize("ring", ["1", "0"], ids=["ring", "generic"])
+@pytest.mark.parametrize("k,e,box", [
+    (5, 0, (5, 0)),    # threshold plane 0
+    (6, 1, (7, 2)),    # two shapes: the first takes the ring, the second the generic sweep
+    (7, 2, (7, 2)),    # odd length: trips of 14 rows
+    (8, 1, (8, 1)),
+    (8, 4, (5, 3)),    # threshold plane 3 for the first shape
+    (8, 7, (8, 7)),    # nearly every window passes: slots overflow to the per-pair kernel
+    (9, 1, (9, 1)),    # odd length: trips of 18 rows
+    (10, 3, (10, 3)),
+    (11, 2, (11, 2)),  # beyond the ring lengths: the generic sweep either way
+])
+def test_range_ring_lengths(ctx, oracle, monkeypatch, ring, k, e, box):
+    # box lengths 5-10 sweep with the leaving column in a register ring and rows with hits
+    # queued in LDS (MOTEX_SR_RING=0: the generic sweep); some x are shorter than a box, some
+    # shorter than a trip
+    monkeypatch.setenv("MOTEX_STRUCT_RANGE", "1")
+    monkeypatch.setenv("MOTEX_SR_RING", ring)
+    rng = np.random.default_rng(k * 10 + e)
+    seqs = rand_seqs(rng, 20, 3, 400, b"ACGT")
+    check(ctx, oracle, seqs, k, e, [(2, 9, box[0], box[1])], sym="range")
