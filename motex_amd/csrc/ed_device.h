@@ -68,15 +68,16 @@ __host__ __device__ constexpr int ed_width(uint32_t k) { return k <= 14 ? 0 : (k
 // Columns per lane of the one-text-per-lane kernels for each width.
 __host__ __device__ constexpr int ed_cols(int w) { return w == 0 ? 4 : (w == 1 ? 2 : 1); }
 
+// (bound_ctrl: the lane without a source receives zero, so no old value is set up first)
 __device__ __forceinline__ uint32_t dpp_shr1(uint32_t x)
 {
     // wave_shr:1 (lane L receives lane L-1; lane 0 receives 0)
-    return __builtin_amdgcn_update_dpp(0u, x, 0x138, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0u, x, 0x138, 0xF, 0xF, true);
 }
 __device__ __forceinline__ uint32_t dpp_shl1(uint32_t x)
 {
     // wave_shl:1 (lane L receives lane L+1; lane 63 receives 0)
-    return __builtin_amdgcn_update_dpp(0u, x, 0x130, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0u, x, 0x130, 0xF, 0xF, true);
 }
 
 // min*: among a, b, c the word with the fewest set bits, largest on ties; returns its

@@ -33,6 +33,12 @@ __device__ __forceinline__ uint32_t shr1_from(uint32_t in0, uint32_t x)
     return __builtin_amdgcn_update_dpp(in0, x, 0x138, 0xF, 0xF, false);
 }
 
+// wave_shr:1 with lane 0 receiving zero (bound_ctrl: no old value to set up first)
+__device__ __forceinline__ uint32_t shr1_zero(uint32_t x)
+{
+    return __builtin_amdgcn_update_dpp(0u, x, 0x138, 0xF, 0xF, true);
+}
+
 // Past the first strip every column has j > K, where the left and diagonal candidates keep
 // only bits 0 .. K-2 of their source: those SP = K-1 planes are all a cell's right neighbour
 // reads from it, so only they cross lanes and strips.
@@ -133,7 +139,7 @@ __device__ __forceinline__ void slice_left(Slice<K> &L, const uint32_t *ud, cons
     constexpr int W = Lay::W, SP = Lay::SP;
     if constexpr (FIRST) {
 #pragma unroll
-        for (int p = 0; p < W; ++p) L.d[p] = shr1_from(0u, ud[p]);
+        for (int p = 0; p < W; ++p) L.d[p] = shr1_zero(ud[p]);
         slice_popcount<W, slice_pcq<K>()>(L.d, L.pc);
     } else {
 #pragma unroll
