@@ -137,8 +137,9 @@ static hipError_t launch_p(const HdArgs &a, uint32_t grid, size_t lds, hipStream
 size_t hd_lds_bytes(uint32_t sigma, uint32_t wl, uint32_t rows_max, uint32_t sym_group)
 {
     // NE window | v, sequence mask | range starts | row offsets (+2) | u | text-side bitmaps
+    // (pair-symmetric launches) or the range's word-to-text table (ordered launches)
     return (size_t)sigma * wl * 4 + (size_t)rows_max * 8 + (kMaxRangeSeqs + 1) * 4 +
-           ((size_t)rows_max + 2) * 4 + (size_t)rows_max * 4 + (size_t)sym_group * 1024;
+           ((size_t)rows_max + 2) * 4 + (size_t)rows_max * 4 + (sym_group ? (size_t)sym_group * 1024 : kTextTabBytes);
 }
 
 uint32_t hd_lds_words(uint32_t blocks, uint32_t rows_max)

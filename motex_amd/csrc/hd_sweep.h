@@ -53,7 +53,7 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
                                            uint32_t px_start, uint32_t xi, bool sym,
                                            const uint32_t *hq_meta, const uint32_t *hq_res,
                                            uint32_t nq, const uint32_t *starts, uint32_t *acc_v,
-                                           uint32_t *ymask, uint32_t *tb)
+                                           uint32_t *ymask, uint32_t *tb, const uint8_t *ttab)
 {
     __syncthreads();  // the queue stores of every lane are visible to the loads below
     const int32_t ys = (int32_t)R.ys, ye = (int32_t)R.ye, km1 = (int32_t)a.k - 1;
@@ -87,14 +87,21 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
                 if (b == (uint32_t)bb) wb = w[bb];
             const int32_t base = base0 + (int32_t)(kDiagPerBlock * b);
             const int32_t first = base > ys ? base : ys;
+            // the first text to look at: the one holding the word of `first` (from the range's
+            // table in ordered launches; a text at or before first's own, which the walk below
+            // steps over) or a binary search over the starts
             uint32_t y = 0;
             if (R.ny > 1) {
-                uint32_t lo = 0, hi = R.ny - 1;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi + 1) >> 1;
-                    if ((int32_t)starts[mid] <= first) lo = mid; else hi = mid - 1;
+                if (ttab) {
+                    y = ttab[(uint32_t)(first - ys + 32) >> 5];
+                } else {
+                    uint32_t lo = 0, hi = R.ny - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if ((int32_t)starts[mid] <= first) lo = mid; else hi = mid - 1;
+                    }
+                    y = lo;
                 }
-                y = lo;
             }
             for (;;) {
                 int32_t s1 = (int32_t)starts[y + 1];
@@ -223,8 +230,11 @@ void hd_batched_kernel(HdArgs a)
     // u counts distinct texts per row over all of the item's ranges: 32 bits, since an item
     // may span more texts than 16 bits can count (short sequences in large inputs)
     uint32_t *acc_u = xb + a.rows_max + 2;
-    // pair-symmetric launches: 256 words per range of an item (a.sym_group ranges at most)
+    // pair-symmetric launches: 256 words per range of an item (a.sym_group ranges at most);
+    // ordered launches: the range's word-to-text table in their place
     uint32_t *tbits = acc_u + a.rows_max;
+    uint8_t *ttab_l = reinterpret_cast<uint8_t *>(tbits);
+    const uint8_t *ttab = a.sym_group ? nullptr : ttab_l;
     const uint32_t lane4 = lane * 4;
     // this wave's hit queue: tags, then B words per entry (16-byte aligned)
     uint32_t *hq_meta = a.hitq + (size_t)blockIdx.x * a.hitq_cap * (B + 1);
@@ -291,6 +301,20 @@ void hd_batched_kernel(HdArgs a)
                 }
                 if (lane <= R.ny) starts[lane] = a.seq_start[R.j0 + lane];
                 __syncthreads();
+                if (ttab && R.ny > 1) {
+                    // word w of the range's positions starts at ys - 32 + 32 w: the last text
+                    // starting at or before it (the first text for words before the range)
+                    for (uint32_t w = lane; w < kTextTabEntries; w += 64) {
+                        const int32_t p0 = (int32_t)R.ys - 32 + 32 * (int32_t)w;
+                        uint32_t lo = 0, hi = R.ny - 1;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi + 1) >> 1;
+                            if ((int32_t)starts[mid] <= p0) lo = mid; else hi = mid - 1;
+                        }
+                        ttab_l[w] = (uint8_t)lo;
+                    }
+                    __syncthreads();
+                }
 
                 uint32_t nq = 0;  // queued (row, lane) entries, wave-uniform
                 uint32_t cnt[B][P];
@@ -398,7 +422,7 @@ void hd_batched_kernel(HdArgs a)
                                 if constexpr (K == 0) {
                                     if (nq > a.hitq_cap - 64) {
                                         drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq,
-                                                      starts, acc_v, ymask, tb);
+                                                      starts, acc_v, ymask, tb, ttab);
                                         nq = 0;
                                     }
                                 }
@@ -434,7 +458,7 @@ void hd_batched_kernel(HdArgs a)
                     auto trip_drain = [&] {
                         if (nq > a.hitq_cap - 64 * TRIP) {
                             drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v,
-                                          ymask, tb);
+                                          ymask, tb, ttab);
                             nq = 0;
                         }
                     };
@@ -509,7 +533,8 @@ void hd_batched_kernel(HdArgs a)
                     }
                 }
                 if (nq)
-                    drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask, tb);
+                    drain_hits<B>(a, R, q0, px_start, I.xi, I.sym != 0, hq_meta, hq_res, nq, starts, acc_v, ymask, tb,
+                                  ttab);
                 __syncthreads();
                 // distinct sequences of this range per row
                 for (uint32_t r = lane; r < nrows; r += 64) {

@@ -19,6 +19,10 @@ constexpr uint32_t kHitQueueCap = 512;       // Hamming hit queue entries per wa
 // Text ranges per pair-symmetric Hamming item at most: each holds a 1 KB first-occurrence
 // bitmap in LDS.
 constexpr uint32_t kMaxSymGroup = 3;
+// Ordered Hamming launches: per range, the text holding each 32-position word of the range's
+// positions (word w starts at ys - 32 + 32 w; words up to (8192 + 63) / 32), one byte each.
+constexpr uint32_t kTextTabEntries = 258;
+constexpr uint32_t kTextTabBytes = 264;
 constexpr uint32_t kSymBitmapBits = 8192;    // a range's first-occurrence bitmap (256 words)
 constexpr uint32_t kMaxSymItems = 1u << 21;  // pair-symmetric Hamming plans: work items at most
 constexpr uint32_t kSlicePipe = 4;           // waves per item of the pipelined text-parallel edit kernel
