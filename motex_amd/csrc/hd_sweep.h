@@ -57,19 +57,28 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
 {
     __syncthreads();  // the queue stores of every lane are visible to the loads below
     const int32_t ys = (int32_t)R.ys, ye = (int32_t)R.ye, km1 = (int32_t)a.k - 1;
-    for (uint32_t q = threadIdx.x; q < nq; q += 64) {
-        const uint32_t meta = hq_meta[q];
-        uint32_t w[B];
-        // entries hold the high-plane words: a clear bit is a hit
+    // entries hold the high-plane words (a clear bit is a hit); each lane's next entry loads
+    // while it works on the current one
+    auto load_entry = [&](uint32_t q, uint32_t &meta, uint32_t *w) {
+        meta = hq_meta[q];
         if constexpr (B == 4) {
             const uint4 v4 = *reinterpret_cast<const uint4 *>(hq_res + q * 4);
-            w[0] = ~v4.x; w[1] = ~v4.y; w[2] = ~v4.z; w[3] = ~v4.w;
+            w[0] = v4.x; w[1] = v4.y; w[2] = v4.z; w[3] = v4.w;
         } else if constexpr (B == 2) {
             const uint2 v2 = *reinterpret_cast<const uint2 *>(hq_res + q * 2);
-            w[0] = ~v2.x; w[1] = ~v2.y;
+            w[0] = v2.x; w[1] = v2.y;
         } else {
-            w[0] = ~hq_res[q];
+            w[0] = hq_res[q];
         }
+    };
+    uint32_t meta_n = 0, w_n[B];
+    if (threadIdx.x < nq) load_entry(threadIdx.x, meta_n, w_n);
+    for (uint32_t q = threadIdx.x; q < nq; q += 64) {
+        const uint32_t meta = meta_n;
+        uint32_t w[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) w[b] = ~w_n[b];
+        if (q + 64 < nq) load_entry(q + 64, meta_n, w_n);
         const uint32_t r = meta & 0xFFFFu;
         const int32_t base0 = q0 + (int32_t)(32 * (meta >> 16) + px_start + r);
         uint32_t hits = 0, ym = 0;
