@@ -424,8 +424,17 @@ void hd_batched_kernel(HdArgs a)
                                     } else {
                                         *reinterpret_cast<uint32_t *>(res_b) = high[0];
                                     }
+                                    // (the trips of odd k from 13, 2 k rows long, leave no register
+                                    // for the lane number: it is recomputed here, where the compiler
+                                    // can neither hoist it nor spill it; k = 13, e = 3 on 1200 x
+                                    // 1000 nt: 17.2 -> 14.8 ms.  For k = 9 and 11 the register is
+                                    // there.)
+                                    uint32_t tag_lane = lane;
+                                    if constexpr (K % 2 == 1 && K >= 13)
+                                        asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
+                                                     : "=v"(tag_lane));
                                     *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hq_meta) + slot * 4u) =
-                                        r | (lane << 16);
+                                        r | (tag_lane << 16);
                                 }
                                 nq += (uint32_t)__builtin_popcountll(m);
                                 if constexpr (K == 0) {
