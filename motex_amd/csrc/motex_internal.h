@@ -25,6 +25,12 @@ constexpr uint32_t kTextTabEntries = 258;
 constexpr uint32_t kTextTabBytes = 264;
 constexpr uint32_t kSymBitmapBits = 8192;    // a range's first-occurrence bitmap (256 words)
 constexpr uint32_t kMaxSymItems = 1u << 21;  // pair-symmetric Hamming plans: work items at most
+// Pair-symmetric Hamming plans below this occurrence rate per cell: each occurrence in a later
+// text also credits the text side (first ones through the range's bitmap and list, repeats by a
+// global atomic), which beyond it costs more than the halved sweep saves (1200 x 1000 nt DNA:
+// k=8 e=1 at 4e-4 10.5 ms against 17.2 ordered, k=9 e=2 at 1.6e-3 20.0 against 24.1; k=12
+// e=4 at 3.4e-3 53.9 against 37.8, k=8 e=2 at 4.2e-3 95 against 50)
+constexpr double kSymMaxRate = 2e-3;
 constexpr uint32_t kSlicePipe = 4;           // waves per item of the pipelined text-parallel edit kernel
 // Edit kernel choice, from sweeps of 40-500 protein and DNA sequences of U[100,2000], k 8-14:
 // below kSliceItemsPerCu (x, group of 32 texts) items per CU the one-text-per-lane kernel

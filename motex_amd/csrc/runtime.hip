@@ -462,26 +462,29 @@ static bool hd_symmetric(motex_batch *b, const HdPlan &plan)
     // many short texts (hundreds of ranges per x) would make the work queue huge
     if ((double)ss.num * plan.ranges.size() / (2.0 * kMaxSymGroup) > (double)kMaxSymItems) return false;
     if (const char *env = getenv("MOTEX_SYM")) return env[0] == '1';
-    return hd_cell_hit_rate(ss, b->params.k, b->params.e) < 1e-4;
+    return hd_cell_hit_rate(ss, b->params.k, b->params.e) < kSymMaxRate;
 }
 
-// Capacity of every range's first-occurrence list (pair-symmetric runs): three times the
-// expected first occurrences of the x sequences before its last text (rows x span x the
-// cell rate), plus one full bitmap; a list that fills up takes the rest as atomics.
+// Capacity of every range's first-occurrence list (pair-symmetric runs), plus one full
+// bitmap; a list that fills up takes the rest as atomics.  x with r rows reaches a text
+// position first with chance 1 - (1 - p)^r <= p r for the cell rate p, and at most once, so a
+// range of span s gets s x the sum of min(1, 3 p r) over the x sequences before its last text:
+// three times the expectation, but never more than one entry per position and x.
 static std::vector<uint32_t> hd_sym_list_offsets(const SeqSet &ss, const HdPlan &plan, uint32_t k, uint32_t e)
 {
     const double p = hd_cell_hit_rate(ss, k, e);
     // MOTEX_SYM_LIST_CAP=n: n entries per range (tests of the lists' atomic fallback)
     uint64_t forced = 0;
     if (const char *env = getenv("MOTEX_SYM_LIST_CAP")) forced = strtoull(env, nullptr, 10);
-    std::vector<uint64_t> rows_upto(ss.num + 1, 0);  // rows of sequences 0 .. j-1
-    for (uint32_t j = 0; j < ss.num; ++j) rows_upto[j + 1] = rows_upto[j] + (ss.len[j] >= k ? ss.len[j] - k + 1 : 0);
+    std::vector<double> reach_upto(ss.num + 1, 0);  // sum of min(1, p r) over sequences 0 .. j-1
+    for (uint32_t j = 0; j < ss.num; ++j)
+        reach_upto[j + 1] = reach_upto[j] + std::min(1.0, 3.0 * p * (double)(ss.len[j] >= k ? ss.len[j] - k + 1 : 0));
     std::vector<uint32_t> off(plan.ranges.size() + 1, 0);
     uint64_t total = 0;
     for (size_t r = 0; r < plan.ranges.size(); ++r) {
         const HdRange &R = plan.ranges[r];
-        const double rows = (double)rows_upto[R.j0 + R.ny - 1];
-        const double cap = forced ? (double)forced : 3.0 * p * rows * (R.ye - R.ys) + kSymBitmapBits;
+        const double reach = reach_upto[R.j0 + R.ny - 1];
+        const double cap = forced ? (double)forced : reach * (R.ye - R.ys) + kSymBitmapBits;
         total += ((uint64_t)std::min(cap, (double)(1u << 28)) + 7) & ~7ull;  // lists start at multiples of 8
         off[r + 1] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFF8u);
     }

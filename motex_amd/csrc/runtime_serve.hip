@@ -74,9 +74,11 @@ static size_t footprint(const Job &j, const motex_ctx *c)
         bytes += 32 * (T + kPadBits + 16384) / 8;                // NE tables (up to 32 symbols)
         bytes += (T / 256 + 1) * (T / 7000 + 1) * sizeof(HdItem); // work items (upper bound)
         bytes += grid * 2 * hd_hitq_cap(j.params.k, j.params.e, 4) * 5 * 4;  // hit queues (uncapped grid)
-        // pair-symmetric text-side lists: 3 x (occurrence rate < 1e-4) x cells/2, 2 B each,
-        // and a bitmap's worth per range
-        bytes += (uint64_t)(3e-4 * (double)T * (double)T) + 4 * T;
+        // pair-symmetric text-side lists: per range, its positions x min(1, 3 x rate x rows)
+        // summed over the sequences before it (rate < kSymMaxRate), 2 B each, and a bitmap's
+        // worth per range
+        bytes += (uint64_t)(2.0 * (double)T * std::min<double>((double)j.lens.size(), 3.0 * kSymMaxRate * (double)T)) +
+                 4 * T;
     } else {
         // boundary streams (text-parallel kernel: up to 32 words per row, grid of 12 waves/CU)
         bytes += std::max<uint64_t>(grid * 2 * (maxlen + 128) * 8, (uint64_t)c->num_cu * 12 * 2 * maxlen * 32 * 4);
