@@ -26,7 +26,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 # Per-source device flags.  The Hamming kernels add each row's hit count to a per-row LDS word
 # from the lanes that hit (a handful); the atomic optimizer would turn that uniform-address add
 # into a scalar loop over the active lanes on every row.
-HD_RING_SOURCES = [f"hd_ring_kernels_k{k}.hip" for k in (8, 10, 12, 14, 16)]
+HD_RING_SOURCES = [f"hd_ring_kernels_k{k}.hip" for k in (8, 10, 12, 14)]
 SOURCE_FLAGS = {s: ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"] for s in ["hd_kernels.hip", *HD_RING_SOURCES]}
 LIB_SOURCES = ["hd_kernels.hip", *HD_RING_SOURCES, "ed_kernels.hip", "ed_slice_kernels.hip", "ed_slice_kernels_k15.hip",
                "ed_slice_kernels_k23.hip", "struct_kernels.hip", "struct_range_kernels.hip", "struct_range_ring_short.hip",

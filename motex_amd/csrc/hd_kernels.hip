@@ -186,7 +186,6 @@ hipError_t launch_hd(const HdArgs &a, uint32_t blocks, uint32_t grid, hipStream_
             case 10: case 11: return launch_hd_ring_k10(a, grid, lds, s);
             case 12: case 13: return launch_hd_ring_k12(a, grid, lds, s);
             case 14: case 15: return launch_hd_ring_k14(a, grid, lds, s);
-            case 16: return launch_hd_ring_k16(a, grid, lds, s);
             default: break;
             }
         }
