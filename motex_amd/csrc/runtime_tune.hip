@@ -40,14 +40,18 @@ static TileShape default_tile(const motex_params &p, const SeqSet &ss)
         t.blocks = ss.sigma <= 8 ? 4 : (ss.sigma <= 24 ? 2 : 1);
         t.rows = 128;
         t.waves_per_cu = 32;
-        if (t.blocks == 4 && hd_has_ring(p.k, p.e)) {
-            // the register-ring kernels hold 4 waves per SIMD (LDS and registers), and their
+        if (ss.sigma <= 24 && hd_has_ring(p.k, p.e)) {
+            // the register-ring kernels take four blocks up to 24 symbols too, though the NE
+            // window then holds a CU to 5-6 waves (500 proteins, k=8 e=1: 3.44 ms against
+            // 4.54 with the generic two-block kernels at 32 waves per CU; k=12 e=3: 3.38
+            // against 4.38).  They hold 4 waves per SIMD (LDS and registers), and their
             // tiles take 144 rows: 1000-nt texts still pack 8 to a range, and the k - 1 warm-up
             // rows are a smaller share (1200 x 1000 nt: 6.40 ms at 128 rows, 6.31 at 144,
             // 6.36 at 160; H. sapiens genes 6.98, 6.83, 6.84).  Texts longer than a window are
             // cut into blocks of diagonals that do not depend on the row tile, so when every
             // text is, longer tiles amortise the warm-up rows further (one 4-Mnt sequence:
             // 168 ms at 128 rows, 162.5 at 256; 1000-nt texts lose a text per range above 160)
+            t.blocks = 4;
             t.waves_per_cu = 16;
             t.rows = 144;
             bool all_long = true;
