@@ -38,7 +38,10 @@ static TileShape default_tile(const motex_params &p, const SeqSet &ss)
     TileShape t;
     if (!p.nb_gaps && p.distance == 0) {
         t.blocks = ss.sigma <= 8 ? 4 : (ss.sigma <= 24 ? 2 : 1);
-        t.rows = 128;
+        // (144 rows with four blocks: the warm-up rows a smaller share, 1000-nt texts still 8
+        // to a range; 1200 x 1000 nt, generic kernels: k=16 e=2 8.35 -> 8.26 ms, k=6 e=0
+        // 10.16 -> 9.90, k=20 e=4 8.13 -> 7.92)
+        t.rows = t.blocks == 4 ? 144 : 128;
         t.waves_per_cu = 32;
         if (ss.sigma <= 24 && hd_has_ring(p.k, p.e)) {
             // the register-ring kernels take four blocks up to 24 symbols too, though the NE
