@@ -80,13 +80,16 @@ __device__ __forceinline__ void sr_scan(const SrUnit &U, uint32_t d, int32_t lo,
     }
 }
 
-// Text of the range holding position r (starts[0 .. ny] are the range's text starts).
+// Text of the range holding position r (starts[0 .. ny] are the range's text starts, ny <= 32,
+// starts[0] <= r < starts[ny]): the last start at or before r, by five fixed halving steps
+// (no loop control; an index past the last text reads the end, which is above r).
 __device__ __forceinline__ uint32_t sr_text_of(const int32_t *starts, uint32_t ny, int32_t r)
 {
-    uint32_t lo = 0, hi = ny - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (starts[mid] <= r) lo = mid; else hi = mid - 1;
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t step = 16; step; step >>= 1) {
+        const uint32_t idx = min(lo + step, ny);
+        if (starts[idx] <= r) lo = idx;
     }
     return lo;
 }
