@@ -104,12 +104,13 @@ __device__ __forceinline__ void drain_hits(const HdArgs &a, const HdRange &R, in
                 if (ttab) {
                     y = ttab[(uint32_t)(first - ys + 32) >> 5];
                 } else {
-                    uint32_t lo = 0, hi = R.ny - 1;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi + 1) >> 1;
-                        if ((int32_t)starts[mid] <= first) lo = mid; else hi = mid - 1;
+                    // the last text starting at or before first: fixed halving steps over
+                    // ny <= 32 texts (first may lie past the range's end: the last text then)
+#pragma unroll
+                    for (uint32_t step = 16; step; step >>= 1) {
+                        const uint32_t idx = min(y + step, R.ny - 1);
+                        if ((int32_t)starts[idx] <= first) y = idx;
                     }
-                    y = lo;
                 }
             }
             for (;;) {
