@@ -82,6 +82,10 @@ def test_symmetric_is_the_default_for_rare_occurrences(ctx):
     (10, 65, 20, b"AC", 5, 60, 300),       # beyond 64: generic single-block kernel
     (11, 200, 70, b"AC", 4, 150, 600),     # eleven-plane counters, long warm-up rows
     (12, 1024, 400, b"AC", 3, 1000, 1600),  # longest factor
+    (13, 16, 2, b"ACGT", 12, 200, 900),     # k = 16: generic kernels (five counter planes)
+    (14, 13, 3, b"ACGT", 12, 200, 900),     # odd k from 13: trips of 26 rows
+    (15, 15, 1, b"ARNDCQEGHILKMFPSTWYV", 10, 100, 900),  # ring kernels on 20 symbols
+    (16, 8, 1, b"ACGT", 30, 100, 700),      # occurrence rate 4e-4: pair-symmetric by default
 ])
 def test_random_edges(ctx, oracle, sym, seed, k, e, alpha, n, lo, hi):
     rng = np.random.default_rng(seed)
